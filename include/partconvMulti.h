@@ -1,0 +1,158 @@
+/*
+ * partconvMulti.h -- C++ drop-in for the reference's PartConvMulti,
+ * PartConvMultiRelaxed (src/partconvMulti.h:52-148) and PartConvFilter
+ * (src/partconv.h:40-114), header-only over the C ABI of partconv_amd.h.
+ *
+ * Same class names, method names, argument meaning and public fields.  Where
+ * the reference calls exit(-1) (bad config, unreadable IR, frameCount !=
+ * buffer_size), these methods return the nonzero status instead and
+ * pc_last_error() has the message.
+ *
+ *   #include "partconvMulti.h"
+ *   PartConvMulti pc(1);
+ *   pc.setup("pc.config");
+ *   ... in the audio callback: pc.run(out, in, frameCount);
+ *   pc.cleanup();
+ *
+ * Link with -lpartconv_amd (libpartconv_amd.so, built by partconv_amd/build.py).
+ */
+#ifndef PARTCONV_AMD_PARTCONVMULTI_H
+#define PARTCONV_AMD_PARTCONVMULTI_H
+
+#include <stdio.h>
+
+#include "partconv_amd.h"
+
+class PartConvMulti
+{
+public:
+    PartConvMulti(int inVerbosity = 0) : verbosity(inVerbosity), h_(pcm_create(inVerbosity, 0)) { clear(); }
+    virtual ~PartConvMulti() { pcm_destroy(h_); }
+
+    int setup(const char *config_file_name, int max_threads_per_level = 1, int max_level0_threads = 0,
+              double min_work = 0)
+    {
+        int rc = pcm_setup(h_, config_file_name, max_threads_per_level, max_level0_threads, min_work);
+        if (rc) fprintf(stderr, "%s\n", pc_last_error());
+        refresh();
+        return rc;
+    }
+    int cleanup() { return pcm_cleanup(h_); }
+    virtual int run(float *const output, const float *const input, const int frameCount)
+    {
+        int rc = pcm_run(h_, output, input, frameCount);
+        if (rc) fprintf(stderr, "%s\n", pc_last_error());
+        frameNum++;
+        return rc;
+    }
+    void reset()
+    {
+        pcm_reset(h_);
+        frameNum = 0;
+    }
+    /* engine layout: PC_BATCHED (default) or PC_PLAIN, before setup */
+    int set_mode(int mode) { return pcm_set_mode(h_, mode); }
+    pc_multi *handle() { return h_; }
+
+    float *output_mix = nullptr; /* kept for source compatibility; mixing happens on the GPU */
+    float *input_mix = nullptr;
+    int FS;
+    int buffer_size;
+    int max_block_size;
+    int verbosity;
+    int numInputChannels;
+    int numOutputChannels;
+    unsigned long frameNum;
+    void *doneWaiter = nullptr;
+    unsigned long lastFrame = 0;
+
+protected:
+    PartConvMulti(int inVerbosity, int relaxed) : verbosity(inVerbosity), h_(pcm_create(inVerbosity, relaxed))
+    {
+        clear();
+    }
+    void clear()
+    {
+        FS = buffer_size = max_block_size = numInputChannels = numOutputChannels = 0;
+        frameNum = 0;
+    }
+    void refresh()
+    {
+        pc_multi_info i;
+        if (pcm_get_info(h_, &i) == PC_OK) {
+            FS = i.FS;
+            buffer_size = i.buffer_size;
+            max_block_size = i.max_block_size;
+            numInputChannels = i.numInputChannels;
+            numOutputChannels = i.numOutputChannels;
+        }
+    }
+    pc_multi *h_;
+};
+
+class PartConvMultiRelaxed : public PartConvMulti
+{
+public:
+    PartConvMultiRelaxed(int inVerbosity = 0) : PartConvMulti(inVerbosity, 1) {}
+};
+
+class PartConvFilter
+{
+public:
+    PartConvFilter(int inVerbosity = 0, int inSampleRate = 44100)
+        : M(0), N(nullptr), num_parts(nullptr), FS(inSampleRate), verbosity(inVerbosity),
+          h_(pcf_create(inVerbosity, inSampleRate))
+    {
+    }
+    ~PartConvFilter()
+    {
+        pcf_destroy(h_);
+        delete[] N;
+        delete[] num_parts;
+    }
+    /* impulse: IR samples (impulse_len of them), or nullptr for an all-ones IR */
+    int setup(int M_in, int *N_in, int *num_parts_in, const float *impulse = nullptr, int impulse_len = 0,
+              int fftwOptLevel = 1, const char *wisdom = nullptr)
+    {
+        pcf_set_fs(h_, FS);
+        int rc = pcf_setup(h_, M_in, N_in, num_parts_in, impulse, impulse_len, fftwOptLevel, wisdom);
+        if (rc) {
+            fprintf(stderr, "%s\n", pc_last_error());
+            return 1;
+        }
+        delete[] N;
+        delete[] num_parts;
+        M = M_in;
+        N = new int[M];
+        num_parts = new int[M];
+        for (int i = 0; i < M; i++) {
+            N[i] = N_in[i];
+            num_parts[i] = num_parts_in[i];
+        }
+        return 0;
+    }
+    double bench(const int L, const int trials = 100, const int pollute_every = 0, const float max_time = 0.1,
+                 double *benchData = nullptr)
+    {
+        return pcf_bench(h_, L, trials, pollute_every, max_time, benchData);
+    }
+    void reset() { pcf_reset(h_); }
+    void sync_levels() { pcf_sync_levels(h_); }
+    void sync_levels_relaxed() { pcf_sync_levels_relaxed(h_); }
+    void input(const float *const input) { pcf_input(h_, input); }
+    void output(float *const output) { pcf_output(h_, output); }
+    void output_relaxed(float *const output) { pcf_output_relaxed(h_, output); }
+    int runLevel(int L) { return pcf_run_level(h_, L); }
+    int cleanup() { return pcf_cleanup(h_); }
+
+    int M;
+    int *N;
+    int *num_parts;
+    int FS;
+    int verbosity;
+
+private:
+    pc_filter *h_;
+};
+
+#endif /* PARTCONV_AMD_PARTCONVMULTI_H */

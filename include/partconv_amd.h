@@ -1,0 +1,223 @@
+/*
+ * partconv_amd.h -- C ABI of the MI355X (gfx950) partitioned-convolution engine.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (PC_OK == 0) unless stated, and never exits the process: where the reference
+ * calls exit(-1) (config/IR/frameCount errors) these return an error code and
+ * pc_last_error() holds the message.
+ *
+ * Two handle types mirror the reference's two public classes:
+ *   pcm_*  PartConvMulti / PartConvMultiRelaxed   (src/partconvMulti.h:52-148)
+ *   pcf_*  PartConvFilter                         (src/partconv.h:40-114)
+ * The C++ classes in partconv.h / partconvMulti.h and the Python module
+ * partconv_amd are thin layers over these functions.
+ *
+ * Threading: a handle is driven by one host thread at a time (as the reference's
+ * run() is called serially from one audio callback).  All GPU work is internal
+ * to the engine's HIP streams.
+ */
+#ifndef PARTCONV_AMD_H
+#define PARTCONV_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pc_status {
+    PC_OK = 0,
+    PC_ERR_ARG = 1,        /* bad argument / null handle */
+    PC_ERR_CONFIG = 2,     /* config file missing or malformed */
+    PC_ERR_PARTITION = 3,  /* partition rejected by the reference's rules */
+    PC_ERR_IO = 4,         /* impulse-response file unreadable */
+    PC_ERR_HIP = 5,        /* HIP runtime error (message has the HIP string) */
+    PC_ERR_STATE = 6,      /* call out of order (run before setup, ...) */
+    PC_ERR_FRAMECOUNT = 7, /* frameCount != buffer_size */
+    PC_ERR_TIMEOUT = 8     /* a tick did not complete within the engine's bound */
+};
+
+/* Engine layouts for pcm_set_mode(): how instances share kernel launches. */
+enum pc_mode {
+    PC_BATCHED = 0, /* all instances of a level in one launch per level tick (default) */
+    PC_PLAIN = 1    /* one launch per instance per level tick: the per-instance
+                       runLevel granularity of the reference's work units */
+};
+
+/* Fields of PartConvMulti that the reference exposes publicly
+ * (src/partconvMulti.h:64-79). */
+typedef struct pc_multi_info {
+    int FS;
+    int buffer_size;
+    int max_block_size;
+    int verbosity;
+    int numInputChannels;
+    int numOutputChannels;
+    int numPCs;           /* instances after repeats */
+    int num_levels;       /* distinct block sizes (global levels) */
+    unsigned long long frameNum;
+} pc_multi_info;
+
+/* Per-tick timing collected by the engine (host wall clock, submit -> output
+ * visible).  Latencies in microseconds. */
+typedef struct pc_stats {
+    unsigned long long ticks;
+    unsigned long long deadline_misses; /* ticks whose latency exceeded the block period */
+    double p50_us, p99_us, max_us, mean_us;
+    double period_us;                   /* buffer_size / FS */
+} pc_stats;
+
+/* One instance of an in-memory setup: the fields of one config line
+ * (src/partconvMulti.cpp:73-114) with the IR passed by pointer. */
+typedef struct pc_instance {
+    int mute;
+    int in_ch;
+    int out_ch;
+    float scale;
+    int M;             /* number of levels */
+    const int *N;      /* M block sizes, N[0] == buffer_size */
+    const int *P;      /* M counts; the last one is recounted from the IR length */
+    const float *ir;   /* impulse response */
+    int ir_len;
+} pc_instance;
+
+typedef struct pc_multi pc_multi;
+typedef struct pc_filter pc_filter;
+
+/* Last error message of the calling thread ("" if none). */
+const char *pc_last_error(void);
+
+/* Library / device information: writes the HIP device name into buf. */
+int pc_device_name(char *buf, int len);
+
+/* ----------------------------------------------------------------------------
+ * PartConvMulti
+ * ------------------------------------------------------------------------- */
+
+/* PartConvMulti(int verbosity) / PartConvMultiRelaxed(int verbosity)
+ * (src/partconvMulti.h:57, :145).  relaxed != 0 selects the relaxed schedule:
+ * output delayed by buffer_size samples, every level gets a full period. */
+pc_multi *pcm_create(int verbosity, int relaxed);
+
+/* Engine layout (PC_BATCHED / PC_PLAIN); call before setup.  No reference
+ * counterpart: the reference has one layout (per-instance work units). */
+int pcm_set_mode(pc_multi *h, int mode);
+
+/* PartConvMulti::setup(config_file, max_threads_per_level, max_level0_threads,
+ * min_work) (src/partconvMulti.cpp:23-718).  Reads the pc.config grammar and the
+ * RIFF/WAVE IR files it names (first channel).  The three thread arguments are
+ * accepted for API compatibility; work is distributed over HIP streams. */
+int pcm_setup(pc_multi *h, const char *config_file, int max_threads_per_level,
+              int max_level0_threads, double min_work);
+
+/* In-memory setup: same semantics as a config file with FS, buffer_size, nIn,
+ * nOut and numPCs instances (numRepeats = 1), IRs passed by pointer (the
+ * reference's in-memory front end is PartConvMax::setup,
+ * src/partconv.cpp:653-847). */
+int pcm_setup_mem(pc_multi *h, int FS, int buffer_size, int nIn, int nOut, int numPCs,
+                  const pc_instance *inst);
+
+/* PartConvMulti::run (src/partconvMulti.cpp:815-928) /
+ * PartConvMultiRelaxed::run (src/partconvMulti.cpp:1017-1134): one tick.
+ * input: frameCount x numInputChannels interleaved, output: frameCount x
+ * numOutputChannels interleaved (host memory).  Blocks until output is ready. */
+int pcm_run(pc_multi *h, float *output, const float *input, int frameCount);
+
+/* Device-resident ticks: runs nframes consecutive ticks reading frame t from
+ * d_in + t*buffer_size*nIn and writing d_out + t*buffer_size*nOut (device
+ * pointers, interleaved).  sync_each_tick != 0 waits for every tick's output
+ * (real-time semantics, per-tick latency recorded); 0 queues all ticks and
+ * waits once at the end.  No reference counterpart (the reference only has the
+ * host callback); the tick semantics are those of pcm_run. */
+int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long long nframes,
+                   int sync_each_tick);
+
+/* PartConvMulti::reset (src/partconvMulti.cpp:806-813).  Unlike the
+ * reference, also rewinds the global level counters and frameNum. */
+int pcm_reset(pc_multi *h);
+
+/* PartConvMulti::cleanup (src/partconvMulti.cpp:720-804).  Idempotent. */
+int pcm_cleanup(pc_multi *h);
+
+/* Free the handle (runs cleanup if needed). */
+void pcm_destroy(pc_multi *h);
+
+int pcm_get_info(const pc_multi *h, pc_multi_info *info);
+int pcm_get_stats(const pc_multi *h, pc_stats *st);
+int pcm_clear_stats(pc_multi *h);
+/* Copies up to cap per-tick latencies (us) of the most recent ticks; returns the
+ * number written (or -1). */
+long long pcm_get_latencies(const pc_multi *h, double *out_us, long long cap);
+/* Instance partition after setup (last count recounted): fills N, P (cap
+ * entries each); returns M or -1. */
+int pcm_instance_partition(const pc_multi *h, int inst, int *N, int *P, int cap);
+
+/* Kernel probe (instrumentation, no reference counterpart; the reference's
+ * nearest is the per-stage Timer of its older engine,
+ * src/partconvMulti_cuda.cpp:1679-1818): from now on, time with HIP events the
+ * multiply-accumulate kernel of global level `global_level` (0 = the tick
+ * kernel, which carries level 0) each time it launches, on the stream it runs
+ * on.  -1 disables. */
+int pcm_probe(pc_multi *h, int global_level);
+/* Average probed kernel duration (us), number of launches timed, and the
+ * algorithmic HBM bytes one launch moves: per job 8(N+1)(2P+1) (filter
+ * spectra + delay-line rows + new slot), plus 12*N0 per instance for the tick
+ * kernel (window read, output write). */
+int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, double *bytes_per_launch);
+
+/* ----------------------------------------------------------------------------
+ * PartConvFilter (single instance, caller-driven levels)
+ * ------------------------------------------------------------------------- */
+
+/* PartConvFilter(int verbosity, int sampleRate) (src/partconv.h:46). */
+pc_filter *pcf_create(int verbosity, int sample_rate);
+
+/* PartConvFilter::setup(M, N, num_parts, impulse, fftwOptLevel, wisdom)
+ * (src/partconv.cpp:70-306).  impulse == NULL gives an all-ones IR of sum N*P
+ * taps (src/partconv.cpp:132-142).  num_parts is updated in place with the
+ * recounted last level, as the reference's member array is.  fftwOptLevel and
+ * wisdom are accepted and ignored (no planner). */
+int pcf_setup(pc_filter *h, int M, const int *N, int *num_parts, const float *impulse,
+              int impulse_len, int fftwOptLevel, const char *wisdom);
+
+/* PartConvFilter::runLevel(L) (src/partconv.cpp:354-490): enqueues level L's
+ * next block on the level's stream and returns. */
+int pcf_run_level(pc_filter *h, int L);
+
+/* PartConvFilter::sync_levels / sync_levels_relaxed (src/partconv.cpp:568-603). */
+int pcf_sync_levels(pc_filter *h);
+int pcf_sync_levels_relaxed(pc_filter *h);
+int pcf_start_sync_depth(const pc_filter *h);
+int pcf_end_sync_depth(const pc_filter *h);
+
+/* PartConvFilter::input (src/partconv.cpp:605-613): N[0] samples (host). */
+int pcf_input(pc_filter *h, const float *input);
+
+/* PartConvFilter::output / output_relaxed (src/partconv.cpp:616-648): N[0]
+ * samples (host).  Waits for the levels whose blocks are read. */
+int pcf_output(pc_filter *h, float *output);
+int pcf_output_relaxed(pc_filter *h, float *output);
+
+/* PartConvFilter::reset (src/partconv.cpp:492-506). */
+int pcf_reset(pc_filter *h);
+
+/* PartConvFilter::bench(L, trials, pollute_every, max_time, benchData)
+ * (src/partconv.cpp:508-566): mean seconds of one level-L block on the GPU
+ * (HIP events), stopping after max_time seconds; benchData[2L], [2L+1] get the
+ * percentage of the level's block period.  Resets the filter afterwards.
+ * pollute_every is accepted and ignored.  Returns < 0 on error. */
+double pcf_bench(pc_filter *h, int L, int trials, int pollute_every, float max_time,
+                 double *benchData);
+
+/* PartConvFilter::cleanup (src/partconv.cpp:308-352).  Idempotent. */
+int pcf_cleanup(pc_filter *h);
+void pcf_destroy(pc_filter *h);
+
+/* Public members M, N[], num_parts[], FS, verbosity (src/partconv.h:66-71). */
+int pcf_get_levels(const pc_filter *h, int *M, int *N, int *num_parts, int cap);
+int pcf_get_fs(const pc_filter *h);
+int pcf_set_fs(pc_filter *h, int fs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARTCONV_AMD_H */

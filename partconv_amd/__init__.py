@@ -1,0 +1,10 @@
+"""partconv_amd: multi-level non-uniform partitioned convolution on AMD MI355X.
+
+Python surface over libpartconv_amd.so (C ABI in include/partconv_amd.h).
+"""
+from ._lib import last_error, lib  # noqa: F401
+from .partconv import (PartConvFilter, PartConvMulti, PartConvMultiRelaxed,  # noqa: F401
+                       doubleArray, intArray)
+
+__all__ = ["PartConvFilter", "PartConvMulti", "PartConvMultiRelaxed", "intArray", "doubleArray",
+           "last_error", "lib"]

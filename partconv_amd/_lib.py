@@ -1,0 +1,103 @@
+"""Loader for libpartconv_amd.so (the C ABI declared in include/partconv_amd.h).
+
+The library is built in-tree by partconv_amd.build; nothing here falls back to
+a CPU implementation: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libpartconv_amd.so"
+
+PC_OK = 0
+PC_BATCHED = 0
+PC_PLAIN = 1
+
+_lib = None
+
+
+class PcMultiInfo(ctypes.Structure):
+    _fields_ = [("FS", ctypes.c_int), ("buffer_size", ctypes.c_int), ("max_block_size", ctypes.c_int),
+                ("verbosity", ctypes.c_int), ("numInputChannels", ctypes.c_int),
+                ("numOutputChannels", ctypes.c_int), ("numPCs", ctypes.c_int),
+                ("num_levels", ctypes.c_int), ("frameNum", ctypes.c_ulonglong)]
+
+
+class PcStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_ulonglong), ("deadline_misses", ctypes.c_ulonglong),
+                ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double),
+                ("mean_us", ctypes.c_double), ("period_us", ctypes.c_double)]
+
+
+class PcInstance(ctypes.Structure):
+    _fields_ = [("mute", ctypes.c_int), ("in_ch", ctypes.c_int), ("out_ch", ctypes.c_int),
+                ("scale", ctypes.c_float), ("M", ctypes.c_int), ("N", ctypes.POINTER(ctypes.c_int)),
+                ("P", ctypes.POINTER(ctypes.c_int)), ("ir", ctypes.POINTER(ctypes.c_float)),
+                ("ir_len", ctypes.c_int)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/partconv_amd.h
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "pc_last_error": (ctypes.c_char_p, []),
+    "pc_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "pcm_create": (_vp, [ctypes.c_int, ctypes.c_int]),
+    "pcm_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "pcm_setup_mem": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(PcInstance)]),
+    "pcm_run": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "pcm_run_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int]),
+    "pcm_reset": (ctypes.c_int, [_vp]),
+    "pcm_cleanup": (ctypes.c_int, [_vp]),
+    "pcm_destroy": (None, [_vp]),
+    "pcm_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(PcMultiInfo)]),
+    "pcm_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
+    "pcm_clear_stats": (ctypes.c_int, [_vp]),
+    "pcm_get_latencies": (ctypes.c_longlong, [_vp, _dp, ctypes.c_longlong]),
+    "pcm_instance_partition": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int]),
+    "pcm_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
+    "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),
+    "pcf_setup": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _fp, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_char_p]),
+    "pcf_run_level": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcf_sync_levels": (ctypes.c_int, [_vp]),
+    "pcf_sync_levels_relaxed": (ctypes.c_int, [_vp]),
+    "pcf_start_sync_depth": (ctypes.c_int, [_vp]),
+    "pcf_end_sync_depth": (ctypes.c_int, [_vp]),
+    "pcf_input": (ctypes.c_int, [_vp, _fp]),
+    "pcf_output": (ctypes.c_int, [_vp, _fp]),
+    "pcf_output_relaxed": (ctypes.c_int, [_vp, _fp]),
+    "pcf_reset": (ctypes.c_int, [_vp]),
+    "pcf_bench": (ctypes.c_double, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _dp]),
+    "pcf_cleanup": (ctypes.c_int, [_vp]),
+    "pcf_destroy": (None, [_vp]),
+    "pcf_get_levels": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.c_int]),
+    "pcf_get_fs": (ctypes.c_int, [_vp]),
+    "pcf_set_fs": (ctypes.c_int, [_vp, ctypes.c_int]),
+}
+
+
+def lib():
+    """Load the HIP library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m partconv_amd.build`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().pc_last_error().decode(errors="replace")

@@ -1,0 +1,1422 @@
+// pc_engine.hip -- host engine and C ABI of the MI355X partitioned-convolution
+// library (libpartconv_amd.so).
+//
+// What the reference does with FFTW plans, pthread level workers and condvars
+// (src/partconv.cpp:70-648, src/partconvMulti.cpp:23-1134) is done here with:
+//   * the kernels of pc_kernels.hpp (level jobs batched across instances);
+//   * one HIP stream per partition level (levels >= 1; at most PC_LEVEL_STREAMS
+//     streams, levels beyond that share round-robin) plus the tick stream;
+//   * events for the deadlines: a level's block started at tick f_s must be
+//     complete before the tick kernel that first reads it (strict: tick
+//     f_s + k - 1; relaxed: tick f_s + k), enforced with hipStreamWaitEvent;
+//   * a host-visible completion flag that the tick's last workgroup raises, so
+//     a tick costs one launch plus a spin on pinned memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/partconv_amd.h"
+#include "pc_io.hpp"
+#include "pc_kernels.hpp"
+
+#define PC_API extern "C" __attribute__((visibility("default")))
+#define PC_LEVEL_STREAMS 3
+#define PC_BIG_MIN 4096       // block sizes >= this use the four-step kernels
+#define PC_MAX_BLOCK 65536
+#define PC_MIN_BLOCK 16
+#define PC_TW_MAX (2 * PC_MAX_BLOCK)
+#define PC_TICK_TIMEOUT_S 10.0
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(PC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int ilog2(long long x)
+{
+    int l = 0;
+    while ((1ll << l) < x) l++;
+    return l;
+}
+static bool is_pow2(long long x) { return x > 0 && (x & (x - 1)) == 0; }
+
+// ----------------------------------------------------------------------------
+// Twiddle table, one per device: tw[n + t] = exp(-2 pi i t / n), double -> float.
+// ----------------------------------------------------------------------------
+static cf *g_tw[64];
+
+static int get_twiddles(int dev, cf **out)
+{
+    if (dev < 0 || dev >= 64) return fail(PC_ERR_ARG, "device index out of range");
+    if (!g_tw[dev]) {
+        std::vector<cf> h((size_t)2 * PC_TW_MAX);
+        h[0] = make_float2(1.f, 0.f);
+        for (long long n = 1; n <= PC_TW_MAX; n <<= 1)
+            for (long long t = 0; t < n; t++) {
+                const double a = -2.0 * M_PI * (double)t / (double)n;
+                h[n + t] = make_float2((float)cos(a), (float)sin(a));
+            }
+        cf *d = nullptr;
+        HIPCHK(hipMalloc(&d, h.size() * sizeof(cf)));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(cf), hipMemcpyHostToDevice));
+        g_tw[dev] = d;
+    }
+    *out = g_tw[dev];
+    return PC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Partition rules (PartConvFilter::setup validation, src/partconv.cpp:86-182)
+// ----------------------------------------------------------------------------
+static int check_partition(std::vector<int> &N, std::vector<int> &P, long long ir_len,
+                           std::vector<int> &delay, std::vector<int> &T)
+{
+    const int M = (int)N.size();
+    if (M < 1 || (int)P.size() != M) return fail(PC_ERR_PARTITION, "need M >= 1 levels");
+    if (M > PC_MAX_LEVELS) return fail(PC_ERR_PARTITION, "too many levels");
+    for (int i = 0; i < M; i++)
+        if (!is_pow2(N[i])) {
+            char m[160];
+            snprintf(m, sizeof m, "PartConvFilter ERROR: block size of level %d must be a power of 2 (N[%d] = %d)", i, i, N[i]);
+            return fail(PC_ERR_PARTITION, m);
+        }
+    for (int i = 1; i < M; i++)
+        if (N[i] <= N[i - 1]) {
+            char m[160];
+            snprintf(m, sizeof m, "PartConvFilter ERROR: N[%d]=%d <= N[%d]=%d", i, N[i], i - 1, N[i - 1]);
+            return fail(PC_ERR_PARTITION, m);
+        }
+    long long position = 2ll * N[0];
+    delay.assign(M, 0);
+    for (int i = 0; i < M - 1; i++) {
+        delay[i] = (int)(position / N[0]);
+        position += (long long)N[i] * P[i];
+        if (position < 2ll * N[i + 1]) {
+            char m[160];
+            snprintf(m, sizeof m, "PartConvFilter ERROR: sufficient delay not allowed for block size of level %d", i + 2);
+            return fail(PC_ERR_PARTITION, m);
+        }
+    }
+    delay[M - 1] = (int)(position / N[0]);
+    const long long rest = ir_len - (position - 2ll * N[0]);
+    // float division then ceil, as src/partconv.cpp:169
+    const int plast = (int)ceilf((float)rest / (float)N[M - 1]);
+    if (rest < 0 || plast <= 0) {
+        char m[160];
+        snprintf(m, sizeof m, "PartConvFilter ERROR: last partitioning level (%d) not required", M);
+        return fail(PC_ERR_PARTITION, m);
+    }
+    P[M - 1] = plast;
+    T.assign(M, 0);
+    long long t = 0;
+    for (int i = 0; i < M; i++) {
+        T[i] = (int)t;
+        t += (long long)N[i] * P[i];
+    }
+    if (t > (1ll << 30)) return fail(PC_ERR_PARTITION, "impulse response too long");
+    return PC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Engine state
+// ----------------------------------------------------------------------------
+struct LevelSpec {
+    int N, P, T, delay, kshift;
+    bool big;
+    BigShape sh;
+    size_t off_H, off_fdl, off_ybuf, off_scr;
+};
+
+struct InstDesc {
+    int mute = 0, in_ch = 0, out_ch = 0;
+    float scale = 1.f;
+    std::vector<int> N, P;
+    std::vector<float> ir;
+};
+
+struct Inst {
+    InstDesc d;
+    std::vector<LevelSpec> lv;
+    std::vector<int> gmap;  // global level -> local level or -1
+    int R;
+    size_t off_ring;
+};
+
+struct Batch {
+    int g = 0, N = 0;
+    bool big = false;
+    BigShape sh{0, 0};
+    LevelJob *d_jobs = nullptr;
+    int njobs = 0;
+};
+
+struct Group {
+    std::vector<int> members;
+    TickJob *d_tick = nullptr;
+    LevelJob *d_l0 = nullptr;     // level-0 jobs (filter API)
+    std::vector<Batch> batches;   // levels >= 1
+};
+
+static BigShape big_shape(int N)
+{
+    const int lg = ilog2(N);
+    BigShape s;
+    s.logB = std::min(7, lg / 2);
+    s.logA = lg - s.logB;
+    return s;
+}
+
+struct Engine {
+    int dev = 0, verbosity = 0, relaxed = 0, mode = PC_BATCHED;
+    int FS = 0, N0 = 0, nIn = 0, nOut = 0;
+    bool ready = false;
+    std::vector<Inst> inst;
+    std::vector<int> gsize, gshift;  // per global level: k_g (frames), log2
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    cf *tw = nullptr;
+    hipStream_t s_main = nullptr;
+    std::vector<hipStream_t> s_lvl;   // indexed by stream slot
+    std::vector<int> g_stream;        // global level -> stream slot
+    std::vector<hipEvent_t> ev_lvl;   // per global level
+    std::vector<bool> lvl_pending;
+    hipEvent_t ev_tick = nullptr;
+    std::vector<Group> groups;
+    int mix = 1;
+    float *y_inst = nullptr;
+    int *d_src_off = nullptr, *d_src_list = nullptr;
+    unsigned *d_ticket = nullptr;
+    float *h_in = nullptr, *h_out = nullptr;
+    unsigned long long *h_flag = nullptr;
+    float *d_in_map = nullptr, *d_out_map = nullptr;   // device views of the pinned buffers
+    unsigned long long *d_flag_map = nullptr;
+    uint64_t frame = 0;
+    // kernel probe: HIP events around one kernel of one global level
+    int probe_g = -1;
+    std::vector<hipEvent_t> pa, pb;
+    std::vector<bool> pused;
+    int pnext = 0;
+    double probe_ms = 0, probe_bytes = 0;
+    long long probe_n = 0;
+    // stats
+    std::vector<double> lat;
+    size_t lat_pos = 0;
+    bool lat_wrap = false;
+    unsigned long long ticks = 0, misses = 0;
+    double lat_sum = 0, lat_max = 0;
+};
+
+static hipStream_t level_stream(Engine &E, int g) { return E.s_lvl[E.g_stream[g]]; }
+
+static void engine_free(Engine &E)
+{
+    if (E.s_main) hipStreamSynchronize(E.s_main);
+    for (hipStream_t s : E.s_lvl) hipStreamSynchronize(s);
+    for (Group &G : E.groups) {
+        hipFree(G.d_tick);
+        hipFree(G.d_l0);
+        for (Batch &b : G.batches) hipFree(b.d_jobs);
+    }
+    E.groups.clear();
+    hipFree(E.arena);
+    E.arena = nullptr;
+    hipFree(E.y_inst);
+    hipFree(E.d_src_off);
+    hipFree(E.d_src_list);
+    hipFree(E.d_ticket);
+    E.y_inst = nullptr;
+    E.d_src_off = E.d_src_list = nullptr;
+    E.d_ticket = nullptr;
+    if (E.h_in) hipHostFree(E.h_in);
+    if (E.h_out) hipHostFree(E.h_out);
+    if (E.h_flag) hipHostFree(E.h_flag);
+    E.h_in = E.h_out = nullptr;
+    E.h_flag = nullptr;
+    for (hipEvent_t e : E.ev_lvl) hipEventDestroy(e);
+    E.ev_lvl.clear();
+    for (hipEvent_t e : E.pa) hipEventDestroy(e);
+    for (hipEvent_t e : E.pb) hipEventDestroy(e);
+    E.pa.clear();
+    E.pb.clear();
+    E.pused.clear();
+    E.probe_g = -1;
+    if (E.ev_tick) hipEventDestroy(E.ev_tick);
+    E.ev_tick = nullptr;
+    for (hipStream_t s : E.s_lvl) hipStreamDestroy(s);
+    E.s_lvl.clear();
+    if (E.s_main) hipStreamDestroy(E.s_main);
+    E.s_main = nullptr;
+    E.inst.clear();
+    E.ready = false;
+}
+
+// ---- kernel dispatch --------------------------------------------------------
+template <int LOGN>
+static void launch_small_t(const LevelJob *jobs, int njobs, int N0, uint64_t m, uint64_t bcur,
+                           int ypar, const cf *tw, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_small<LOGN>, dim3(njobs), dim3(SmallCfg<LOGN>::NT), 0, s, jobs, N0, m,
+                       bcur, ypar, tw, mode);
+}
+
+static int launch_small(int N, const LevelJob *jobs, int njobs, int N0, uint64_t m, uint64_t bcur,
+                        int ypar, const cf *tw, int mode, hipStream_t s)
+{
+    switch (ilog2(N)) {
+    case 4: launch_small_t<4>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 5: launch_small_t<5>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 6: launch_small_t<6>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 7: launch_small_t<7>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 8: launch_small_t<8>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 9: launch_small_t<9>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 10: launch_small_t<10>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 11: launch_small_t<11>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    default: return fail(PC_ERR_ARG, "unsupported small block size");
+    }
+    return PC_OK;
+}
+
+static size_t big1_lds(BigShape sh) { return (size_t)2 * BIG_G * ((1 << sh.logB) + 1) * sizeof(cf); }
+static size_t big2_lds(BigShape sh) { return (size_t)2 * 2 * ((1 << sh.logA) + 1) * sizeof(cf); }
+
+static void launch_big(BigShape sh, const LevelJob *jobs, int njobs, int N0, uint64_t m,
+                       uint64_t bcur, int ypar, const cf *tw, int mode, hipStream_t s)
+{
+    const int A = 1 << sh.logA, B = 1 << sh.logB;
+    hipLaunchKernelGGL(k_big1, dim3((A / BIG_G) * njobs), dim3(BIG_NT), big1_lds(sh), s, jobs, sh, N0,
+                       m, tw, mode);
+    hipLaunchKernelGGL(k_big2, dim3((B / 2) * njobs), dim3(BIG_NT), big2_lds(sh), s, jobs, sh, bcur,
+                       tw, mode);
+    if (mode == PC_MODE_RUN)
+        hipLaunchKernelGGL(k_big3, dim3((A / BIG_G) * njobs), dim3(BIG_NT), big1_lds(sh), s, jobs, sh,
+                           ypar, tw);
+}
+
+template <int LOGN>
+static void launch_tick_t(const TickArgs &a, const cf *tw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tick<LOGN>, dim3(a.njobs), dim3(SmallCfg<LOGN>::NT), 0, s, a, tw);
+}
+
+static int launch_tick(int N0, const TickArgs &a, const cf *tw, hipStream_t s)
+{
+    switch (ilog2(N0)) {
+    case 4: launch_tick_t<4>(a, tw, s); break;
+    case 5: launch_tick_t<5>(a, tw, s); break;
+    case 6: launch_tick_t<6>(a, tw, s); break;
+    case 7: launch_tick_t<7>(a, tw, s); break;
+    case 8: launch_tick_t<8>(a, tw, s); break;
+    case 9: launch_tick_t<9>(a, tw, s); break;
+    case 10: launch_tick_t<10>(a, tw, s); break;
+    case 11: launch_tick_t<11>(a, tw, s); break;
+    default: return fail(PC_ERR_ARG, "unsupported buffer_size");
+    }
+    return PC_OK;
+}
+
+// ring write for the caller-driven filter API
+__global__ void k_ring_write(float *ring, long long mask, long long t0, const float *x, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        const long long p = (t0 + j) & mask;
+        ring[p] = x[j];
+        ring[p + mask + 1] = x[j];
+    }
+}
+
+// y = level-0 block + sum_L level L sub-block (filter API output)
+struct FilterOut {
+    const float *src[PC_MAX_LEVELS];
+    int nsrc;
+};
+__global__ void k_filter_out(FilterOut fo, int N0, float *y)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < N0) {
+        float a = 0.f;
+        for (int q = 0; q < fo.nsrc; q++) a += fo.src[q][j];
+        y[j] = a;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Setup
+// ----------------------------------------------------------------------------
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vector<InstDesc> &descs)
+{
+    if (E.ready) return fail(PC_ERR_STATE, "setup called twice without cleanup");
+    if (descs.empty()) return fail(PC_ERR_ARG, "no instances");
+    if (!is_pow2(N0) || N0 < PC_MIN_BLOCK || N0 > 2048)
+        return fail(PC_ERR_ARG, "buffer_size must be a power of two in [16, 2048]");
+    if (nIn < 1 || nOut < 1) return fail(PC_ERR_ARG, "need at least one input and output channel");
+    HIPCHK(hipGetDevice(&E.dev));
+    E.FS = FS;
+    E.N0 = N0;
+    E.nIn = nIn;
+    E.nOut = nOut;
+    E.inst.clear();
+    int max_block = N0;
+    for (size_t i = 0; i < descs.size(); i++) {
+        InstDesc &d = descs[i];
+        if (d.in_ch < 0 || d.in_ch >= nIn || d.out_ch < 0 || d.out_ch >= nOut)
+            return fail(PC_ERR_CONFIG, "instance " + std::to_string(i) + ": channel out of range");
+        if (d.N.empty() || d.N[0] != N0) {
+            char m[96];
+            snprintf(m, sizeof m, "line %zu: first block size must equal buffer_size", i + 1);
+            return fail(PC_ERR_CONFIG, m);
+        }
+        Inst I;
+        std::vector<int> delay, T;
+        std::vector<int> N = d.N, P = d.P;
+        int rc = check_partition(N, P, (long long)d.ir.size(), delay, T);
+        if (rc) return rc;
+        for (int n : N)
+            if (n > PC_MAX_BLOCK) return fail(PC_ERR_PARTITION, "block size above 65536");
+        d.P = P;
+        I.d = d;
+        long long need = 0;
+        for (size_t L = 0; L < N.size(); L++) {
+            LevelSpec s{};
+            s.N = N[L];
+            s.P = P[L];
+            s.T = T[L];
+            s.delay = delay[L];
+            s.kshift = ilog2(N[L] / N0);
+            s.big = (L > 0) && N[L] >= PC_BIG_MIN;
+            if (s.big) s.sh = big_shape(N[L]);
+            I.lv.push_back(s);
+            need = std::max(need, std::max(2ll * N[L], (long long)N[L] + N0 + T[L]));
+            max_block = std::max(max_block, N[L]);
+        }
+        I.R = (int)(1ll << ilog2(need + N0));
+        E.inst.push_back(I);
+    }
+    // global level set (src/partconvMulti.cpp:139-244)
+    const int temp_levels = ilog2(max_block / N0) + 1;
+    E.gsize.clear();
+    E.gshift.clear();
+    for (int g = 0; g < temp_levels; g++) {
+        const int bs = N0 << g;
+        bool used = false;
+        for (Inst &I : E.inst)
+            for (LevelSpec &s : I.lv) used |= (s.N == bs);
+        if (!used) continue;
+        E.gsize.push_back(1 << g);
+        E.gshift.push_back(g);
+    }
+    const int G = (int)E.gsize.size();
+    for (Inst &I : E.inst) {
+        I.gmap.assign(G, -1);
+        for (int g = 0; g < G; g++)
+            for (size_t L = 0; L < I.lv.size(); L++)
+                if (I.lv[L].N == (N0 << E.gshift[g])) I.gmap[g] = (int)L;
+    }
+    // arena layout
+    size_t off = 0;
+    for (Inst &I : E.inst) {
+        I.off_ring = off;
+        off = align_up(off + (size_t)2 * I.R * sizeof(float));
+        for (LevelSpec &s : I.lv) {
+            s.off_H = off;
+            off = align_up(off + (size_t)s.P * s.N * sizeof(cf));
+            s.off_fdl = off;
+            off = align_up(off + (size_t)s.P * s.N * sizeof(cf));
+            s.off_ybuf = off;
+            off = align_up(off + (size_t)2 * s.N * sizeof(float));
+            s.off_scr = off;
+            if (s.big) off = align_up(off + (size_t)s.N * sizeof(cf));
+        }
+    }
+    E.arena_bytes = off;
+    HIPCHK(hipMalloc(&E.arena, E.arena_bytes));
+    HIPCHK(hipMemset(E.arena, 0, E.arena_bytes));
+    int rc = get_twiddles(E.dev, &E.tw);
+    if (rc) return rc;
+    HIPCHK(hipStreamCreateWithFlags(&E.s_main, hipStreamNonBlocking));
+
+    // ---- filter spectra H[L][p] = r2c([0_N | h_{L,p}]) / 2N (src/partconv.cpp:199-230)
+    {
+        size_t tot = 0;
+        std::vector<size_t> ir_off(E.inst.size());
+        for (size_t i = 0; i < E.inst.size(); i++) {
+            ir_off[i] = tot;
+            tot += E.inst[i].d.ir.size();
+        }
+        float *d_ir = nullptr;
+        HIPCHK(hipMalloc(&d_ir, std::max<size_t>(tot, 1) * sizeof(float)));
+        for (size_t i = 0; i < E.inst.size(); i++)
+            if (!E.inst[i].d.ir.empty())
+                HIPCHK(hipMemcpy(d_ir + ir_off[i], E.inst[i].d.ir.data(),
+                                 E.inst[i].d.ir.size() * sizeof(float), hipMemcpyHostToDevice));
+        const int CHUNK = 256;
+        cf *d_scr = nullptr;
+        LevelJob *d_jobs = nullptr;
+        HIPCHK(hipMalloc(&d_jobs, CHUNK * sizeof(LevelJob)));
+        size_t scr_elems = 0;
+        for (int g = 0; g < G; g++) {
+            const int N = N0 << E.gshift[g];
+            std::vector<LevelJob> jobs;
+            for (size_t i = 0; i < E.inst.size(); i++) {
+                Inst &I = E.inst[i];
+                const int L = I.gmap[g];
+                if (L < 0) continue;
+                LevelSpec &s = I.lv[L];
+                for (int p = 0; p < s.P; p++) {
+                    LevelJob J{};
+                    J.ring = d_ir + ir_off[i];
+                    J.ring_mask = (int)I.d.ir.size();
+                    J.T = s.T + p * s.N;
+                    J.fdl = reinterpret_cast<cf *>(E.arena + s.off_H) + (size_t)p * s.N;
+                    J.P = 1;
+                    jobs.push_back(J);
+                }
+            }
+            const bool big = (g > 0) && N >= PC_BIG_MIN;
+            if (big && scr_elems < (size_t)CHUNK * N) {
+                hipFree(d_scr);
+                scr_elems = (size_t)CHUNK * N;
+                HIPCHK(hipMalloc(&d_scr, scr_elems * sizeof(cf)));
+            }
+            for (size_t c0 = 0; c0 < jobs.size(); c0 += CHUNK) {
+                const int n = (int)std::min<size_t>(CHUNK, jobs.size() - c0);
+                for (int q = 0; q < n; q++)
+                    if (big) jobs[c0 + q].scratch = d_scr + (size_t)q * N;
+                HIPCHK(hipMemcpyAsync(d_jobs, jobs.data() + c0, n * sizeof(LevelJob),
+                                      hipMemcpyHostToDevice, E.s_main));
+                if (big) launch_big(big_shape(N), d_jobs, n, N0, 0, 0, 0, E.tw, PC_MODE_SPECTRUM, E.s_main);
+                else {
+                    rc = launch_small(N, d_jobs, n, N0, 0, 0, 0, E.tw, PC_MODE_SPECTRUM, E.s_main);
+                    if (rc) return rc;
+                }
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipStreamSynchronize(E.s_main));
+            }
+        }
+        hipFree(d_scr);
+        hipFree(d_jobs);
+        hipFree(d_ir);
+    }
+
+    // ---- launch groups: batched = one group; plain = one per instance
+    std::vector<std::vector<int>> members;
+    if (E.mode == PC_BATCHED) {
+        members.emplace_back();
+        for (size_t i = 0; i < E.inst.size(); i++) members[0].push_back((int)i);
+    } else {
+        for (size_t i = 0; i < E.inst.size(); i++) members.push_back({(int)i});
+    }
+    auto mk_level_job = [&](Inst &I, LevelSpec &s) {
+        LevelJob J{};
+        J.ring = reinterpret_cast<const float *>(E.arena + I.off_ring);
+        J.fdl = reinterpret_cast<cf *>(E.arena + s.off_fdl);
+        J.H = reinterpret_cast<const cf *>(E.arena + s.off_H);
+        J.ybuf = reinterpret_cast<float *>(E.arena + s.off_ybuf);
+        J.scratch = s.big ? reinterpret_cast<cf *>(E.arena + s.off_scr) : nullptr;
+        J.P = s.P;
+        J.T = s.T;
+        J.ring_mask = I.R - 1;
+        return J;
+    };
+    for (auto &mem : members) {
+        Group Gr;
+        Gr.members = mem;
+        std::vector<TickJob> tj;
+        std::vector<LevelJob> l0;
+        for (int i : mem) {
+            Inst &I = E.inst[i];
+            TickJob t{};
+            t.l0 = mk_level_job(I, I.lv[0]);
+            t.in_ch = I.d.in_ch;
+            t.out_ch = I.d.out_ch;
+            t.mute = I.d.mute;
+            t.scale = I.d.scale;
+            t.nlev = (int)I.lv.size() - 1;
+            for (int L = 1; L < (int)I.lv.size(); L++) {
+                t.kshift[L - 1] = I.lv[L].kshift;
+                t.NL[L - 1] = I.lv[L].N;
+                t.ybuf[L - 1] = reinterpret_cast<const float *>(E.arena + I.lv[L].off_ybuf);
+            }
+            tj.push_back(t);
+            l0.push_back(t.l0);
+        }
+        HIPCHK(hipMalloc(&Gr.d_tick, tj.size() * sizeof(TickJob)));
+        HIPCHK(hipMemcpy(Gr.d_tick, tj.data(), tj.size() * sizeof(TickJob), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&Gr.d_l0, l0.size() * sizeof(LevelJob)));
+        HIPCHK(hipMemcpy(Gr.d_l0, l0.data(), l0.size() * sizeof(LevelJob), hipMemcpyHostToDevice));
+        for (int g = 1; g < G; g++) {
+            std::vector<LevelJob> js;
+            for (int i : mem) {
+                Inst &I = E.inst[i];
+                if (I.gmap[g] >= 0) js.push_back(mk_level_job(I, I.lv[I.gmap[g]]));
+            }
+            if (js.empty()) continue;
+            Batch b;
+            b.g = g;
+            b.N = N0 << E.gshift[g];
+            b.big = b.N >= PC_BIG_MIN;
+            if (b.big) b.sh = big_shape(b.N);
+            b.njobs = (int)js.size();
+            HIPCHK(hipMalloc(&b.d_jobs, js.size() * sizeof(LevelJob)));
+            HIPCHK(hipMemcpy(b.d_jobs, js.data(), js.size() * sizeof(LevelJob), hipMemcpyHostToDevice));
+            Gr.batches.push_back(b);
+        }
+        E.groups.push_back(Gr);
+    }
+    // ---- output mix routing
+    std::vector<int> src_off(nOut + 1, 0), src_list;
+    bool direct = (E.mode == PC_BATCHED);
+    for (int c = 0; c < nOut; c++) {
+        int cnt = 0;
+        for (size_t i = 0; i < E.inst.size(); i++)
+            if (!E.inst[i].d.mute && E.inst[i].d.out_ch == c) {
+                src_list.push_back((int)i);
+                cnt++;
+            }
+        src_off[c + 1] = (int)src_list.size();
+        if (cnt != 1) direct = false;
+    }
+    E.mix = (E.mode == PC_PLAIN) ? 0 : (direct ? 1 : 2);
+    HIPCHK(hipMalloc(&E.y_inst, E.inst.size() * (size_t)N0 * sizeof(float)));
+    HIPCHK(hipMalloc(&E.d_src_off, src_off.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(E.d_src_off, src_off.data(), src_off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&E.d_src_list, std::max<size_t>(src_list.size(), 1) * sizeof(int)));
+    if (!src_list.empty())
+        HIPCHK(hipMemcpy(E.d_src_list, src_list.data(), src_list.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&E.d_ticket, 64));
+    HIPCHK(hipMemset(E.d_ticket, 0, 64));
+    // ---- streams, events, pinned staging
+    const int nls = std::min(PC_LEVEL_STREAMS, std::max(0, G - 1));
+    for (int q = 0; q < nls; q++) {
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        E.s_lvl.push_back(s);
+    }
+    E.g_stream.assign(G, 0);
+    for (int g = 1; g < G; g++) E.g_stream[g] = (g - 1) % std::max(1, nls);
+    E.ev_lvl.assign(G, nullptr);
+    E.lvl_pending.assign(G, false);
+    for (int g = 0; g < G; g++) HIPCHK(hipEventCreateWithFlags(&E.ev_lvl[g], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&E.ev_tick, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc(&E.h_in, (size_t)N0 * nIn * sizeof(float), hipHostMallocCoherent | hipHostMallocMapped));
+    HIPCHK(hipHostMalloc(&E.h_out, (size_t)N0 * std::max(nOut, 1) * sizeof(float),
+                         hipHostMallocCoherent | hipHostMallocMapped));
+    HIPCHK(hipHostMalloc(&E.h_flag, 64, hipHostMallocCoherent | hipHostMallocMapped));
+    memset(E.h_flag, 0, 64);
+    HIPCHK(hipHostGetDevicePointer((void **)&E.d_in_map, E.h_in, 0));
+    HIPCHK(hipHostGetDevicePointer((void **)&E.d_out_map, E.h_out, 0));
+    HIPCHK(hipHostGetDevicePointer((void **)&E.d_flag_map, E.h_flag, 0));
+    memset(E.h_in, 0, (size_t)N0 * nIn * sizeof(float));
+    E.frame = 0;
+    E.lat.assign(1 << 20, 0.0);
+    E.lat_pos = 0;
+    E.lat_wrap = false;
+    E.ticks = E.misses = 0;
+    E.lat_sum = E.lat_max = 0;
+    E.ready = true;
+    if (E.verbosity > 0) {
+        for (size_t i = 0; i < E.inst.size(); i++) {
+            Inst &I = E.inst[i];
+            printf("\nPartitioning Specs (instance %zu):\nimpulse response: %.2f sec\npartitioning: ", i,
+                   (double)I.d.ir.size() / (FS > 0 ? FS : 1));
+            for (size_t L = 0; L < I.lv.size(); L++)
+                printf("%ux%u%s", I.lv[L].N, I.lv[L].P, L + 1 < I.lv.size() ? ", " : "\n");
+            printf("filter latency: %g ms\n", FS > 0 ? 1000.0 * N0 / FS : 0.0);
+        }
+        printf("Total PartConv instances   : %zu\nbuffer_size: %d\nmax_block_size: %d\n", E.inst.size(), N0,
+               max_block);
+    }
+    return PC_OK;
+}
+
+static int engine_reset(Engine &E)
+{
+    if (!E.ready) return fail(PC_ERR_STATE, "reset before setup");
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    for (hipStream_t s : E.s_lvl) HIPCHK(hipStreamSynchronize(s));
+    for (Inst &I : E.inst) {
+        HIPCHK(hipMemsetAsync(E.arena + I.off_ring, 0, (size_t)2 * I.R * sizeof(float), E.s_main));
+        for (LevelSpec &s : I.lv) {
+            HIPCHK(hipMemsetAsync(E.arena + s.off_fdl, 0, (size_t)s.P * s.N * sizeof(cf), E.s_main));
+            HIPCHK(hipMemsetAsync(E.arena + s.off_ybuf, 0, (size_t)2 * s.N * sizeof(float), E.s_main));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    E.frame = 0;
+    std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
+    *E.h_flag = 0;
+    return PC_OK;
+}
+
+static void record_latency(Engine &E, double us)
+{
+    E.lat[E.lat_pos] = us;
+    if (++E.lat_pos == E.lat.size()) {
+        E.lat_pos = 0;
+        E.lat_wrap = true;
+    }
+    E.ticks++;
+    E.lat_sum += us;
+    E.lat_max = std::max(E.lat_max, us);
+    const double period = E.FS > 0 ? 1e6 * E.N0 / E.FS : 0.0;
+    if (period > 0 && us > period) E.misses++;
+}
+
+static int wait_flag(Engine &E, unsigned long long target)
+{
+    volatile unsigned long long *fl = E.h_flag;
+    if (*fl >= target) return PC_OK;
+    const double t0 = now_s();
+    unsigned spins = 0;
+    while (*fl < target) {
+        if ((++spins & 0xFFFF) == 0) {
+            hipError_t q = hipStreamQuery(E.s_main);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(PC_ERR_HIP, std::string("tick failed: ") + hipGetErrorString(q));
+            if (now_s() - t0 > PC_TICK_TIMEOUT_S) return fail(PC_ERR_TIMEOUT, "tick did not complete");
+        }
+    }
+    return PC_OK;
+}
+
+// Probe bookkeeping: slot q's pair is folded into the running total before reuse.
+static int probe_fold(Engine &E, int q)
+{
+    if (!E.pused[q]) return PC_OK;
+    HIPCHK(hipEventSynchronize(E.pb[q]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, E.pa[q], E.pb[q]));
+    E.probe_ms += ms;
+    E.probe_n++;
+    E.pused[q] = false;
+    return PC_OK;
+}
+
+static int probe_begin(Engine &E, hipStream_t s, int *slot)
+{
+    const int q = E.pnext;
+    E.pnext = (E.pnext + 1) % (int)E.pa.size();
+    int rc = probe_fold(E, q);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(E.pa[q], s));
+    *slot = q;
+    return PC_OK;
+}
+
+static int probe_end(Engine &E, hipStream_t s, int slot)
+{
+    HIPCHK(hipEventRecord(E.pb[slot], s));
+    E.pused[slot] = true;
+    return PC_OK;
+}
+
+// Launch one level batch (block m) on stream s; a probed level times its
+// multiply-accumulate kernel (k_big2, or the fused small kernel).
+static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
+{
+    const bool probe = (E.probe_g == b.g);
+    int slot = -1, rc;
+    if (b.big) {
+        const BigShape sh = b.sh;
+        const int A = 1 << sh.logA, B = 1 << sh.logB;
+        hipLaunchKernelGGL(k_big1, dim3((A / BIG_G) * b.njobs), dim3(BIG_NT), big1_lds(sh), s, b.d_jobs, sh,
+                           E.N0, m, E.tw, (int)PC_MODE_RUN);
+        if (probe && (rc = probe_begin(E, s, &slot))) return rc;
+        hipLaunchKernelGGL(k_big2, dim3((B / 2) * b.njobs), dim3(BIG_NT), big2_lds(sh), s, b.d_jobs, sh, m,
+                           E.tw, (int)PC_MODE_RUN);
+        if (probe && (rc = probe_end(E, s, slot))) return rc;
+        hipLaunchKernelGGL(k_big3, dim3((A / BIG_G) * b.njobs), dim3(BIG_NT), big1_lds(sh), s, b.d_jobs, sh,
+                           (int)(m & 1), E.tw);
+    } else {
+        if (probe && (rc = probe_begin(E, s, &slot))) return rc;
+        rc = launch_small(b.N, b.d_jobs, b.njobs, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
+        if (rc) return rc;
+        if (probe && (rc = probe_end(E, s, slot))) return rc;
+    }
+    return PC_OK;
+}
+
+// One tick of PartConvMulti[Relaxed]::run on the device.
+static int engine_tick(Engine &E, const float *in, long long in_stride, long long fbase, float *out,
+                       long long out_stride, bool wait)
+{
+    const double t0 = now_s();
+    const uint64_t f = E.frame;
+    const int G = (int)E.gsize.size();
+    // deadlines: wait for the blocks this tick reads for the first time
+    for (int g = 1; g < G; g++) {
+        const uint64_t k = (uint64_t)E.gsize[g];
+        const bool due = E.relaxed ? (f % k == 0 && f >= k) : ((f + 1) % k == 0 && f + 1 >= k);
+        if (due && E.lvl_pending[g]) {
+            HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
+            E.lvl_pending[g] = false;
+        }
+    }
+    TickArgs a{};
+    a.N0 = E.N0;
+    a.f = f;
+    a.in = in;
+    a.in_stride = in_stride;
+    a.fbase = fbase;
+    a.nIn = E.nIn;
+    a.out = out;
+    a.out_stride = out_stride;
+    a.nOut = E.nOut;
+    a.relaxed = E.relaxed;
+    a.src_off = E.d_src_off;
+    a.src_list = E.d_src_list;
+    a.ticket = E.d_ticket;
+    if (E.mode == PC_BATCHED) {
+        Group &Gr = E.groups[0];
+        a.jobs = Gr.d_tick;
+        a.njobs = (int)Gr.members.size();
+        a.mix = E.mix;
+        a.y_inst = E.y_inst;
+        a.done_flag = E.d_flag_map;
+        int slot = -1;
+        if (E.probe_g == 0) {
+            int rc = probe_begin(E, E.s_main, &slot);
+            if (rc) return rc;
+        }
+        int rc = launch_tick(E.N0, a, E.tw, E.s_main);
+        if (rc) return rc;
+        if (slot >= 0 && (rc = probe_end(E, E.s_main, slot))) return rc;
+    } else {
+        for (Group &Gr : E.groups) {
+            a.jobs = Gr.d_tick;
+            a.njobs = 1;
+            a.mix = 0;
+            a.y_inst = E.y_inst + (size_t)Gr.members[0] * E.N0;
+            a.done_flag = nullptr;
+            int rc = launch_tick(E.N0, a, E.tw, E.s_main);
+            if (rc) return rc;
+        }
+        const int n = E.nOut * E.N0;
+        hipLaunchKernelGGL(k_mix, dim3((n + 255) / 256), dim3(256), 0, E.s_main, E.y_inst, E.d_src_off,
+                           E.d_src_list, E.N0, E.nOut, out, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
+                           E.d_flag_map, f);
+    }
+    HIPCHK(hipGetLastError());
+    // start the levels whose period begins at this tick (src/partconvMulti.cpp:860-873)
+    bool recorded = false;
+    for (int g = 1; g < G; g++) {
+        const uint64_t k = (uint64_t)E.gsize[g];
+        if (f % k != 0) continue;
+        if (!recorded) {
+            HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
+            recorded = true;
+        }
+        hipStream_t s = level_stream(E, g);
+        HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
+        const uint64_t m = f >> E.gshift[g];
+        for (Group &Gr : E.groups)
+            for (Batch &b : Gr.batches) {
+                if (b.g != g) continue;
+                int rc = launch_batch(E, b, m, s);
+                if (rc) return rc;
+            }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E.ev_lvl[g], s));
+        E.lvl_pending[g] = true;
+    }
+    E.frame++;
+    if (wait) {
+        int rc = wait_flag(E, f + 1);
+        if (rc) return rc;
+        record_latency(E, 1e6 * (now_s() - t0));
+    }
+    return PC_OK;
+}
+
+static int engine_drain(Engine &E)
+{
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    for (hipStream_t s : E.s_lvl) HIPCHK(hipStreamSynchronize(s));
+    return PC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// C ABI: PartConvMulti
+// ----------------------------------------------------------------------------
+struct pc_multi {
+    Engine E;
+    int max_block = 0;
+};
+
+PC_API const char *pc_last_error(void) { return g_err.c_str(); }
+
+PC_API int pc_device_name(char *buf, int len)
+{
+    if (!buf || len <= 0) return fail(PC_ERR_ARG, "null buffer");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+    return PC_OK;
+}
+
+PC_API pc_multi *pcm_create(int verbosity, int relaxed)
+{
+    pc_multi *h = new pc_multi();
+    h->E.verbosity = verbosity;
+    h->E.relaxed = relaxed ? 1 : 0;
+    return h;
+}
+
+PC_API int pcm_set_mode(pc_multi *h, int mode)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    if (h->E.ready) return fail(PC_ERR_STATE, "mode must be set before setup");
+    if (mode != PC_BATCHED && mode != PC_PLAIN) return fail(PC_ERR_ARG, "unknown mode");
+    h->E.mode = mode;
+    return PC_OK;
+}
+
+static int multi_setup(pc_multi *h, int FS, int N0, int nIn, int nOut, std::vector<InstDesc> &d)
+{
+    int rc = engine_setup(h->E, FS, N0, nIn, nOut, d);
+    if (rc) {
+        std::string msg = g_err;
+        engine_free(h->E);
+        g_err = msg;
+        return rc;
+    }
+    h->max_block = 0;
+    for (Inst &I : h->E.inst)
+        for (LevelSpec &s : I.lv) h->max_block = std::max(h->max_block, s.N);
+    return PC_OK;
+}
+
+PC_API int pcm_setup(pc_multi *h, const char *config_file, int max_threads_per_level,
+                     int max_level0_threads, double min_work)
+{
+    (void)max_threads_per_level;
+    (void)max_level0_threads;
+    (void)min_work;
+    if (!h || !config_file) return fail(PC_ERR_ARG, "null argument");
+    pcio::Config cfg;
+    std::string err;
+    if (!pcio::parse_config(config_file, cfg, err)) return fail(PC_ERR_CONFIG, err);
+    std::vector<InstDesc> d;
+    for (const pcio::ConfigLine &L : cfg.lines) {
+        InstDesc x;
+        x.mute = L.mute;
+        x.in_ch = L.in_ch;
+        x.out_ch = L.out_ch;
+        x.scale = L.scale;
+        x.N = L.N;
+        x.P = L.P;
+        int sr = 0;
+        if (!pcio::read_wav(L.ir_file, x.ir, &sr, err)) return fail(PC_ERR_IO, err);
+        d.push_back(std::move(x));
+    }
+    return multi_setup(h, cfg.FS, cfg.buffer_size, cfg.nIn, cfg.nOut, d);
+}
+
+PC_API int pcm_setup_mem(pc_multi *h, int FS, int buffer_size, int nIn, int nOut, int numPCs,
+                         const pc_instance *inst)
+{
+    if (!h || !inst || numPCs < 1) return fail(PC_ERR_ARG, "null argument");
+    std::vector<InstDesc> d;
+    for (int i = 0; i < numPCs; i++) {
+        const pc_instance &s = inst[i];
+        if (s.M < 1 || !s.N || !s.P || (s.ir_len > 0 && !s.ir) || s.ir_len < 0)
+            return fail(PC_ERR_ARG, "instance " + std::to_string(i) + ": bad partition or IR");
+        InstDesc x;
+        x.mute = s.mute;
+        x.in_ch = s.in_ch;
+        x.out_ch = s.out_ch;
+        x.scale = s.scale;
+        x.N.assign(s.N, s.N + s.M);
+        x.P.assign(s.P, s.P + s.M);
+        x.ir.assign(s.ir, s.ir + s.ir_len);
+        d.push_back(std::move(x));
+    }
+    return multi_setup(h, FS, buffer_size, nIn, nOut, d);
+}
+
+PC_API int pcm_run(pc_multi *h, float *output, const float *input, int frameCount)
+{
+    if (!h || !output || !input) return fail(PC_ERR_ARG, "null argument");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    if (frameCount != E.N0) {
+        char m[96];
+        snprintf(m, sizeof m, "frameCount (%d) != buffer_size (%d)", frameCount, E.N0);
+        return fail(PC_ERR_FRAMECOUNT, m);
+    }
+    memcpy(E.h_in, input, (size_t)E.N0 * E.nIn * sizeof(float));
+    int rc = engine_tick(E, E.d_in_map, 0, (long long)E.frame, E.d_out_map, 0, true);
+    if (rc) return rc;
+    memcpy(output, E.h_out, (size_t)E.N0 * E.nOut * sizeof(float));
+    return PC_OK;
+}
+
+PC_API int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long long nframes,
+                          int sync_each_tick)
+{
+    if (!h || !d_out || !d_in || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    const long long fbase = (long long)E.frame;
+    for (long long t = 0; t < nframes; t++) {
+        int rc = engine_tick(E, d_in, (long long)E.N0 * E.nIn, fbase, d_out, (long long)E.N0 * E.nOut,
+                             sync_each_tick != 0);
+        if (rc) return rc;
+    }
+    if (!sync_each_tick) return engine_drain(E);
+    return PC_OK;
+}
+
+PC_API int pcm_reset(pc_multi *h)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    return engine_reset(h->E);
+}
+
+PC_API int pcm_cleanup(pc_multi *h)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    engine_free(h->E);
+    return PC_OK;
+}
+
+PC_API void pcm_destroy(pc_multi *h)
+{
+    if (!h) return;
+    engine_free(h->E);
+    delete h;
+}
+
+PC_API int pcm_get_info(const pc_multi *h, pc_multi_info *info)
+{
+    if (!h || !info) return fail(PC_ERR_ARG, "null argument");
+    const Engine &E = h->E;
+    info->FS = E.FS;
+    info->buffer_size = E.N0;
+    info->max_block_size = h->max_block;
+    info->verbosity = E.verbosity;
+    info->numInputChannels = E.nIn;
+    info->numOutputChannels = E.nOut;
+    info->numPCs = (int)E.inst.size();
+    info->num_levels = (int)E.gsize.size();
+    info->frameNum = E.frame;
+    return PC_OK;
+}
+
+PC_API int pcm_get_stats(const pc_multi *h, pc_stats *st)
+{
+    if (!h || !st) return fail(PC_ERR_ARG, "null argument");
+    const Engine &E = h->E;
+    memset(st, 0, sizeof(*st));
+    st->ticks = E.ticks;
+    st->deadline_misses = E.misses;
+    st->period_us = E.FS > 0 ? 1e6 * E.N0 / E.FS : 0.0;
+    st->max_us = E.lat_max;
+    st->mean_us = E.ticks ? E.lat_sum / (double)E.ticks : 0.0;
+    const size_t n = E.lat_wrap ? E.lat.size() : E.lat_pos;
+    if (n) {
+        std::vector<double> v(E.lat.begin(), E.lat.begin() + n);
+        auto pct = [&](double q) {
+            size_t i = (size_t)std::min<double>((double)(n - 1), std::floor(q * (double)(n - 1) + 0.5));
+            std::nth_element(v.begin(), v.begin() + i, v.end());
+            return v[i];
+        };
+        st->p50_us = pct(0.50);
+        st->p99_us = pct(0.99);
+    }
+    return PC_OK;
+}
+
+PC_API int pcm_clear_stats(pc_multi *h)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    Engine &E = h->E;
+    E.lat_pos = 0;
+    E.lat_wrap = false;
+    E.ticks = E.misses = 0;
+    E.lat_sum = E.lat_max = 0;
+    return PC_OK;
+}
+
+PC_API long long pcm_get_latencies(const pc_multi *h, double *out_us, long long cap)
+{
+    if (!h || !out_us || cap < 0) return -1;
+    const Engine &E = h->E;
+    const size_t n = E.lat_wrap ? E.lat.size() : E.lat_pos;
+    const size_t m = std::min<size_t>(n, (size_t)cap);
+    for (size_t q = 0; q < m; q++) {
+        size_t idx = (E.lat_pos + E.lat.size() - m + q) % E.lat.size();
+        out_us[q] = E.lat[idx];
+    }
+    return (long long)m;
+}
+
+PC_API int pcm_instance_partition(const pc_multi *h, int inst, int *N, int *P, int cap)
+{
+    if (!h || inst < 0 || inst >= (int)h->E.inst.size()) return -1;
+    const Inst &I = h->E.inst[inst];
+    for (int L = 0; L < (int)I.lv.size() && L < cap; L++) {
+        if (N) N[L] = I.lv[L].N;
+        if (P) P[L] = I.lv[L].P;
+    }
+    return (int)I.lv.size();
+}
+
+PC_API int pcm_probe(pc_multi *h, int global_level)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "probe before setup");
+    if (global_level >= (int)E.gsize.size()) return fail(PC_ERR_ARG, "level out of range");
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    for (hipStream_t s : E.s_lvl) HIPCHK(hipStreamSynchronize(s));
+    if (E.pa.empty()) {
+        E.pa.resize(16);
+        E.pb.resize(16);
+        E.pused.assign(16, false);
+        for (int q = 0; q < 16; q++) {
+            HIPCHK(hipEventCreate(&E.pa[q]));
+            HIPCHK(hipEventCreate(&E.pb[q]));
+        }
+    }
+    std::fill(E.pused.begin(), E.pused.end(), false);
+    E.probe_g = global_level;
+    E.probe_ms = 0;
+    E.probe_n = 0;
+    E.probe_bytes = 0;
+    // algorithmic bytes per launch: per job 8(N+1)(2P+1) for the multiply-
+    // accumulate (H + FDL rows + slot write); the tick kernel adds the window
+    // and output of level 0 (+12 N0) per instance
+    if (global_level >= 0) {
+        for (Inst &I : E.inst) {
+            const int L = I.gmap[global_level];
+            if (L < 0) continue;
+            const LevelSpec &s = I.lv[L];
+            E.probe_bytes += 8.0 * (s.N + 1) * (2.0 * s.P + 1);
+            if (global_level == 0) E.probe_bytes += 12.0 * s.N;
+        }
+    }
+    return PC_OK;
+}
+
+PC_API int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, double *bytes_per_launch)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    Engine &E = h->E;
+    for (int q = 0; q < (int)E.pa.size(); q++) {
+        int rc = probe_fold(E, q);
+        if (rc) return rc;
+    }
+    if (avg_us) *avg_us = E.probe_n ? 1e3 * E.probe_ms / (double)E.probe_n : 0.0;
+    if (launches) *launches = E.probe_n;
+    if (bytes_per_launch) *bytes_per_launch = E.probe_bytes;
+    return PC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// C ABI: PartConvFilter (one instance, caller-driven)
+// ----------------------------------------------------------------------------
+struct pc_filter {
+    Engine E;
+    int FS = 44100;
+    int M = 0;
+    std::vector<int> N, P;
+    std::vector<int> num_N0, ind_N0, rb;
+    std::vector<unsigned long long> q, bcurr;
+    int start_sync_depth = 0, end_sync_depth = 0;
+    unsigned long long nin = 0;
+};
+
+static void filter_rewind(pc_filter *h)
+{
+    for (int L = 0; L < h->M; L++) {
+        h->ind_N0[L] = h->num_N0[L] - 1;
+        h->q[L] = 0;
+        h->bcurr[L] = 0;
+        h->rb[L] = 0;
+    }
+    h->nin = 0;
+    h->start_sync_depth = h->end_sync_depth = 0;
+}
+
+PC_API pc_filter *pcf_create(int verbosity, int sample_rate)
+{
+    pc_filter *h = new pc_filter();
+    h->E.verbosity = verbosity;
+    h->FS = sample_rate;
+    return h;
+}
+
+PC_API int pcf_setup(pc_filter *h, int M, const int *N, int *num_parts, const float *impulse,
+                     int impulse_len, int fftwOptLevel, const char *wisdom)
+{
+    (void)fftwOptLevel;
+    (void)wisdom;
+    if (!h || M < 1 || !N || !num_parts) return fail(PC_ERR_ARG, "bad argument");
+    InstDesc d;
+    d.N.assign(N, N + M);
+    d.P.assign(num_parts, num_parts + M);
+    if (impulse) {
+        if (impulse_len < 0) return fail(PC_ERR_ARG, "negative impulse length");
+        d.ir.assign(impulse, impulse + impulse_len);
+    } else {
+        long long len = 0;
+        for (int L = 0; L < M; L++) len += (long long)N[L] * num_parts[L];
+        d.ir.assign((size_t)std::max(0ll, len), 1.0f);
+    }
+    std::vector<InstDesc> v{d};
+    h->E.mode = PC_PLAIN;
+    int rc = engine_setup(h->E, h->FS, N[0], 1, 1, v);
+    if (rc) {
+        std::string msg = g_err;
+        engine_free(h->E);
+        g_err = msg;
+        return rc;
+    }
+    const Inst &I = h->E.inst[0];
+    h->M = M;
+    h->N.assign(N, N + M);
+    h->P.resize(M);
+    for (int L = 0; L < M; L++) {
+        h->P[L] = I.lv[L].P;
+        num_parts[L] = I.lv[L].P;
+    }
+    h->num_N0.resize(M);
+    h->ind_N0.resize(M);
+    h->rb.resize(M);
+    h->q.resize(M);
+    h->bcurr.resize(M);
+    for (int L = 0; L < M; L++) h->num_N0[L] = N[L] / N[0];
+    filter_rewind(h);
+    return PC_OK;
+}
+
+static hipStream_t filter_stream(pc_filter *h, int L)
+{
+    if (L == 0) return h->E.s_main;
+    return level_stream(h->E, L);  // global level == local level for one instance
+}
+
+PC_API int pcf_run_level(pc_filter *h, int L)
+{
+    if (!h || !h->E.ready) return fail(PC_ERR_STATE, "runLevel before setup");
+    if (L < 0 || L >= h->M) return fail(PC_ERR_ARG, "level out of range");
+    Engine &E = h->E;
+    if (h->q[L] == 0) return fail(PC_ERR_STATE, "runLevel before any input for this level");
+    const uint64_t m = h->q[L] - 1;
+    const uint64_t b = h->bcurr[L]++;
+    hipStream_t s = filter_stream(h, L);
+    if (L == 0) {
+        int rc = launch_small(E.N0, E.groups[0].d_l0, 1, E.N0, m, b, 0, E.tw, PC_MODE_RUN, s);
+        if (rc) return rc;
+    } else {
+        HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
+        HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
+        for (Batch &bt : E.groups[0].batches) {
+            if (bt.g != L) continue;
+            const int ypar = h->rb[L] ^ 1;
+            if (bt.big) launch_big(bt.sh, bt.d_jobs, 1, E.N0, m, b, ypar, E.tw, PC_MODE_RUN, s);
+            else {
+                int rc = launch_small(bt.N, bt.d_jobs, 1, E.N0, m, b, ypar, E.tw, PC_MODE_RUN, s);
+                if (rc) return rc;
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(E.ev_lvl[L], s));
+    E.lvl_pending[L] = true;
+    return PC_OK;
+}
+
+PC_API int pcf_sync_levels(pc_filter *h)
+{
+    if (!h || !h->E.ready) return fail(PC_ERR_STATE, "sync_levels before setup");
+    h->start_sync_depth = h->end_sync_depth = 0;
+    for (int L = 0; L < h->M; L++) {
+        if (++h->ind_N0[L] == h->num_N0[L]) {
+            h->ind_N0[L] = 0;
+            h->start_sync_depth = L;
+        } else if (h->ind_N0[L] == h->num_N0[L] - 1)
+            h->end_sync_depth = L;
+    }
+    return PC_OK;
+}
+
+PC_API int pcf_sync_levels_relaxed(pc_filter *h)
+{
+    if (!h || !h->E.ready) return fail(PC_ERR_STATE, "sync_levels before setup");
+    h->start_sync_depth = h->end_sync_depth = 0;
+    for (int L = 0; L < h->M; L++) {
+        if (++h->ind_N0[L] == h->num_N0[L]) {
+            h->ind_N0[L] = 0;
+            h->start_sync_depth = L;
+            h->end_sync_depth = L;
+        }
+    }
+    return PC_OK;
+}
+
+PC_API int pcf_start_sync_depth(const pc_filter *h) { return h ? h->start_sync_depth : -1; }
+PC_API int pcf_end_sync_depth(const pc_filter *h) { return h ? h->end_sync_depth : -1; }
+
+PC_API int pcf_input(pc_filter *h, const float *input)
+{
+    if (!h || !input) return fail(PC_ERR_ARG, "null argument");
+    if (!h->E.ready) return fail(PC_ERR_STATE, "input before setup");
+    Engine &E = h->E;
+    HIPCHK(hipStreamSynchronize(E.s_main));  // staging buffer reuse
+    memcpy(E.h_in, input, (size_t)E.N0 * sizeof(float));
+    const Inst &I = E.inst[0];
+    hipLaunchKernelGGL(k_ring_write, dim3((E.N0 + 255) / 256), dim3(256), 0, E.s_main,
+                       reinterpret_cast<float *>(E.arena + I.off_ring), (long long)I.R - 1,
+                       (long long)(h->nin * (unsigned long long)E.N0), E.d_in_map, E.N0);
+    HIPCHK(hipGetLastError());
+    h->nin++;
+    for (int L = 0; L < h->M; L++)
+        if (L <= h->start_sync_depth) h->q[L]++;
+    return PC_OK;
+}
+
+static int filter_output(pc_filter *h, float *y, bool relaxed)
+{
+    if (!h || !y) return fail(PC_ERR_ARG, "null argument");
+    if (!h->E.ready) return fail(PC_ERR_STATE, "output before setup");
+    Engine &E = h->E;
+    const Inst &I = E.inst[0];
+    FilterOut fo{};
+    fo.src[fo.nsrc++] = reinterpret_cast<const float *>(E.arena + I.lv[0].off_ybuf);
+    for (int L = 1; L < h->M; L++) {
+        if (L <= h->end_sync_depth) h->rb[L] ^= 1;
+        int r = relaxed ? h->ind_N0[L] : h->ind_N0[L] + 1;
+        if (r == h->num_N0[L]) r = 0;
+        fo.src[fo.nsrc++] = reinterpret_cast<const float *>(E.arena + I.lv[L].off_ybuf) +
+                            (size_t)h->rb[L] * I.lv[L].N + (size_t)r * E.N0;
+    }
+    for (int L = 1; L < h->M; L++)
+        if (E.lvl_pending[L]) HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[L], 0));
+    hipLaunchKernelGGL(k_filter_out, dim3((E.N0 + 255) / 256), dim3(256), 0, E.s_main, fo, E.N0, E.d_out_map);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    memcpy(y, E.h_out, (size_t)E.N0 * sizeof(float));
+    return PC_OK;
+}
+
+PC_API int pcf_output(pc_filter *h, float *output) { return filter_output(h, output, false); }
+PC_API int pcf_output_relaxed(pc_filter *h, float *output) { return filter_output(h, output, true); }
+
+PC_API int pcf_reset(pc_filter *h)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    int rc = engine_reset(h->E);
+    if (rc) return rc;
+    filter_rewind(h);
+    return PC_OK;
+}
+
+PC_API double pcf_bench(pc_filter *h, int L, int trials, int pollute_every, float max_time,
+                        double *benchData)
+{
+    (void)pollute_every;
+    if (!h || !h->E.ready || L < 0 || L >= h->M || trials < 1) {
+        fail(PC_ERR_ARG, "bench: bad argument or not set up");
+        return -1.0;
+    }
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        fail(PC_ERR_HIP, "bench: event creation failed");
+        return -1.0;
+    }
+    // give the level a block to read
+    for (int L2 = 0; L2 < h->M; L2++)
+        if (h->q[L2] == 0) h->q[L2] = 1;
+    hipStream_t s = filter_stream(h, L);
+    double total = 0.0;
+    int t;
+    for (t = 0; t < trials; t++) {
+        hipEventRecord(a, s);
+        if (pcf_run_level(h, L) != PC_OK) {
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+            return -1.0;
+        }
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        total += 1e-3 * ms;
+        if (total > max_time) {
+            t++;
+            break;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    const double mean = total / (double)t;
+    if (benchData) {
+        const int value = (int)(100.0 * mean / ((double)h->N[L] / (h->FS > 0 ? h->FS : 1)));
+        benchData[2 * L] = value;
+        benchData[2 * L + 1] = value;
+    }
+    if (pcf_reset(h) != PC_OK) return -1.0;
+    return mean;
+}
+
+PC_API int pcf_cleanup(pc_filter *h)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    engine_free(h->E);
+    h->M = 0;
+    return PC_OK;
+}
+
+PC_API void pcf_destroy(pc_filter *h)
+{
+    if (!h) return;
+    engine_free(h->E);
+    delete h;
+}
+
+PC_API int pcf_get_levels(const pc_filter *h, int *M, int *N, int *num_parts, int cap)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    if (M) *M = h->M;
+    for (int L = 0; L < h->M && L < cap; L++) {
+        if (N) N[L] = h->N[L];
+        if (num_parts) num_parts[L] = h->P[L];
+    }
+    return PC_OK;
+}
+
+PC_API int pcf_get_fs(const pc_filter *h) { return h ? h->FS : -1; }
+PC_API int pcf_set_fs(pc_filter *h, int fs)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    h->FS = fs;
+    h->E.FS = fs;
+    return PC_OK;
+}

@@ -1,0 +1,75 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point that
+include/partconv_amd.h declares, and reports errors without a GPU."""
+import ctypes
+import pathlib
+import re
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "partconv_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(pc[mf]?_[a-z_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_expected_api():
+    names = declared_functions()
+    for n in ["pcm_create", "pcm_setup", "pcm_setup_mem", "pcm_run", "pcm_reset", "pcm_cleanup",
+              "pcf_setup", "pcf_run_level", "pcf_input", "pcf_output", "pcf_bench", "pc_last_error"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from partconv_amd import _lib
+    L = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python signature table covers them all
+    assert set(declared_functions()) <= set(_lib.SIGNATURES)
+
+
+def test_errors_are_returned_not_fatal():
+    from partconv_amd import _lib
+    L = _lib.lib()
+    h = L.pcm_create(0, 0)
+    assert h
+    import numpy as np
+    y = np.zeros(64, dtype=np.float32)
+    # run before setup -> PC_ERR_STATE, message set
+    assert L.pcm_run(h, y.ctypes.data, y.ctypes.data, 64) == 6
+    assert "before setup" in _lib.last_error()
+    # bad mode
+    assert L.pcm_set_mode(h, 7) == 1
+    # missing config file -> PC_ERR_CONFIG
+    assert L.pcm_setup(h, b"/nonexistent/pc.config", 1, 0, 0.0) == 2
+    assert "Unable to open config file" in _lib.last_error()
+    L.pcm_destroy(h)
+
+
+def test_config_parse_errors_without_gpu(tmp_path):
+    from partconv_amd import _lib
+    L = _lib.lib()
+    cfg = tmp_path / "bad.config"
+    cfg.write_text("48000 32 1 1 1 1\nx.wav 0 0 0 1.0 2 64 6 256\n")
+    h = L.pcm_create(0, 0)
+    assert L.pcm_setup(h, str(cfg).encode(), 1, 0, 0.0) == 2
+    assert "first block size must equal buffer_size" in _lib.last_error()
+    cfg.write_text("48000 64 1 1 1 1\nmissing.wav 0 0 0 1.0 2 64 6 256\n")
+    assert L.pcm_setup(h, str(cfg).encode(), 1, 0, 0.0) == 4
+    assert "Can't open audio file" in _lib.last_error()
+    L.pcm_destroy(h)
+
+
+def test_python_surface_shapes():
+    import partconv_amd as pc
+    a = pc.intArray(3)
+    a[0], a[1], a[2] = 64, 256, 1024
+    assert [a[i] for i in range(3)] == [64, 256, 1024]
+    d = pc.doubleArray(4)
+    d[3] = 2.5
+    assert d[3] == 2.5

@@ -1,0 +1,214 @@
+"""GPU parity tests: the HIP engine (through the C ABI) against the CPU oracle
+and the float64 direct convolution on the same seeded inputs.
+
+Bars: max |error| <= 1e-5 x output RMS against the fp32 oracle (the
+reference's algorithm restated) and against the fp64 direct convolution.
+"""
+import sys
+import wave
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from partconv_amd import PartConvFilter, PartConvMulti, PartConvMultiRelaxed, last_error
+from partconv_amd.workloads import decaying_noise_ir
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(y, r):
+    return float(np.max(np.abs(np.asarray(y, np.float64) - r)) / np.sqrt(np.mean(np.asarray(r) ** 2)))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_multi(insts, irs, N0, n_in, n_out, relaxed=False, mode="batched", FS=48000):
+    cls = PartConvMultiRelaxed if relaxed else PartConvMulti
+    m = cls(0, mode=mode)
+    d = [dict(ir=h, **i) for i, h in zip(insts, irs)]
+    assert m.setup_mem(FS, N0, n_in, n_out, d) == 0, last_error()
+    return m
+
+
+PARTS = [
+    ([64, 256], [6, 0]),
+    ([64, 256, 2048], [6, 14, 0]),
+    ([32, 128, 512, 2048], [6, 6, 6, 0]),
+    ([64, 512, 4096], [14, 14, 0]),            # first big level (4096 = 64 x 64)
+    ([128, 1024, 8192], [14, 14, 0]),          # 8192 = 128 x 64
+    ([64, 256, 2048, 16384], [6, 14, 14, 0]),  # config-2 partition
+]
+
+
+@pytest.mark.parametrize("N,P", PARTS)
+@pytest.mark.parametrize("relaxed", [False, True])
+def test_single_instance_matches_oracle(N, P, relaxed):
+    say(f"case N={N} P={P} relaxed={relaxed}")
+    rng = np.random.default_rng(sum(N) + relaxed)
+    taps = 2 * N[-1] + sum(n * p for n, p in zip(N[:-1], P[:-1])) + 123
+    h = decaying_noise_ir(taps, taps / 48000, 48000, 5)
+    nfr = 2 * (N[-1] // N[0]) + 40
+    x = rng.standard_normal(nfr * N[0]).astype(np.float32)
+    m = make_multi([dict(N=N, P=P)], [h], N[0], 1, 1, relaxed=relaxed)
+    y = m.process(x)[:, 0]
+    yo = O.OracleMulti(48000, N[0], 1, 1, [dict(N=N, P=P)], [h], relaxed=relaxed).process(x)[:, 0]
+    r = O.direct_conv_f64(x, h)
+    if relaxed:
+        r = np.concatenate([np.zeros(N[0]), r[:-N[0]]])
+    assert rel(y, yo) < TOL
+    assert rel(y, r) < TOL
+
+
+def test_impulse_reads_back_h():
+    say("case impulse")
+    N, P = [64, 256, 2048, 16384], [6, 14, 14, 0]
+    h = decaying_noise_ir(40000, 0.5, 48000, 9)
+    nfr = 800
+    x = np.zeros(nfr * 64, dtype=np.float32)
+    t0 = 64 * 37 + 11
+    x[t0] = 1.0
+    m = make_multi([dict(N=N, P=P)], [h], 64, 1, 1)
+    y = m.process(x)[:, 0]
+    ref = np.zeros_like(y, dtype=np.float64)
+    ref[t0:t0 + h.size] = h[:y.size - t0]
+    assert np.max(np.abs(y - ref)) < 1e-6 * np.max(np.abs(h))
+
+
+def test_multi_instance_mix_mute_scale_uneven():
+    say("case multi mix")
+    rng = np.random.default_rng(3)
+    insts = [
+        dict(N=[64, 256, 2048], P=[6, 14, 0], in_ch=0, out_ch=1, scale=0.5),
+        dict(N=[64, 128, 512, 2048], P=[2, 6, 6, 0], in_ch=1, out_ch=1, scale=2.0),
+        dict(N=[64, 256], P=[6, 0], in_ch=2, out_ch=0),
+        dict(N=[64, 256], P=[6, 0], in_ch=0, out_ch=0, mute=1),
+        dict(N=[64, 1024, 8192], P=[30, 15, 0], in_ch=1, out_ch=2),
+    ]
+    lens = [9000, 12000, 1500, 800, 20000]
+    irs = [decaying_noise_ir(n, n / 48000, 48000, 20 + i) for i, n in enumerate(lens)]
+    nfr = 3 * 8192 // 64
+    x = rng.standard_normal((nfr * 64, 3)).astype(np.float32)
+    ref = O.OracleMulti(48000, 64, 3, 3, insts, irs).process(x)
+    for mode in ("batched", "plain"):
+        m = make_multi(insts, irs, 64, 3, 3, mode=mode)
+        y = m.process(x)
+        for c in range(3):
+            assert rel(y[:, c], ref[:, c]) < TOL, (mode, c)
+    d = O.direct_conv_f64
+    r1 = d(0.5 * x[:, 0].astype(np.float64), irs[0]) + d(2.0 * x[:, 1].astype(np.float64), irs[1])
+    assert rel(y[:, 1], r1) < TOL
+
+
+def test_plain_and_batched_agree_bitwise_close():
+    say("case plain vs batched")
+    rng = np.random.default_rng(4)
+    insts = [dict(N=[64, 256, 2048, 16384], P=[6, 14, 14, 0], in_ch=i, out_ch=i) for i in range(4)]
+    irs = [decaying_noise_ir(40000, 1.0, 48000, 40 + i) for i in range(4)]
+    x = rng.standard_normal((600 * 64, 4)).astype(np.float32)
+    yb = make_multi(insts, irs, 64, 4, 4, mode="batched").process(x)
+    yp = make_multi(insts, irs, 64, 4, 4, mode="plain").process(x)
+    # same kernels, same order of arithmetic per instance
+    assert np.array_equal(yb, yp)
+
+
+def test_filter_api_matches_multi():
+    say("case PartConvFilter API")
+    rng = np.random.default_rng(5)
+    N, P = [64, 256, 2048, 8192], [6, 14, 6, 0]
+    h = decaying_noise_ir(30000, 0.6, 48000, 50)
+    x = rng.standard_normal(64 * 400).astype(np.float32)
+    f = PartConvFilter(0, 48000)
+    assert f.setup(4, N, P, h) == 0
+    assert f.num_parts[-1] == -(-(30000 - (384 + 3584 + 12288)) // 8192)
+    y = f.process(x)
+    yo = O.OracleFilter(N, P, h, "f32").process(x)
+    assert rel(y, yo) < TOL
+    f.reset()
+    y2 = f.process(x, relaxed=True)
+    yr = O.OracleFilter(N, P, h, "f32").process(x, relaxed=True)
+    assert rel(y2, yr) < TOL
+    t = f.bench(1, 20, 0, 0.05)
+    assert 0 < t < 0.05
+    f.cleanup()
+
+
+def test_filter_setup_errors_like_reference():
+    f = PartConvFilter()
+    assert f.setup(3, [64, 1024, 8192], [16, 15, 7], np.ones(65536, np.float32)) == 1
+    assert "sufficient delay" in last_error()
+    assert f.setup(2, [48, 96], [2, 0], np.ones(100, np.float32)) == 1
+
+
+def test_frame_count_mismatch_is_an_error():
+    m = make_multi([dict(N=[64, 256], P=[6, 0])], [np.ones(500, np.float32)], 64, 1, 1)
+    buf = np.zeros(32, np.float32)
+    assert m.run(buf, buf, 32) == 7
+    assert "frameCount" in last_error()
+
+
+def test_reset_replays_identically():
+    say("case reset")
+    rng = np.random.default_rng(6)
+    h = decaying_noise_ir(20000, 0.5, 48000, 60)
+    x = rng.standard_normal(64 * 300).astype(np.float32)
+    m = make_multi([dict(N=[64, 256, 4096], P=[6, 30, 0])], [h], 64, 1, 1)
+    y1 = m.process(x)
+    m.reset()
+    y2 = m.process(x)
+    assert np.array_equal(y1, y2)
+    assert m.frameNum == 300
+
+
+def test_config_file_with_wav(tmp_path):
+    say("case config file")
+    rng = np.random.default_rng(7)
+    h = decaying_noise_ir(6000, 0.1, 48000, 70)
+    pcm = np.clip(np.round(h / np.max(np.abs(h)) * 0.9 * 32767), -32768, 32767).astype("<i2")
+    st = np.stack([pcm, -pcm], axis=1)   # stereo: the reader takes channel 0
+    wp = tmp_path / "ir.wav"
+    with wave.open(str(wp), "wb") as w:
+        w.setnchannels(2)
+        w.setsampwidth(2)
+        w.setframerate(48000)
+        w.writeframes(st.tobytes())
+    cfg = tmp_path / "pc.config"
+    cfg.write_text("48000 64 1 1 1 2\nir.wav 0 0 0 0.25 3 64 6 256 14 2048 5\n# comment ignored\n")
+    m = PartConvMulti(0)
+    assert m.setup(str(cfg)) == 0, last_error()
+    assert m.info().numPCs == 2       # one repeat, muted
+    assert m.instance_partition(0) == ([64, 256, 2048], [6, 14, 1])
+    hf = pcm.astype(np.float32) / 32768.0
+    x = rng.standard_normal(64 * 200).astype(np.float32)
+    y = m.process(x)[:, 0]
+    r = O.direct_conv_f64(0.25 * x.astype(np.float64), hf)
+    assert rel(y, r) < TOL
+
+
+def test_device_resident_run_matches_host_run():
+    say("case device-resident")
+    import torch
+    rng = np.random.default_rng(8)
+    insts = [dict(N=[64, 256, 2048, 16384], P=[6, 14, 14, 0], in_ch=i, out_ch=i) for i in range(3)]
+    irs = [decaying_noise_ir(50000, 1.0, 48000, 80 + i) for i in range(3)]
+    nfr = 520
+    x = rng.standard_normal((nfr * 64, 3)).astype(np.float32)
+    y_host = make_multi(insts, irs, 64, 3, 3).process(x)
+    m = make_multi(insts, irs, 64, 3, 3)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros((nfr * 64, 3), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert m.run_device(yd.data_ptr(), xd.data_ptr(), nfr, True) == 0, last_error()
+    assert np.array_equal(yd.cpu().numpy(), y_host)
+    m.reset()
+    yd.zero_()
+    torch.cuda.synchronize()
+    assert m.run_device(yd.data_ptr(), xd.data_ptr(), nfr, False) == 0, last_error()
+    assert np.array_equal(yd.cpu().numpy(), y_host)
+    st = m.stats()
+    assert st.ticks == nfr and st.p99_us > 0
