@@ -63,6 +63,7 @@ typedef struct pc_stats {
     unsigned long long deadline_misses; /* ticks whose latency exceeded the block period */
     double p50_us, p99_us, max_us, mean_us;
     double period_us;                   /* buffer_size / FS */
+    unsigned long long persistent_ticks; /* ticks served by the persistent tick kernel */
 } pc_stats;
 
 /* One instance of an in-memory setup: the fields of one config line
@@ -123,10 +124,16 @@ int pcm_run(pc_multi *h, float *output, const float *input, int frameCount);
 
 /* Device-resident ticks: runs nframes consecutive ticks reading frame t from
  * d_in + t*buffer_size*nIn and writing d_out + t*buffer_size*nOut (device
- * pointers, interleaved).  sync_each_tick != 0 waits for every tick's output
- * (real-time semantics, per-tick latency recorded); 0 queues all ticks and
- * waits once at the end.  No reference counterpart (the reference only has the
- * host callback); the tick semantics are those of pcm_run. */
+ * pointers, interleaved).  sync_each_tick:
+ *   0  queue every tick, wait once at the end;
+ *   1  one tick-kernel launch per tick, wait for each tick's output (real-time
+ *      semantics, per-tick latency recorded);
+ *   2  as 1, but one persistent tick kernel serves all ticks: the host
+ *      releases each tick through pinned memory instead of launching it
+ *      (batched engine; falls back to 1 if the kernel cannot be made resident
+ *      or a level stream stalls beside it).
+ * No reference counterpart (the reference only has the host callback); the
+ * tick semantics are those of pcm_run. */
 int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long long nframes,
                    int sync_each_tick);
 
@@ -152,15 +159,16 @@ int pcm_instance_partition(const pc_multi *h, int inst, int *N, int *P, int cap)
 
 /* Kernel probe (instrumentation, no reference counterpart; the reference's
  * nearest is the per-stage Timer of its older engine,
- * src/partconvMulti_cuda.cpp:1679-1818): from now on, time with HIP events the
- * multiply-accumulate kernel of global level `global_level` (0 = the tick
- * kernel, which carries level 0) each time it launches, on the stream it runs
- * on.  -1 disables. */
+ * src/partconvMulti_cuda.cpp:1679-1818): from now on, time with HIP events
+ * stamped by the launch itself (hipExtLaunchKernelGGL) the delay-line
+ * partial-sum kernel of global level `global_level` (0 = the tick kernel,
+ * which carries level 0) each time it launches, on the stream it runs on.
+ * -1 disables. */
 int pcm_probe(pc_multi *h, int global_level);
 /* Average probed kernel duration (us), number of launches timed, and the
- * algorithmic HBM bytes one launch moves: per job 8(N+1)(2P+1) (filter
- * spectra + delay-line rows + new slot), plus 12*N0 per instance for the tick
- * kernel (window read, output write). */
+ * algorithmic HBM bytes one launch moves: per job 8N(2P-1) for the partial
+ * sum (P-1 delay-line rows + P-1 filter rows read, one row written); per
+ * instance 8(N0+1)(2P0+1) + 12*N0 for the tick kernel. */
 int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, double *bytes_per_launch);
 
 /* ----------------------------------------------------------------------------

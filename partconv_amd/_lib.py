@@ -28,7 +28,8 @@ class PcMultiInfo(ctypes.Structure):
 class PcStats(ctypes.Structure):
     _fields_ = [("ticks", ctypes.c_ulonglong), ("deadline_misses", ctypes.c_ulonglong),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double),
-                ("mean_us", ctypes.c_double), ("period_us", ctypes.c_double)]
+                ("mean_us", ctypes.c_double), ("period_us", ctypes.c_double),
+                ("persistent_ticks", ctypes.c_ulonglong)]
 
 
 class PcInstance(ctypes.Structure):

@@ -11,6 +11,7 @@
 //     f_s + k - 1; relaxed: tick f_s + k), enforced with hipStreamWaitEvent;
 //   * a host-visible completion flag that the tick's last workgroup raises, so
 //     a tick costs one launch plus a spin on pinned memory.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -145,7 +146,7 @@ struct LevelSpec {
     int N, P, T, delay, kshift;
     bool big;
     BigShape sh;
-    size_t off_H, off_fdl, off_ybuf, off_scr;
+    size_t off_H, off_fdl, off_ybuf, off_scr, off_S;
 };
 
 struct InstDesc {
@@ -211,6 +212,8 @@ struct Engine {
     unsigned long long *h_flag = nullptr;
     float *d_in_map = nullptr, *d_out_map = nullptr;   // device views of the pinned buffers
     unsigned long long *d_flag_map = nullptr;
+    unsigned long long *h_wg = nullptr, *d_wg_map = nullptr;  // per-workgroup flags (direct mix)
+    int nwg = 0;
     uint64_t frame = 0;
     // kernel probe: HIP events around one kernel of one global level
     int probe_g = -1;
@@ -251,6 +254,8 @@ static void engine_free(Engine &E)
     if (E.h_in) hipHostFree(E.h_in);
     if (E.h_out) hipHostFree(E.h_out);
     if (E.h_flag) hipHostFree(E.h_flag);
+    if (E.h_wg) hipHostFree(E.h_wg);
+    E.h_wg = E.d_wg_map = nullptr;
     E.h_in = E.h_out = nullptr;
     E.h_flag = nullptr;
     for (hipEvent_t e : E.ev_lvl) hipEventDestroy(e);
@@ -272,26 +277,32 @@ static void engine_free(Engine &E)
 }
 
 // ---- kernel dispatch --------------------------------------------------------
+// Optional launch decorations: HIP events bound to the kernel's own start/stop
+// (hipExtLaunchKernelGGL) and a level-completion counter.
+struct LaunchExtra {
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
 template <int LOGN>
 static void launch_small_t(const LevelJob *jobs, int njobs, int N0, uint64_t m, uint64_t bcur,
-                           int ypar, const cf *tw, int mode, hipStream_t s)
+                           int ypar, const cf *tw, int mode, hipStream_t s, const LaunchExtra &x)
 {
-    hipLaunchKernelGGL(k_level_small<LOGN>, dim3(njobs), dim3(SmallCfg<LOGN>::NT), 0, s, jobs, N0, m,
-                       bcur, ypar, tw, mode);
+    hipExtLaunchKernelGGL(k_level_small<LOGN>, dim3(njobs), dim3(SmallCfg<LOGN>::NT), 0, s, x.ev0, x.ev1, 0,
+                          jobs, N0, m, bcur, ypar, tw, mode);
 }
 
 static int launch_small(int N, const LevelJob *jobs, int njobs, int N0, uint64_t m, uint64_t bcur,
-                        int ypar, const cf *tw, int mode, hipStream_t s)
+                        int ypar, const cf *tw, int mode, hipStream_t s, const LaunchExtra &x = LaunchExtra())
 {
     switch (ilog2(N)) {
-    case 4: launch_small_t<4>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 5: launch_small_t<5>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 6: launch_small_t<6>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 7: launch_small_t<7>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 8: launch_small_t<8>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 9: launch_small_t<9>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 10: launch_small_t<10>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
-    case 11: launch_small_t<11>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s); break;
+    case 4: launch_small_t<4>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 5: launch_small_t<5>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 6: launch_small_t<6>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 7: launch_small_t<7>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 8: launch_small_t<8>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 9: launch_small_t<9>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 10: launch_small_t<10>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
+    case 11: launch_small_t<11>(jobs, njobs, N0, m, bcur, ypar, tw, mode, s, x); break;
     default: return fail(PC_ERR_ARG, "unsupported small block size");
     }
     return PC_OK;
@@ -300,36 +311,49 @@ static int launch_small(int N, const LevelJob *jobs, int njobs, int N0, uint64_t
 static size_t big1_lds(BigShape sh) { return (size_t)2 * BIG_G * ((1 << sh.logB) + 1) * sizeof(cf); }
 static size_t big2_lds(BigShape sh) { return (size_t)2 * 2 * ((1 << sh.logA) + 1) * sizeof(cf); }
 
+// Four-step level job: k_big1 -> k_big2 (multiply-accumulate; optional
+// events) -> k_big3 (optional level-completion counter).
 static void launch_big(BigShape sh, const LevelJob *jobs, int njobs, int N0, uint64_t m,
-                       uint64_t bcur, int ypar, const cf *tw, int mode, hipStream_t s)
+                       uint64_t bcur, int ypar, const cf *tw, int mode, hipStream_t s,
+                       const LaunchExtra &x = LaunchExtra())
 {
     const int A = 1 << sh.logA, B = 1 << sh.logB;
     hipLaunchKernelGGL(k_big1, dim3((A / BIG_G) * njobs), dim3(BIG_NT), big1_lds(sh), s, jobs, sh, N0,
                        m, tw, mode);
-    hipLaunchKernelGGL(k_big2, dim3((B / 2) * njobs), dim3(BIG_NT), big2_lds(sh), s, jobs, sh, bcur,
-                       tw, mode);
+    hipExtLaunchKernelGGL(k_big2, dim3((B / 2) * njobs), dim3(BIG_NT), (uint32_t)big2_lds(sh), s, x.ev0,
+                          x.ev1, 0, jobs, sh, bcur, tw, mode);
     if (mode == PC_MODE_RUN)
         hipLaunchKernelGGL(k_big3, dim3((A / BIG_G) * njobs), dim3(BIG_NT), big1_lds(sh), s, jobs, sh,
                            ypar, tw);
 }
 
-template <int LOGN>
-static void launch_tick_t(const TickArgs &a, const cf *tw, hipStream_t s)
+// Partial sum S for block b of every job of a batch (k_partial).
+static void launch_partial(int N, const LevelJob *jobs, int njobs, uint64_t b, hipStream_t s,
+                           const LaunchExtra &x = LaunchExtra())
 {
-    hipLaunchKernelGGL(k_tick<LOGN>, dim3(a.njobs), dim3(SmallCfg<LOGN>::NT), 0, s, a, tw);
+    const int wpj = (N / 2 + PART_NT - 1) / PART_NT;
+    hipExtLaunchKernelGGL(k_partial, dim3(wpj * njobs), dim3(PART_NT), 0, s, x.ev0, x.ev1, 0, jobs, N, wpj,
+                          b);
 }
 
-static int launch_tick(int N0, const TickArgs &a, const cf *tw, hipStream_t s)
+template <int LOGN>
+static void launch_tick_t(const TickArgs &a, const cf *tw, hipStream_t s, const LaunchExtra &x)
+{
+    hipExtLaunchKernelGGL(k_tick<LOGN>, dim3(a.njobs), dim3(TickCfg<LOGN>::NT), 0, s, x.ev0, x.ev1, 0, a, tw);
+}
+
+static int launch_tick(int N0, const TickArgs &a, const cf *tw, hipStream_t s,
+                       const LaunchExtra &x = LaunchExtra())
 {
     switch (ilog2(N0)) {
-    case 4: launch_tick_t<4>(a, tw, s); break;
-    case 5: launch_tick_t<5>(a, tw, s); break;
-    case 6: launch_tick_t<6>(a, tw, s); break;
-    case 7: launch_tick_t<7>(a, tw, s); break;
-    case 8: launch_tick_t<8>(a, tw, s); break;
-    case 9: launch_tick_t<9>(a, tw, s); break;
-    case 10: launch_tick_t<10>(a, tw, s); break;
-    case 11: launch_tick_t<11>(a, tw, s); break;
+    case 4: launch_tick_t<4>(a, tw, s, x); break;
+    case 5: launch_tick_t<5>(a, tw, s, x); break;
+    case 6: launch_tick_t<6>(a, tw, s, x); break;
+    case 7: launch_tick_t<7>(a, tw, s, x); break;
+    case 8: launch_tick_t<8>(a, tw, s, x); break;
+    case 9: launch_tick_t<9>(a, tw, s, x); break;
+    case 10: launch_tick_t<10>(a, tw, s, x); break;
+    case 11: launch_tick_t<11>(a, tw, s, x); break;
     default: return fail(PC_ERR_ARG, "unsupported buffer_size");
     }
     return PC_OK;
@@ -449,6 +473,8 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
             off = align_up(off + (size_t)2 * s.N * sizeof(float));
             s.off_scr = off;
             if (s.big) off = align_up(off + (size_t)s.N * sizeof(cf));
+            s.off_S = off;
+            off = align_up(off + (size_t)s.N * sizeof(cf));
         }
     }
     E.arena_bytes = off;
@@ -536,6 +562,7 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         J.H = reinterpret_cast<const cf *>(E.arena + s.off_H);
         J.ybuf = reinterpret_cast<float *>(E.arena + s.off_ybuf);
         J.scratch = s.big ? reinterpret_cast<cf *>(E.arena + s.off_scr) : nullptr;
+        J.S = reinterpret_cast<cf *>(E.arena + s.off_S);
         J.P = s.P;
         J.T = s.T;
         J.ring_mask = I.R - 1;
@@ -608,6 +635,7 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         HIPCHK(hipMemcpy(E.d_src_list, src_list.data(), src_list.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&E.d_ticket, 64));
     HIPCHK(hipMemset(E.d_ticket, 0, 64));
+
     // ---- streams, events, pinned staging
     const int nls = std::min(PC_LEVEL_STREAMS, std::max(0, G - 1));
     for (int q = 0; q < nls; q++) {
@@ -629,6 +657,13 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_in_map, E.h_in, 0));
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_out_map, E.h_out, 0));
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_flag_map, E.h_flag, 0));
+    E.nwg = (E.mode == PC_BATCHED) ? (int)E.inst.size() : 0;
+    if (E.mix == 1) {
+        HIPCHK(hipHostMalloc(&E.h_wg, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long),
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        memset(E.h_wg, 0, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long));
+        HIPCHK(hipHostGetDevicePointer((void **)&E.d_wg_map, E.h_wg, 0));
+    }
     memset(E.h_in, 0, (size_t)N0 * nIn * sizeof(float));
     E.frame = 0;
     E.lat.assign(1 << 20, 0.0);
@@ -662,12 +697,15 @@ static int engine_reset(Engine &E)
         for (LevelSpec &s : I.lv) {
             HIPCHK(hipMemsetAsync(E.arena + s.off_fdl, 0, (size_t)s.P * s.N * sizeof(cf), E.s_main));
             HIPCHK(hipMemsetAsync(E.arena + s.off_ybuf, 0, (size_t)2 * s.N * sizeof(float), E.s_main));
+            HIPCHK(hipMemsetAsync(E.arena + s.off_S, 0, (size_t)s.N * sizeof(cf), E.s_main));
         }
     }
+    HIPCHK(hipMemsetAsync(E.d_ticket, 0, 64, E.s_main));
     HIPCHK(hipStreamSynchronize(E.s_main));
     E.frame = 0;
     std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
     *E.h_flag = 0;
+    if (E.h_wg) memset(E.h_wg, 0, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long));
     return PC_OK;
 }
 
@@ -685,18 +723,40 @@ static void record_latency(Engine &E, double us)
     if (period > 0 && us > period) E.misses++;
 }
 
-static int wait_flag(Engine &E, unsigned long long target)
+// Completion of tick target-1: every per-workgroup flag (direct mix) or the
+// single flag raised by the last workgroup / the mix kernel.
+static bool tick_done(const Engine &E, unsigned long long target)
 {
-    volatile unsigned long long *fl = E.h_flag;
-    if (*fl >= target) return PC_OK;
+    if (E.mix == 1) {
+        const volatile unsigned long long *w = E.h_wg;
+        for (int i = 0; i < E.nwg; i++)
+            if (w[i] < target) return false;
+        return true;
+    }
+    return *(const volatile unsigned long long *)E.h_flag >= target;
+}
+
+// Spin until tick target-1 completes.  stall_ok: return 1 (instead of an
+// error) when the tick stream has drained without completing it, or after the
+// timeout, so the persistent runner can fall back.
+static int wait_flag(Engine &E, unsigned long long target, bool stall_ok = false)
+{
+    if (tick_done(E, target)) return PC_OK;
     const double t0 = now_s();
     unsigned spins = 0;
-    while (*fl < target) {
-        if ((++spins & 0xFFFF) == 0) {
+    while (!tick_done(E, target)) {
+        if ((++spins & 0x3FFF) == 0) {
             hipError_t q = hipStreamQuery(E.s_main);
             if (q != hipSuccess && q != hipErrorNotReady)
                 return fail(PC_ERR_HIP, std::string("tick failed: ") + hipGetErrorString(q));
-            if (now_s() - t0 > PC_TICK_TIMEOUT_S) return fail(PC_ERR_TIMEOUT, "tick did not complete");
+            if (q == hipSuccess && !tick_done(E, target)) {
+                if (stall_ok) return 1;
+                return fail(PC_ERR_STATE, "tick stream drained without completing the tick");
+            }
+            if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
+                if (stall_ok) return 1;
+                return fail(PC_ERR_TIMEOUT, "tick did not complete");
+            }
         }
     }
     return PC_OK;
@@ -715,21 +775,18 @@ static int probe_fold(Engine &E, int q)
     return PC_OK;
 }
 
-static int probe_begin(Engine &E, hipStream_t s, int *slot)
+// Reserve an event pair; the kernel launch itself stamps it
+// (hipExtLaunchKernelGGL start/stop events), so the measured time is the
+// kernel's own execution.
+static int probe_take(Engine &E, LaunchExtra &x)
 {
     const int q = E.pnext;
     E.pnext = (E.pnext + 1) % (int)E.pa.size();
     int rc = probe_fold(E, q);
     if (rc) return rc;
-    HIPCHK(hipEventRecord(E.pa[q], s));
-    *slot = q;
-    return PC_OK;
-}
-
-static int probe_end(Engine &E, hipStream_t s, int slot)
-{
-    HIPCHK(hipEventRecord(E.pb[slot], s));
-    E.pused[slot] = true;
+    x.ev0 = E.pa[q];
+    x.ev1 = E.pb[q];
+    E.pused[q] = true;
     return PC_OK;
 }
 
@@ -737,26 +794,59 @@ static int probe_end(Engine &E, hipStream_t s, int slot)
 // multiply-accumulate kernel (k_big2, or the fused small kernel).
 static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
 {
-    const bool probe = (E.probe_g == b.g);
-    int slot = -1, rc;
-    if (b.big) {
-        const BigShape sh = b.sh;
-        const int A = 1 << sh.logA, B = 1 << sh.logB;
-        hipLaunchKernelGGL(k_big1, dim3((A / BIG_G) * b.njobs), dim3(BIG_NT), big1_lds(sh), s, b.d_jobs, sh,
-                           E.N0, m, E.tw, (int)PC_MODE_RUN);
-        if (probe && (rc = probe_begin(E, s, &slot))) return rc;
-        hipLaunchKernelGGL(k_big2, dim3((B / 2) * b.njobs), dim3(BIG_NT), big2_lds(sh), s, b.d_jobs, sh, m,
-                           E.tw, (int)PC_MODE_RUN);
-        if (probe && (rc = probe_end(E, s, slot))) return rc;
-        hipLaunchKernelGGL(k_big3, dim3((A / BIG_G) * b.njobs), dim3(BIG_NT), big1_lds(sh), s, b.d_jobs, sh,
-                           (int)(m & 1), E.tw);
-    } else {
-        if (probe && (rc = probe_begin(E, s, &slot))) return rc;
-        rc = launch_small(b.N, b.d_jobs, b.njobs, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
+    LaunchExtra x;
+    if (E.probe_g == b.g) {
+        int rc = probe_take(E, x);
         if (rc) return rc;
-        if (probe && (rc = probe_end(E, s, slot))) return rc;
+    }
+    if (b.big) launch_big(b.sh, b.d_jobs, b.njobs, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
+    else {
+        int rc = launch_small(b.N, b.d_jobs, b.njobs, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
+        if (rc) return rc;
+    }
+    // the next block's delay-line sum, off its critical path
+    launch_partial(b.N, b.d_jobs, b.njobs, m + 1, s, x);
+    return PC_OK;
+}
+
+// Launch the level blocks whose period starts at tick f.  With use_event the
+// level streams wait for the tick kernel of f on the device; without it the
+// caller has already observed tick f's completion on the host.
+static int start_levels(Engine &E, uint64_t f, bool use_event)
+{
+    const int G = (int)E.gsize.size();
+    bool recorded = false;
+    for (int g = 1; g < G; g++) {
+        const uint64_t k = (uint64_t)E.gsize[g];
+        if (f % k != 0) continue;
+        hipStream_t s = level_stream(E, g);
+        if (use_event) {
+            if (!recorded) {
+                HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
+                recorded = true;
+            }
+            HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
+        }
+        const uint64_t m = f >> E.gshift[g];
+        for (Group &Gr : E.groups)
+            for (Batch &b : Gr.batches) {
+                if (b.g != g) continue;
+                int rc = launch_batch(E, b, m, s);
+                if (rc) return rc;
+            }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E.ev_lvl[g], s));
+        E.lvl_pending[g] = true;
     }
     return PC_OK;
+}
+
+// True when level g's block that tick f reads for the first time must be
+// complete before tick f runs (strict: f+1 = 0 mod k; relaxed: f = 0 mod k).
+static bool level_due(const Engine &E, int g, uint64_t f)
+{
+    const uint64_t k = (uint64_t)E.gsize[g];
+    return E.relaxed ? (f % k == 0 && f >= k) : ((f + 1) % k == 0 && f + 1 >= k);
 }
 
 // One tick of PartConvMulti[Relaxed]::run on the device.
@@ -768,9 +858,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     const int G = (int)E.gsize.size();
     // deadlines: wait for the blocks this tick reads for the first time
     for (int g = 1; g < G; g++) {
-        const uint64_t k = (uint64_t)E.gsize[g];
-        const bool due = E.relaxed ? (f % k == 0 && f >= k) : ((f + 1) % k == 0 && f + 1 >= k);
-        if (due && E.lvl_pending[g]) {
+        if (level_due(E, g, f) && E.lvl_pending[g]) {
             HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
             E.lvl_pending[g] = false;
         }
@@ -796,14 +884,14 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         a.mix = E.mix;
         a.y_inst = E.y_inst;
         a.done_flag = E.d_flag_map;
-        int slot = -1;
+        a.wg_flags = E.d_wg_map;
+        LaunchExtra x;
         if (E.probe_g == 0) {
-            int rc = probe_begin(E, E.s_main, &slot);
+            int rc = probe_take(E, x);
             if (rc) return rc;
         }
-        int rc = launch_tick(E.N0, a, E.tw, E.s_main);
+        int rc = launch_tick(E.N0, a, E.tw, E.s_main, x);
         if (rc) return rc;
-        if (slot >= 0 && (rc = probe_end(E, E.s_main, slot))) return rc;
     } else {
         for (Group &Gr : E.groups) {
             a.jobs = Gr.d_tick;
@@ -821,32 +909,141 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     }
     HIPCHK(hipGetLastError());
     // start the levels whose period begins at this tick (src/partconvMulti.cpp:860-873)
-    bool recorded = false;
-    for (int g = 1; g < G; g++) {
-        const uint64_t k = (uint64_t)E.gsize[g];
-        if (f % k != 0) continue;
-        if (!recorded) {
-            HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
-            recorded = true;
-        }
-        hipStream_t s = level_stream(E, g);
-        HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
-        const uint64_t m = f >> E.gshift[g];
-        for (Group &Gr : E.groups)
-            for (Batch &b : Gr.batches) {
-                if (b.g != g) continue;
-                int rc = launch_batch(E, b, m, s);
-                if (rc) return rc;
-            }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(E.ev_lvl[g], s));
-        E.lvl_pending[g] = true;
-    }
+    int rc = start_levels(E, f, true);
+    if (rc) return rc;
     E.frame++;
     if (wait) {
         int rc = wait_flag(E, f + 1);
         if (rc) return rc;
         record_latency(E, 1e6 * (now_s() - t0));
+    }
+    return PC_OK;
+}
+
+template <int LOGN>
+static hipError_t launch_persist_t(const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks)
+{
+    PersistArgs pa = p;
+    const cf *t = tw;
+    void *args[] = {(void *)&pa, (void *)&t};
+    return hipLaunchCooperativeKernel((const void *)k_persist<LOGN>, dim3(nblocks), dim3(TickCfg<LOGN>::NT),
+                                      args, 0, s);
+}
+
+static hipError_t launch_persist(int N0, const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks)
+{
+    switch (ilog2(N0)) {
+    case 4: return launch_persist_t<4>(p, tw, s, nblocks);
+    case 5: return launch_persist_t<5>(p, tw, s, nblocks);
+    case 6: return launch_persist_t<6>(p, tw, s, nblocks);
+    case 7: return launch_persist_t<7>(p, tw, s, nblocks);
+    case 8: return launch_persist_t<8>(p, tw, s, nblocks);
+    case 9: return launch_persist_t<9>(p, tw, s, nblocks);
+    case 10: return launch_persist_t<10>(p, tw, s, nblocks);
+    case 11: return launch_persist_t<11>(p, tw, s, nblocks);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static void store_go(Engine &E, unsigned long long v)
+{
+    __atomic_store_n(E.h_flag + 1, v, __ATOMIC_RELEASE);
+}
+
+// Device-resident ticks served by one persistent tick kernel (batched engine).
+// Per tick the host (1) checks, with hipEventQuery, that the level blocks due
+// at this tick are complete, (2) releases the tick, (3) spins on the
+// completion flag, (4) launches the level blocks that start at this tick.
+// If anything stalls (a level stream that cannot run beside the persistent
+// kernel, a kernel that exited) the host stops the kernel and finishes the
+// remaining ticks with per-tick launches, so output is never wrong or late
+// beyond the stall itself.
+static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, long long nframes,
+                                 long long *persistent_ticks)
+{
+    const uint64_t f0 = E.frame;
+    const long long in_stride = (long long)E.N0 * E.nIn, out_stride = (long long)E.N0 * E.nOut;
+    *persistent_ticks = 0;
+    PersistArgs P{};
+    TickArgs &a = P.A;
+    Group &Gr = E.groups[0];
+    a.jobs = Gr.d_tick;
+    a.njobs = (int)Gr.members.size();
+    a.N0 = E.N0;
+    a.in = d_in;
+    a.in_stride = in_stride;
+    a.fbase = (long long)f0;
+    a.nIn = E.nIn;
+    a.out = d_out;
+    a.out_stride = out_stride;
+    a.nOut = E.nOut;
+    a.relaxed = E.relaxed;
+    a.mix = E.mix;
+    a.y_inst = E.y_inst;
+    a.src_off = E.d_src_off;
+    a.src_list = E.d_src_list;
+    a.ticket = E.d_ticket;
+    a.done_flag = E.d_flag_map;
+    a.wg_flags = E.d_wg_map;
+    P.f0 = f0;
+    P.nticks = nframes;
+    P.go = E.d_flag_map + 1;
+    P.spin_limit = 200000000ull;  // 2 s at 100 MHz
+    // the levels the first tick reads must be complete before the kernel owns the tick stream
+    const int G = (int)E.gsize.size();
+    for (int g = 1; g < G; g++)
+        if (E.lvl_pending[g]) HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
+    store_go(E, (unsigned long long)f0);
+    hipError_t le = launch_persist(E.N0, P, E.tw, E.s_main, a.njobs);
+    if (le != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;  // caller falls back to per-tick launches
+    }
+    long long t = 0;
+    bool stalled = false;
+    for (; t < nframes; t++) {
+        const uint64_t f = f0 + (uint64_t)t;
+        const double t0 = now_s();
+        for (int g = 1; g < G && !stalled; g++) {
+            if (!(level_due(E, g, f) && E.lvl_pending[g])) continue;
+            for (;;) {
+                hipError_t q = hipEventQuery(E.ev_lvl[g]);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
+                if (now_s() - t0 > 0.2) {
+                    stalled = true;
+                    break;
+                }
+            }
+            if (!stalled) E.lvl_pending[g] = false;
+        }
+        if (stalled) break;
+        store_go(E, f + 1);
+        int wr = wait_flag(E, f + 1, true);
+        if (wr == 1) stalled = true;
+        else if (wr) return wr;
+        if (stalled) break;
+        record_latency(E, 1e6 * (now_s() - t0));
+        E.frame++;
+        int rc = start_levels(E, f, false);
+        if (rc) return rc;
+    }
+    if (stalled) store_go(E, PC_GO_STOP);
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    if (stalled && t < nframes && tick_done(E, f0 + (uint64_t)t + 1)) {
+        // the kernel served the tick after all (slow, not stalled): account for it
+        const uint64_t f = f0 + (uint64_t)t;
+        E.frame++;
+        int rc = start_levels(E, f, true);
+        if (rc) return rc;
+        t++;
+    }
+    *persistent_ticks = t;
+    if (stalled) {
+        for (; t < nframes; t++) {
+            int rc = engine_tick(E, d_in, in_stride, (long long)f0, d_out, out_stride, true);
+            if (rc) return rc;
+        }
     }
     return PC_OK;
 }
@@ -864,6 +1061,7 @@ static int engine_drain(Engine &E)
 struct pc_multi {
     Engine E;
     int max_block = 0;
+    long long persistent_ticks = 0;   // ticks served by the persistent kernel
 };
 
 PC_API const char *pc_last_error(void) { return g_err.c_str(); }
@@ -982,6 +1180,12 @@ PC_API int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long lon
     if (!h || !d_out || !d_in || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
     Engine &E = h->E;
     if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    if (sync_each_tick == 2 && E.mode == PC_BATCHED && nframes > 0) {
+        long long served = 0;
+        int rc = engine_run_persistent(E, d_in, d_out, nframes, &served);
+        h->persistent_ticks += served;
+        if (rc != 1) return rc;  // 1: persistent launch refused -> per-tick launches below
+    }
     const long long fbase = (long long)E.frame;
     for (long long t = 0; t < nframes; t++) {
         int rc = engine_tick(E, d_in, (long long)E.N0 * E.nIn, fbase, d_out, (long long)E.N0 * E.nOut,
@@ -1037,6 +1241,7 @@ PC_API int pcm_get_stats(const pc_multi *h, pc_stats *st)
     st->deadline_misses = E.misses;
     st->period_us = E.FS > 0 ? 1e6 * E.N0 / E.FS : 0.0;
     st->max_us = E.lat_max;
+    st->persistent_ticks = (unsigned long long)h->persistent_ticks;
     st->mean_us = E.ticks ? E.lat_sum / (double)E.ticks : 0.0;
     const size_t n = E.lat_wrap ? E.lat.size() : E.lat_pos;
     if (n) {
@@ -1060,6 +1265,7 @@ PC_API int pcm_clear_stats(pc_multi *h)
     E.lat_wrap = false;
     E.ticks = E.misses = 0;
     E.lat_sum = E.lat_max = 0;
+    h->persistent_ticks = 0;
     return PC_OK;
 }
 
@@ -1109,16 +1315,18 @@ PC_API int pcm_probe(pc_multi *h, int global_level)
     E.probe_ms = 0;
     E.probe_n = 0;
     E.probe_bytes = 0;
-    // algorithmic bytes per launch: per job 8(N+1)(2P+1) for the multiply-
-    // accumulate (H + FDL rows + slot write); the tick kernel adds the window
-    // and output of level 0 (+12 N0) per instance
+    // algorithmic bytes per launch.  Level >= 1 (k_partial): per job the P-1
+    // delay-line rows and P-1 filter rows it reads plus the S row it writes,
+    // 8N(2P-1).  Tick kernel (level 0): per instance the level-block figure
+    // 8(N0+1)(2P0+1) + 12 N0 (its phase B streams the delay line and filter
+    // rows; phase A reads the window, H_0 and S and writes the output).
     if (global_level >= 0) {
         for (Inst &I : E.inst) {
             const int L = I.gmap[global_level];
             if (L < 0) continue;
             const LevelSpec &s = I.lv[L];
-            E.probe_bytes += 8.0 * (s.N + 1) * (2.0 * s.P + 1);
-            if (global_level == 0) E.probe_bytes += 12.0 * s.N;
+            if (global_level == 0) E.probe_bytes += 8.0 * (s.N + 1) * (2.0 * s.P + 1) + 12.0 * s.N;
+            else E.probe_bytes += 8.0 * s.N * (2.0 * s.P - 1);
         }
     }
     return PC_OK;
@@ -1234,6 +1442,7 @@ PC_API int pcf_run_level(pc_filter *h, int L)
     if (L == 0) {
         int rc = launch_small(E.N0, E.groups[0].d_l0, 1, E.N0, m, b, 0, E.tw, PC_MODE_RUN, s);
         if (rc) return rc;
+        launch_partial(E.N0, E.groups[0].d_l0, 1, b + 1, s);
     } else {
         HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
         HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
@@ -1245,6 +1454,7 @@ PC_API int pcf_run_level(pc_filter *h, int L)
                 int rc = launch_small(bt.N, bt.d_jobs, 1, E.N0, m, b, ypar, E.tw, PC_MODE_RUN, s);
                 if (rc) return rc;
             }
+            launch_partial(bt.N, bt.d_jobs, 1, b + 1, s);
         }
     }
     HIPCHK(hipGetLastError());

@@ -37,10 +37,10 @@ struct LevelJob {
     const cf *H;         // P x N filter spectra
     float *ybuf;         // 2 x N output double buffer
     cf *scratch;         // big levels: N float2 work space
+    cf *S;               // N float2: delay-line sum of partitions 1..P-1 for the next block
     int P;
     int T;               // RUN: taps before this level; SPECTRUM: first tap of the row
     int ring_mask;       // RUN: R - 1; SPECTRUM: number of IR taps
-    int pad;
 };
 
 struct TickJob {
@@ -70,8 +70,9 @@ struct TickArgs {
     float *y_inst;         // njobs x N0 per-instance outputs (mix == 0 or 2)
     const int *src_off;    // reduce: per output channel offsets into src_list (nOut+1)
     const int *src_list;   // reduce: job indices per output channel
-    unsigned *ticket;      // reduce/direct: arrival counter (reset by the last block)
-    unsigned long long *done_flag;  // host-visible completion flag (may be null)
+    unsigned *ticket;      // reduce: arrival counter (reset by the last block)
+    unsigned long long *done_flag;  // reduce: host-visible completion flag (may be null)
+    unsigned long long *wg_flags;   // direct: one host-visible completion flag per workgroup
 };
 
 // ---------------------------------------------------------------------------
@@ -81,17 +82,17 @@ struct TickArgs {
 template <int LOGN>
 struct SmallCfg {
     static constexpr int N = 1 << LOGN;
-    static constexpr int NT = (N / 4 < 64) ? 64 : ((N / 4 > 256) ? 256 : N / 4);
+    static constexpr int NT = (N >= 256) ? 256 : 64;       // threads per workgroup
+    static constexpr int IPT = (N + NT - 1) / NT;          // bins / samples per thread
 };
 
 // Forward real FFT of the 2N-sample window already packed in bufA as N float2;
-// result X (packed) in the returned buffer.
-template <int LOGN>
+// returns the buffer holding X (packed: X[0] = (X_0, X_N)).
+template <int LOGN, int NT>
 __device__ cf *small_forward(cf *bufA, cf *bufB, const cf *__restrict__ tw, int tid)
 {
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
+    constexpr int N = 1 << LOGN;
     cf *Z = fft_lds<false>(bufA, bufB, LOGN, 1, N, tw, tid, NT);
-    // split pairs (k, N-k) in place, k in [0, N/2]
     for (int k = tid; k <= N / 2; k += NT) {
         if (k == 0) {
             const cf z = Z[0];
@@ -107,12 +108,13 @@ __device__ cf *small_forward(cf *bufA, cf *bufB, const cf *__restrict__ tw, int 
     return Z;
 }
 
-// Inverse: packed half spectrum Y in `buf` -> first N real samples written as
-// N/2 float2 to `out` (LDS or global).  Returns the LDS buffer holding z'.
-template <int LOGN>
+// Inverse: packed half spectrum in Y -> z' with z'[n] = (x[2n], x[2n+1]);
+// returns the LDS buffer holding z' (the first N/2 entries are the first N
+// output samples).
+template <int LOGN, int NT>
 __device__ cf *small_inverse(cf *Y, cf *other, const cf *__restrict__ tw, int tid)
 {
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
+    constexpr int N = 1 << LOGN;
     for (int k = tid; k <= N / 2; k += NT) {
         if (k == 0) {
             const cf y = Y[0];
@@ -128,66 +130,58 @@ __device__ cf *small_inverse(cf *Y, cf *other, const cf *__restrict__ tw, int ti
     return fft_lds<true>(Y, other, LOGN, 1, N, tw, tid, NT);
 }
 
-// MAC over the FDL: Y[k] = X[k] H_0[k] + sum_{p>=1} FDL[(b-p)%P][k] H_p[k],
-// accumulated p = 0..P-1 as the reference does; stores X into FDL slot b%P.
-// X and Y may alias (each thread owns its bins).
-template <int LOGN>
+// acc += a*b for a packed bin: bin 0 holds the two real bins (DC, Nyquist).
+__device__ __forceinline__ cf pmac(cf acc, cf a, cf b, bool bin0)
+{
+    if (bin0) {
+        acc.x += a.x * b.x;
+        acc.y += a.y * b.y;
+        return acc;
+    }
+    return cmac(acc, a, b);
+}
+__device__ __forceinline__ cf pmul(cf a, cf b, bool bin0)
+{
+    return bin0 ? make_float2(a.x * b.x, a.y * b.y) : cmul(a, b);
+}
+
+// Block output spectrum Y[k] = X[k] H_0[k] + S[k], where S (written by
+// k_partial after the previous block) holds sum_{p=1}^{P-1} X_{b-p}[k] H_p[k].
+// Stores X into FDL row b % P for later blocks.  X and Y alias.
+template <int LOGN, int NT>
 __device__ void small_mac(cf *X, const LevelJob &J, uint64_t bcur, int tid)
 {
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
-    const int P = J.P;
-    const int slot = (int)(bcur % (uint64_t)P);
-    cf *fdl_w = J.fdl + (size_t)slot * N;
-    for (int k = tid; k < N; k += NT) {
-        const cf x = X[k];
-        fdl_w[k] = x;
-        cf acc;
-        const cf h0 = J.H[k];
-        if (k == 0) {
-            acc = make_float2(x.x * h0.x, x.y * h0.y);
-            for (int p = 1; p < P; p++) {
-                int s = slot - p;
-                s += (s < 0) ? P : 0;
-                const cf a = J.fdl[(size_t)s * N];
-                const cf h = J.H[(size_t)p * N];
-                acc.x += a.x * h.x;
-                acc.y += a.y * h.y;
-            }
-        } else {
-            acc = cmul(x, h0);
-            int s = slot;
-            int p = 1;
-            for (; p + 3 < P; p += 4) {
-                int s1 = s - 1; s1 += (s1 < 0) ? P : 0;
-                int s2 = s1 - 1; s2 += (s2 < 0) ? P : 0;
-                int s3 = s2 - 1; s3 += (s3 < 0) ? P : 0;
-                int s4 = s3 - 1; s4 += (s4 < 0) ? P : 0;
-                const cf a1 = J.fdl[(size_t)s1 * N + k], h1 = J.H[(size_t)p * N + k];
-                const cf a2 = J.fdl[(size_t)s2 * N + k], h2 = J.H[(size_t)(p + 1) * N + k];
-                const cf a3 = J.fdl[(size_t)s3 * N + k], h3 = J.H[(size_t)(p + 2) * N + k];
-                const cf a4 = J.fdl[(size_t)s4 * N + k], h4 = J.H[(size_t)(p + 3) * N + k];
-                acc = cmac(acc, a1, h1);
-                acc = cmac(acc, a2, h2);
-                acc = cmac(acc, a3, h3);
-                acc = cmac(acc, a4, h4);
-                s = s4;
-            }
-            for (; p < P; p++) {
-                int s1 = s - 1; s1 += (s1 < 0) ? P : 0;
-                acc = cmac(acc, J.fdl[(size_t)s1 * N + k], J.H[(size_t)p * N + k]);
-                s = s1;
-            }
+    constexpr int N = 1 << LOGN, IPT = (N + NT - 1) / NT;
+    const int slot = (int)(bcur % (uint64_t)J.P);
+    cf h[IPT], sv[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        if (k < N) {
+            h[q] = J.H[k];
+            sv[q] = J.S[k];
         }
-        X[k] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        if (k < N) {
+            const cf x = X[k];
+            J.fdl[(size_t)slot * N + k] = x;
+            cf y = pmul(x, h[q], k == 0);
+            y.x += sv[q].x;
+            y.y += sv[q].y;
+            X[k] = y;
+        }
     }
     __syncthreads();
 }
 
 // Load the 2N-sample window of a RUN job (ring) or SPECTRUM job (taps) as N float2.
-template <int LOGN>
+template <int LOGN, int NT>
 __device__ void small_load_window(cf *buf, const LevelJob &J, int mode, long long s, int tid)
 {
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
+    constexpr int N = 1 << LOGN;
     if (mode == PC_MODE_RUN) {
         const cf *w = reinterpret_cast<const cf *>(J.ring + (s & (long long)J.ring_mask));
         for (int n = tid; n < N; n += NT) buf[n] = w[n];
@@ -222,127 +216,260 @@ k_level_small(const LevelJob *__restrict__ jobs, int N0, uint64_t m, uint64_t bc
     const int tid = threadIdx.x;
     const LevelJob J = jobs[blockIdx.x];
     const long long s = (long long)m * N - N0 - J.T;
-    small_load_window<LOGN>(bufA, J, mode, s, tid);
-    cf *X = small_forward<LOGN>(bufA, bufB, tw, tid);
+    small_load_window<LOGN, NT>(bufA, J, mode, s, tid);
+    cf *X = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
     cf *other = (X == bufA) ? bufB : bufA;
     if (mode == PC_MODE_SPECTRUM) {
         const float sc = 1.0f / (float)(2 * N);
         for (int k = tid; k < N; k += NT) J.fdl[k] = cscale(X[k], sc);
         return;
     }
-    small_mac<LOGN>(X, J, bcur, tid);
-    cf *z = small_inverse<LOGN>(X, other, tw, tid);
+    small_mac<LOGN, NT>(X, J, bcur, tid);
+    cf *z = small_inverse<LOGN, NT>(X, other, tw, tid);
     cf *yo = reinterpret_cast<cf *>(J.ybuf + (size_t)ypar * N);
     for (int n = tid; n < N / 2; n += NT) yo[n] = z[n];
 }
 
 // ---------------------------------------------------------------------------
-// Tick kernel: input scatter + level 0 (N = N0) + output sum/mix, one
-// workgroup per instance.  Mirrors PartConvMulti::run / Relaxed::run
-// (src/partconvMulti.cpp:815-928, 1017-1134) for the work that is due every
-// tick; levels >= 1 run as separate level-job launches on their own streams.
+// Tick: input scatter + level 0 (N = N0) + output sum/mix, one workgroup per
+// instance.  Mirrors PartConvMulti::run / Relaxed::run
+// (src/partconvMulti.cpp:815-928, 1017-1134) for the work due every tick;
+// levels >= 1 run as level-job launches on their own streams.
+//
+// Latency split: phase A (on the tick's critical path) needs only the new
+// frame, the previous frame, H_0 and S, the delay-line sum of partitions
+// 1..P-1, which does not depend on the new block.  Phase B (after the tick's
+// output is published) stores X into the FDL and computes S for the next tick:
+//   S_{f+1}[k] = sum_{p=1}^{P-1} X_{f+1-p}[k] H_p[k],   Y_f = X_f H_0 + S_f.
+// S is kept in the level-0 job's scratch row (N0 float2).
 // ---------------------------------------------------------------------------
 template <int LOGN>
-__global__ void __launch_bounds__(SmallCfg<LOGN>::NT)
-k_tick(TickArgs A, const cf *__restrict__ tw)
+struct TickCfg {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int NT = (N <= 64) ? 64 : ((N >= 256) ? 256 : N);
+    static constexpr int IPT = (N + NT - 1) / NT;
+};
+
+// Per-thread state of one instance's level 0 that does not depend on the new
+// frame: H_0 (constant), S (from the previous tick's phase B) and the previous
+// frame (left half of the window).  k_tick loads it each launch; k_persist
+// keeps it in registers across ticks.
+template <int LOGN>
+struct TickState {
+    cf h0[TickCfg<LOGN>::IPT];
+    cf sp[TickCfg<LOGN>::IPT];
+    float pv[TickCfg<LOGN>::IPT];
+};
+
+template <int LOGN>
+__device__ __forceinline__ void tick_load_state(const LevelJob &J, uint64_t f, int tid, TickState<LOGN> &st)
 {
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
-    __shared__ cf bufA[N];
-    __shared__ cf bufB[N];
-    __shared__ float xin[N];
-    __shared__ float acc[N];
-    __shared__ unsigned s_last;
-    const int tid = threadIdx.x;
-    const int i = blockIdx.x;
-    const TickJob &TJ = A.jobs[i];
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
+    const float *prev = J.ring + (((long long)f * N - N) & (long long)J.ring_mask);
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int j = tid + q * NT;
+        if (j < N) {
+            st.h0[q] = J.H[j];
+            st.sp[q] = J.S[j];
+            st.pv[q] = prev[j];
+        }
+    }
+}
+
+// Phase A (the tick's critical path): new frame -> ring, window
+// [previous frame | frame f] -> r2c -> Y = X H_0 + S -> c2r -> this instance's
+// output = level 0 + the level blocks due (strict: block (f+1)/k - 1,
+// sub-block (f+1) % k; relaxed: block f/k - 1, sub-block f % k, and level 0
+// of tick f-1).  Leaves the output in acc, X_f in xr, and the scaled frame in
+// st.pv (the next tick's previous frame).
+template <int LOGN>
+__device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, cf *bufA,
+                                            cf *bufB, const cf *__restrict__ tw, int tid, TickState<LOGN> &st,
+                                            float (&acc)[TickCfg<LOGN>::IPT], cf (&xr)[TickCfg<LOGN>::IPT])
+{
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
     const LevelJob &J = TJ.l0;
-    const uint64_t f = A.f;
     const long long mask = J.ring_mask;
     const long long R = mask + 1;
-
-    // 1. de-interleave x scale (src/partconvMulti.cpp:839-847) and ring write
-    const float *inf = A.in + (long long)(f - A.fbase) * A.in_stride;
     const long long t0 = (long long)f * N;
+    const float *inf = A.in + (long long)(f - A.fbase) * A.in_stride;
+    float xv[IPT], lev[IPT], y0p[IPT];
+    const float scale = TJ.scale;
+    const int inch = TJ.in_ch;
+    const uint64_t fr = A.relaxed ? f : f + 1;
+    // ---- every load first: the frame and the level blocks due
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int j = tid + q * NT;
+        if (j < N) {
+            xv[q] = inf[inch + (long long)j * A.nIn];
+            y0p[q] = (A.relaxed && f > 0) ? J.ybuf[(size_t)((f - 1) & 1) * N + j] : 0.f;
+            lev[q] = 0.f;
+        }
+    }
+    for (int L = 0; L < TJ.nlev; L++) {
+        const int ks = TJ.kshift[L];
+        const uint64_t qb = fr >> ks;
+        if (qb == 0) continue;
+        const uint64_t mb = qb - 1;
+        const uint64_t sub = fr & ((1ull << ks) - 1);
+        const float *src = TJ.ybuf[L] + (size_t)(mb & 1) * TJ.NL[L] + sub * N;
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int j = tid + q * NT;
+            if (j < N) lev[q] += src[j];
+        }
+    }
+    // ---- scale (src/partconvMulti.cpp:839-847), ring write, window in LDS
     float *ring = const_cast<float *>(J.ring);
-    for (int j = tid; j < N; j += NT) {
-        const float v = inf[TJ.in_ch + (long long)j * A.nIn] * TJ.scale;
-        xin[j] = v;
-        const long long p = (t0 + j) & mask;
-        ring[p] = v;
-        ring[p + R] = v;
+    float *wf = reinterpret_cast<float *>(bufA);
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int j = tid + q * NT;
+        if (j < N) {
+            const float v = xv[q] * scale;
+            const long long p = (t0 + j) & mask;
+            ring[p] = v;
+            ring[p + R] = v;
+            wf[j] = st.pv[q];
+            wf[N + j] = v;
+            st.pv[q] = v;
+        }
     }
-    // 2. window = [frame f-1 from the ring | frame f]
-    const float *prev = J.ring + ((t0 - N) & mask);
     __syncthreads();
-    for (int n = tid; n < N; n += NT) {
-        const int a = 2 * n;
-        if (a < N) bufA[n] = make_float2(prev[a], prev[a + 1]);
-        else bufA[n] = make_float2(xin[a - N], xin[a - N + 1]);
-    }
-    __syncthreads();
-    // 3. level 0 of this tick
-    cf *X = small_forward<LOGN>(bufA, bufB, tw, tid);
+    cf *X = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
     cf *other = (X == bufA) ? bufB : bufA;
-    small_mac<LOGN>(X, J, f, tid);
-    cf *z = small_inverse<LOGN>(X, other, tw, tid);
-    // keep y0 for relaxed output of the next tick
-    float *y0w = J.ybuf + (size_t)(f & 1) * N;
-    for (int n = tid; n < N / 2; n += NT) reinterpret_cast<cf *>(y0w)[n] = z[n];
-
-    // 4. output = level 0 + completed blocks of levels >= 1
-    //    strict : level L reads block m = (f+1)/k - 1, sub-block (f+1)%k
-    //    relaxed: level L reads block m = f/k - 1, sub-block f%k; level 0 = tick f-1
-    const float *zr = reinterpret_cast<const float *>(z);
-    for (int j = tid; j < N; j += NT) {
-        float a;
-        if (!A.relaxed) {
-            a = zr[j];
-        } else {
-            a = (f > 0) ? J.ybuf[(size_t)((f - 1) & 1) * N + j] : 0.f;
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        if (k < N) {
+            const cf x = X[k];
+            xr[q] = x;
+            cf y = pmul(x, st.h0[q], k == 0);
+            y.x += st.sp[q].x;
+            y.y += st.sp[q].y;
+            X[k] = y;
         }
-        const uint64_t fr = A.relaxed ? f : f + 1;
-        for (int L = 0; L < TJ.nlev; L++) {
-            const int ks = TJ.kshift[L];
-            const uint64_t q = fr >> ks;
-            if (q == 0) continue;
-            const uint64_t mb = q - 1;
-            const uint64_t sub = fr & ((1ull << ks) - 1);
-            a += TJ.ybuf[L][(size_t)(mb & 1) * TJ.NL[L] + sub * N + j];
-        }
-        acc[j] = a;
     }
     __syncthreads();
+    cf *z = small_inverse<LOGN, NT>(X, other, tw, tid);
+    const float *zf = reinterpret_cast<const float *>(z);
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int j = tid + q * NT;
+        if (j < N) {
+            if (A.relaxed) {
+                J.ybuf[(size_t)(f & 1) * N + j] = zf[j];   // read back at tick f+1
+                acc[q] = y0p[q] + lev[q];
+            } else {
+                acc[q] = zf[j] + lev[q];
+            }
+        }
+    }
+}
 
-    // 5. mix (src/partconvMulti.cpp:906-921)
+// Phase B (after the output is published): FDL row f % P <- X_f and
+//   S <- sum_{p=1}^{P-1} X_{f+1-p} H_p   (for tick f+1; kept in st.sp and J.S).
+template <int LOGN>
+__device__ __forceinline__ void tick_phaseB(const TickJob &TJ, uint64_t f, int tid,
+                                            const cf (&xr)[TickCfg<LOGN>::IPT], TickState<LOGN> &st)
+{
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
+    const LevelJob &J = TJ.l0;
+    const int P = J.P;
+    const int slot = (int)(f % (uint64_t)P);
+    cf s[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        if (k < N) {
+            J.fdl[(size_t)slot * N + k] = xr[q];
+            s[q] = (P > 1) ? pmul(xr[q], J.H[(size_t)N + k], k == 0) : make_float2(0.f, 0.f);
+        }
+    }
+    int r = slot;
+    for (int p = 2; p < P; p++) {
+        r = (r == 0) ? P - 1 : r - 1;   // (f + 1 - p) mod P
+        const cf *a = J.fdl + (size_t)r * N;
+        const cf *h = J.H + (size_t)p * N;
+        cf av[IPT], hv[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int k = tid + q * NT;
+            if (k < N) {
+                av[q] = a[k];
+                hv[q] = h[k];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int k = tid + q * NT;
+            if (k < N) s[q] = pmac(s[q], av[q], hv[q], k == 0);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        if (k < N) {
+            J.S[k] = s[q];
+            st.sp[q] = s[q];
+        }
+    }
+}
+
+// Publish this instance's output.
+//   direct mix (each output channel fed by exactly one unmuted instance): every
+//     workgroup writes its channel and then its own completion flag
+//     wg_flags[i] = f + 1 (system-scope release); no cross-workgroup step.
+//   reduce / plain: the per-instance output goes to y_inst; in reduce mode the
+//     last workgroup to arrive (device ticket) sums the channels in instance
+//     order (src/partconvMulti.cpp:906-921) and raises done_flag.
+template <int LOGN>
+__device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A, uint64_t f, int tid,
+                                            const float (&acc)[TickCfg<LOGN>::IPT], unsigned *s_last)
+{
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
+    const int i = blockIdx.x;
     if (A.mix == 1) {
         if (!TJ.mute) {
             float *o = A.out + (long long)(f - A.fbase) * A.out_stride + TJ.out_ch;
-            for (int j = tid; j < N; j += NT) o[(long long)j * A.nOut] = acc[j];
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const int j = tid + q * NT;
+                if (j < N) o[(long long)j * A.nOut] = acc[q];
+            }
         }
-    } else {
-        float *yi = A.y_inst + (size_t)i * N;
-        for (int j = tid; j < N; j += NT) yi[j] = acc[j];
+        __syncthreads();
+        if (tid == 0 && A.wg_flags) {
+            __threadfence_system();
+            __hip_atomic_store(A.wg_flags + i, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
+    float *yi = A.y_inst + (size_t)i * N;
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int j = tid + q * NT;
+        if (j < N) yi[j] = acc[q];
     }
     if (A.mix == 0) return;
-
-    // arrival ticket: the last block finishes the mix and raises the flag
     __syncthreads();
     if (tid == 0) {
         __threadfence();
         const unsigned t = atomicAdd(A.ticket, 1u);
-        s_last = (t == (unsigned)(A.njobs - 1));
+        *s_last = (t == (unsigned)(A.njobs - 1));
     }
     __syncthreads();
-    if (!s_last) return;
+    if (!*s_last) return;
     __threadfence();
-    if (A.mix == 2) {
-        for (int e = tid; e < A.nOut * N; e += NT) {
-            const int c = e / N, j = e - c * N;
-            float sum = 0.f;
-            for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++)
-                sum += __builtin_nontemporal_load(&A.y_inst[(size_t)A.src_list[q] * N + j]);
-            A.out[(long long)(f - A.fbase) * A.out_stride + c + (long long)j * A.nOut] = sum;
-        }
+    for (int e = tid; e < A.nOut * N; e += NT) {
+        const int c = e / N, j = e - c * N;
+        float sum = 0.f;
+        for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++)
+            sum += A.y_inst[(size_t)A.src_list[q] * N + j];
+        A.out[(long long)(f - A.fbase) * A.out_stride + c + (long long)j * A.nOut] = sum;
     }
     __syncthreads();
     if (tid == 0) {
@@ -353,6 +480,135 @@ k_tick(TickArgs A, const cf *__restrict__ tw)
                                __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(TickCfg<LOGN>::NT)
+k_tick(TickArgs A, const cf *__restrict__ tw)
+{
+    constexpr int N = TickCfg<LOGN>::N, IPT = TickCfg<LOGN>::IPT;
+    __shared__ cf bufA[N];
+    __shared__ cf bufB[N];
+    __shared__ unsigned s_last;
+    const int tid = threadIdx.x;
+    const TickJob &TJ = A.jobs[blockIdx.x];
+    TickState<LOGN> st;
+    tick_load_state<LOGN>(TJ.l0, A.f, tid, st);
+    float acc[IPT];
+    cf xr[IPT];
+    tick_phaseA<LOGN>(TJ, A, A.f, bufA, bufB, tw, tid, st, acc, xr);
+    tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
+    tick_phaseB<LOGN>(TJ, A.f, tid, xr, st);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent tick kernel (device-resident runs): one launch serves nticks
+// consecutive ticks.  The host releases tick f by writing go = f + 1 into
+// pinned memory (after it has checked that the level blocks tick f reads are
+// complete); each workgroup waits for the release, runs the same phases as
+// k_tick with its per-instance state kept in registers and its job descriptor
+// in LDS, and publishes its output.  Every wait is bounded: a release that
+// does not come within `spin_limit` s_memrealtime ticks (100 MHz) ends the
+// kernel, and go = PC_GO_STOP ends it at once.
+// ---------------------------------------------------------------------------
+#define PC_GO_STOP (1ull << 62)
+
+struct PersistArgs {
+    TickArgs A;
+    uint64_t f0;
+    long long nticks;
+    const unsigned long long *go;
+    unsigned long long spin_limit;
+};
+
+template <int LOGN>
+__global__ void __launch_bounds__(TickCfg<LOGN>::NT)
+k_persist(PersistArgs P, const cf *__restrict__ tw)
+{
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
+    __shared__ cf bufA[N];
+    __shared__ cf bufB[N];
+    __shared__ unsigned s_last;
+    __shared__ int s_go;
+    __shared__ TickJob sTJ;
+    const int tid = threadIdx.x;
+    {
+        const unsigned *src = reinterpret_cast<const unsigned *>(P.A.jobs + blockIdx.x);
+        unsigned *dst = reinterpret_cast<unsigned *>(&sTJ);
+        for (int w = tid; w < (int)(sizeof(TickJob) / 4); w += NT) dst[w] = src[w];
+    }
+    __syncthreads();
+    TickState<LOGN> st;
+    tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
+    for (long long t = 0; t < P.nticks; t++) {
+        const uint64_t f = P.f0 + (uint64_t)t;
+        if (tid == 0) {
+            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + P.spin_limit;
+            int go = 0;
+            for (;;) {
+                const unsigned long long g =
+                    __hip_atomic_load(P.go, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (g & PC_GO_STOP) break;
+                if (g >= f + 1) {
+                    go = 1;
+                    break;
+                }
+                if (__builtin_amdgcn_s_memrealtime() > t_end) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;
+        float acc[IPT];
+        cf xr[IPT];
+        tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr);
+        tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
+        tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Delay-line partial sum for block b of a level (every job of one launch has
+// the same N):  S[k] = sum_{p=1}^{P-1} FDL[(b - p) % P][k] * H_p[k],
+// accumulated p = 1..P-1.  It runs on the level's stream right after block
+// b-1 (whose spectrum is FDL row (b-1) % P), i.e. off the path from a block's
+// input to its output: that path then only needs X_b H_0 + S.  This is the
+// kernel that streams the filter spectra and the delay line: each thread owns
+// two bins (one float4 of FDL and H per partition).
+// ---------------------------------------------------------------------------
+#define PART_NT 256
+
+__global__ void __launch_bounds__(PART_NT)
+k_partial(const LevelJob *__restrict__ jobs, int N, int wg_per_job, uint64_t b)
+{
+    const int jb = blockIdx.x / wg_per_job, w = blockIdx.x - jb * wg_per_job;
+    const LevelJob J = jobs[jb];
+    const int P = J.P;
+    const int k2 = w * PART_NT + threadIdx.x;  // float4 index = bins 2*k2, 2*k2+1
+    if (2 * k2 >= N) return;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int s = (int)(b % (uint64_t)P);
+    const float4 *fdl = reinterpret_cast<const float4 *>(J.fdl);
+    const float4 *H = reinterpret_cast<const float4 *>(J.H);
+    const size_t row = (size_t)(N / 2);
+#pragma unroll 4
+    for (int p = 1; p < P; p++) {
+        s = (s == 0) ? P - 1 : s - 1;
+        const float4 a = fdl[(size_t)s * row + k2];
+        const float4 h = H[(size_t)p * row + k2];
+        if (k2 == 0) {  // bin 0 packs the real DC and Nyquist bins
+            acc.x += a.x * h.x;
+            acc.y += a.y * h.y;
+        } else {
+            acc.x += a.x * h.x - a.y * h.y;
+            acc.y += a.x * h.y + a.y * h.x;
+        }
+        acc.z += a.z * h.z - a.w * h.w;
+        acc.w += a.z * h.w + a.w * h.z;
+    }
+    reinterpret_cast<float4 *>(J.S)[k2] = acc;
 }
 
 // Plain-mode mix: out[c + j nOut] = sum over unmuted instances feeding c, in
@@ -516,52 +772,19 @@ k_big2(const LevelJob *__restrict__ jobs, BigShape sh, uint64_t bcur, const cf *
         }
         return;
     }
-    // FDL store + MAC (slot order, contiguous per workgroup)
-    const int P = J.P;
-    const int slot = (int)(bcur % (uint64_t)P);
+    // FDL store; Y = X H_0 + S (slot order, contiguous per workgroup)
+    const int slot = (int)(bcur % (uint64_t)J.P);
     for (int e = tid; e < 2 * A; e += BIG_NT) {
         const int q = e >> sh.logA, d = e & (A - 1);
-        const cf x = W[q * sst + d];
         const size_t bin = base + e;
-        J.fdl[(size_t)slot * N + bin] = x;
         const cf h0 = J.H[bin];
-        cf acc;
-        if (bin == 0) {
-            acc = make_float2(x.x * h0.x, x.y * h0.y);
-            for (int p = 1; p < P; p++) {
-                int s = slot - p;
-                s += (s < 0) ? P : 0;
-                const cf a = J.fdl[(size_t)s * N];
-                const cf h = J.H[(size_t)p * N];
-                acc.x += a.x * h.x;
-                acc.y += a.y * h.y;
-            }
-        } else {
-            acc = cmul(x, h0);
-            int s = slot;
-            int p = 1;
-            for (; p + 3 < P; p += 4) {
-                int s1 = s - 1; s1 += (s1 < 0) ? P : 0;
-                int s2 = s1 - 1; s2 += (s2 < 0) ? P : 0;
-                int s3 = s2 - 1; s3 += (s3 < 0) ? P : 0;
-                int s4 = s3 - 1; s4 += (s4 < 0) ? P : 0;
-                const cf a1 = J.fdl[(size_t)s1 * N + bin], h1 = J.H[(size_t)p * N + bin];
-                const cf a2 = J.fdl[(size_t)s2 * N + bin], h2 = J.H[(size_t)(p + 1) * N + bin];
-                const cf a3 = J.fdl[(size_t)s3 * N + bin], h3 = J.H[(size_t)(p + 2) * N + bin];
-                const cf a4 = J.fdl[(size_t)s4 * N + bin], h4 = J.H[(size_t)(p + 3) * N + bin];
-                acc = cmac(acc, a1, h1);
-                acc = cmac(acc, a2, h2);
-                acc = cmac(acc, a3, h3);
-                acc = cmac(acc, a4, h4);
-                s = s4;
-            }
-            for (; p < P; p++) {
-                int s1 = s - 1; s1 += (s1 < 0) ? P : 0;
-                acc = cmac(acc, J.fdl[(size_t)s1 * N + bin], J.H[(size_t)p * N + bin]);
-                s = s1;
-            }
-        }
-        W[q * sst + d] = acc;
+        const cf sv = J.S[bin];
+        const cf x = W[q * sst + d];
+        J.fdl[(size_t)slot * N + bin] = x;
+        cf y = pmul(x, h0, bin == 0);
+        y.x += sv.x;
+        y.y += sv.y;
+        W[q * sst + d] = y;
     }
     __syncthreads();
     // inverse split back into Z (column sequences over d)

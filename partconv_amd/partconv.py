@@ -257,12 +257,14 @@ class PartConvMulti:
             raise ValueError("contiguous buffers required")
         return _report(lib().pcm_run(self._h, output.ctypes.data, input.ctypes.data, int(frameCount)), "run")
 
-    def run_device(self, out_ptr, in_ptr, nframes, sync_each_tick=True):
+    def run_device(self, out_ptr, in_ptr, nframes, sync_each_tick=True, persistent=False):
         """Device-resident ticks (pcm_run_device): integer device pointers to
-        interleaved float32 frames."""
+        interleaved float32 frames.  persistent=True serves the ticks with one
+        persistent tick kernel (real-time semantics kept)."""
+        mode = 2 if persistent else (1 if sync_each_tick else 0)
         return _report(lib().pcm_run_device(self._h, ctypes.c_void_p(int(out_ptr)),
-                                            ctypes.c_void_p(int(in_ptr)), int(nframes),
-                                            1 if sync_each_tick else 0), "run_device")
+                                            ctypes.c_void_p(int(in_ptr)), int(nframes), mode),
+                       "run_device")
 
     def process(self, x):
         """Convenience: run interleaved (n, numInputChannels) host samples tick by

@@ -212,3 +212,39 @@ def test_device_resident_run_matches_host_run():
     assert np.array_equal(yd.cpu().numpy(), y_host)
     st = m.stats()
     assert st.ticks == nfr and st.p99_us > 0
+
+
+def test_persistent_tick_kernel_matches_launch_mode():
+    say("case persistent")
+    import gc
+    import torch
+    gc.collect()
+    rng = np.random.default_rng(9)
+    insts = [dict(N=[64, 256, 2048, 16384], P=[6, 14, 14, 0], in_ch=i, out_ch=i) for i in range(4)]
+    irs = [decaying_noise_ir(60000, 1.0, 48000, 90 + i) for i in range(4)]
+    nfr = 700
+    x = rng.standard_normal((nfr * 64, 4)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    for relaxed in (False, True):
+        ref = make_multi(insts, irs, 64, 4, 4, relaxed=relaxed)
+        yl = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        assert ref.run_device(yl.data_ptr(), xd.data_ptr(), nfr, True) == 0, last_error()
+        ref.cleanup()
+        del ref
+        m = make_multi(insts, irs, 64, 4, 4, relaxed=relaxed)
+        yp = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        # two calls: the second continues the stream where the first stopped
+        assert m.run_device(yp.data_ptr(), xd.data_ptr(), 300, True, persistent=True) == 0, last_error()
+        off = 300 * 64 * 4
+        assert m.run_device(yp.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, nfr - 300, True,
+                            persistent=True) == 0, last_error()
+        st = m.stats()
+        say(f"persistent ticks served {st.persistent_ticks} of {nfr}; p50 {st.p50_us:.1f} us")
+        assert np.array_equal(yp.cpu().numpy(), yl.cpu().numpy())
+        r = O.direct_conv_f64(x[:, 2], irs[2])
+        if relaxed:
+            r = np.concatenate([np.zeros(64), r[:-64]])
+        assert rel(yp.cpu().numpy()[:, 2], r) < TOL
+        m.cleanup()
