@@ -147,6 +147,12 @@ def main():
     mb.probe(-1)
     mb.cleanup()
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_partial_cfg2_16ch_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            t = json.load(fh)
+        traffic = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
     out = {
         "metric": "channel-samples/s at p99 block latency < block period; batched vs plain, same GPU",
         "value": value,
@@ -174,7 +180,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": bytes_per / (avg_us * 1e-6) / 1e9 if avg_us else None,
                      "peak": 8000.0, "unit": "GB/s",
                      "frac": (bytes_per / (avg_us * 1e-6) / 1e9) / 8000.0 if avg_us else None,
-                     "traffic": None, "kernel": "k_partial (16384-level delay-line sum over 14 partitions, 16 jobs)",
+                     "traffic": traffic, "kernel": "k_partial (16384-level delay-line sum over 14 partitions, 16 jobs)",
                      "avg_us": avg_us, "launches": launches, "alg_bytes_per_launch": bytes_per},
         "tick_kernel": {"avg_us": tick_us, "alg_bytes_per_launch": tick_bytes,
                         "achieved_GBps": tick_bytes / (tick_us * 1e-6) / 1e9 if tick_us else None},

@@ -877,6 +877,10 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     a.src_off = E.d_src_off;
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
+    // host-memory output needs a system-scope release before the flag; device
+    // output is published to later kernels by the kernel boundary itself
+    a.fence_sys = (out == E.d_out_map) ? 1 : 0;
+    a.fence_mask = ~0ull;
     if (E.mode == PC_BATCHED) {
         Group &Gr = E.groups[0];
         a.jobs = Gr.d_tick;
@@ -908,15 +912,18 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
                            E.d_flag_map, f);
     }
     HIPCHK(hipGetLastError());
-    // start the levels whose period begins at this tick (src/partconvMulti.cpp:860-873)
-    int rc = start_levels(E, f, true);
-    if (rc) return rc;
+    // record the tick kernel's event now so the level streams depend on it on
+    // the device; the level launches themselves (host work) follow the wait,
+    // outside this tick's latency (src/partconvMulti.cpp:860-873 starts them
+    // before its own level-0 work; here they run on other streams anyway)
     E.frame++;
     if (wait) {
         int rc = wait_flag(E, f + 1);
         if (rc) return rc;
         record_latency(E, 1e6 * (now_s() - t0));
     }
+    int rc = start_levels(E, f, true);
+    if (rc) return rc;
     return PC_OK;
 }
 
@@ -985,6 +992,11 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.ticket = E.d_ticket;
     a.done_flag = E.d_flag_map;
     a.wg_flags = E.d_wg_map;
+    a.fence_sys = 0;
+    // level jobs are launched (from the host) only after ticks f = 0 mod k_min;
+    // those ticks publish the ring with a device-scope release
+    a.fence_mask = (E.gsize.size() > 1) ? (unsigned long long)(E.gsize[1] - 1) : ~0ull;
+    P.acq_mask = a.fence_mask;
     P.f0 = f0;
     P.nticks = nframes;
     P.go = E.d_flag_map + 1;

@@ -73,6 +73,8 @@ struct TickArgs {
     unsigned *ticket;      // reduce: arrival counter (reset by the last block)
     unsigned long long *done_flag;  // reduce: host-visible completion flag (may be null)
     unsigned long long *wg_flags;   // direct: one host-visible completion flag per workgroup
+    int fence_sys;                  // output lives in host memory: system-scope release per tick
+    unsigned long long fence_mask;  // else: device-scope release at ticks with (f & mask) == 0
 };
 
 // ---------------------------------------------------------------------------
@@ -82,7 +84,7 @@ struct TickArgs {
 template <int LOGN>
 struct SmallCfg {
     static constexpr int N = 1 << LOGN;
-    static constexpr int NT = (N >= 256) ? 256 : 64;       // threads per workgroup
+    static constexpr int NT = (N / 4 < 64) ? 64 : ((N / 4 > 512) ? 512 : N / 4);  // threads
     static constexpr int IPT = (N + NT - 1) / NT;          // bins / samples per thread
 };
 
@@ -149,11 +151,10 @@ __device__ __forceinline__ cf pmul(cf a, cf b, bool bin0)
 // k_partial after the previous block) holds sum_{p=1}^{P-1} X_{b-p}[k] H_p[k].
 // Stores X into FDL row b % P for later blocks.  X and Y alias.
 template <int LOGN, int NT>
-__device__ void small_mac(cf *X, const LevelJob &J, uint64_t bcur, int tid)
+__device__ __forceinline__ void small_prefetch(const LevelJob &J, int tid, cf (&h)[(((1 << LOGN) + NT - 1) / NT)],
+                                               cf (&sv)[(((1 << LOGN) + NT - 1) / NT)])
 {
     constexpr int N = 1 << LOGN, IPT = (N + NT - 1) / NT;
-    const int slot = (int)(bcur % (uint64_t)J.P);
-    cf h[IPT], sv[IPT];
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const int k = tid + q * NT;
@@ -162,6 +163,14 @@ __device__ void small_mac(cf *X, const LevelJob &J, uint64_t bcur, int tid)
             sv[q] = J.S[k];
         }
     }
+}
+
+template <int LOGN, int NT>
+__device__ void small_mac(cf *X, const LevelJob &J, uint64_t bcur, int tid,
+                          const cf (&h)[(((1 << LOGN) + NT - 1) / NT)], const cf (&sv)[(((1 << LOGN) + NT - 1) / NT)])
+{
+    constexpr int N = 1 << LOGN, IPT = (N + NT - 1) / NT;
+    const int slot = (int)(bcur % (uint64_t)J.P);
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const int k = tid + q * NT;
@@ -214,8 +223,11 @@ k_level_small(const LevelJob *__restrict__ jobs, int N0, uint64_t m, uint64_t bc
     __shared__ cf bufA[N];
     __shared__ cf bufB[N];
     const int tid = threadIdx.x;
+    constexpr int IPT = (N + NT - 1) / NT;
     const LevelJob J = jobs[blockIdx.x];
     const long long s = (long long)m * N - N0 - J.T;
+    cf h[IPT], sv[IPT];
+    if (mode == PC_MODE_RUN) small_prefetch<LOGN, NT>(J, tid, h, sv);
     small_load_window<LOGN, NT>(bufA, J, mode, s, tid);
     cf *X = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
     cf *other = (X == bufA) ? bufB : bufA;
@@ -224,7 +236,7 @@ k_level_small(const LevelJob *__restrict__ jobs, int N0, uint64_t m, uint64_t bc
         for (int k = tid; k < N; k += NT) J.fdl[k] = cscale(X[k], sc);
         return;
     }
-    small_mac<LOGN, NT>(X, J, bcur, tid);
+    small_mac<LOGN, NT>(X, J, bcur, tid, h, sv);
     cf *z = small_inverse<LOGN, NT>(X, other, tw, tid);
     cf *yo = reinterpret_cast<cf *>(J.ybuf + (size_t)ypar * N);
     for (int n = tid; n < N / 2; n += NT) yo[n] = z[n];
@@ -442,8 +454,14 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         }
         __syncthreads();
         if (tid == 0 && A.wg_flags) {
-            __threadfence_system();
-            __hip_atomic_store(A.wg_flags + i, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
+            // What the flag publishes: host-memory output (system scope), or the
+            // ring frames that level jobs launched after this tick will read
+            // from other XCDs (device scope, at level-start ticks); otherwise
+            // only this tick's completion (workgroup scope: the stores issued).
+            if (A.fence_sys) __threadfence_system();
+            else if ((f & A.fence_mask) == 0) __threadfence();
+            else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __hip_atomic_store(A.wg_flags + i, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_SYSTEM);
         }
         return;
@@ -519,6 +537,7 @@ struct PersistArgs {
     long long nticks;
     const unsigned long long *go;
     unsigned long long spin_limit;
+    unsigned long long acq_mask;   // device-scope acquire at ticks whose (f [+1]) & mask == 0
 };
 
 template <int LOGN>
@@ -547,7 +566,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             int go = 0;
             for (;;) {
                 const unsigned long long g =
-                    __hip_atomic_load(P.go, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_load(P.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 if (g & PC_GO_STOP) break;
                 if (g >= f + 1) {
                     go = 1;
@@ -556,6 +575,10 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
                 if (__builtin_amdgcn_s_memrealtime() > t_end) break;
                 __builtin_amdgcn_s_sleep(1);
             }
+            // level blocks written by other kernels (other XCDs) are first read
+            // at deadline ticks: acquire at device scope there
+            if (go && (((P.A.relaxed ? f : f + 1) & P.acq_mask) == 0))
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             s_go = go;
         }
         __syncthreads();
@@ -722,6 +745,20 @@ k_big2(const LevelJob *__restrict__ jobs, BigShape sh, uint64_t bcur, const cf *
     const int tid = threadIdx.x;
     const int c1 = u, c2 = u ? (B - u) : (B / 2);
     cf *S = J.scratch;
+    const size_t base = (size_t)u * 2 * A;
+    // H_0 and S of this workgroup's bins, fetched before the transform
+    constexpr int BQ = 4;   // 2A <= 4 * BIG_NT (A <= 512)
+    cf h0v[BQ], sv[BQ];
+    if (mode == PC_MODE_RUN) {
+#pragma unroll
+        for (int q = 0; q < BQ; q++) {
+            const int e = tid + q * BIG_NT;
+            if (e < 2 * A) {
+                h0v[q] = J.H[base + e];
+                sv[q] = J.S[base + e];
+            }
+        }
+    }
     for (int e = tid; e < 2 * A; e += BIG_NT) {
         const int q = e >> sh.logA, a = e & (A - 1);
         bufA[q * sst + a] = S[(size_t)(q ? c2 : c1) * A + a];
@@ -763,7 +800,6 @@ k_big2(const LevelJob *__restrict__ jobs, BigShape sh, uint64_t bcur, const cf *
         }
     }
     __syncthreads();
-    const size_t base = (size_t)u * 2 * A;
     if (mode == PC_MODE_SPECTRUM) {
         const float sc = 1.0f / (float)(2 * N);
         for (int e = tid; e < 2 * A; e += BIG_NT) {
@@ -774,16 +810,17 @@ k_big2(const LevelJob *__restrict__ jobs, BigShape sh, uint64_t bcur, const cf *
     }
     // FDL store; Y = X H_0 + S (slot order, contiguous per workgroup)
     const int slot = (int)(bcur % (uint64_t)J.P);
-    for (int e = tid; e < 2 * A; e += BIG_NT) {
+#pragma unroll
+    for (int qq = 0; qq < BQ; qq++) {
+        const int e = tid + qq * BIG_NT;
+        if (e >= 2 * A) break;
         const int q = e >> sh.logA, d = e & (A - 1);
         const size_t bin = base + e;
-        const cf h0 = J.H[bin];
-        const cf sv = J.S[bin];
         const cf x = W[q * sst + d];
         J.fdl[(size_t)slot * N + bin] = x;
-        cf y = pmul(x, h0, bin == 0);
-        y.x += sv.x;
-        y.y += sv.y;
+        cf y = pmul(x, h0v[qq], bin == 0);
+        y.x += sv[qq].x;
+        y.y += sv[qq].y;
         W[q * sst + d] = y;
     }
     __syncthreads();
