@@ -89,6 +89,14 @@ const char *pc_last_error(void);
 /* Library / device information: writes the HIP device name into buf. */
 int pc_device_name(char *buf, int len);
 
+/* Greedy partition of PartConvMax::setup (src/partconv.cpp:712-728): over the
+ * candidate block sizes block_sizes[0..levels), P_i = 2*N_{i+1}/N_i - 2, stopping
+ * at the first level whose cumulative span reaches filter_length; the last
+ * level's count is recounted from filter_length (src/partconv.cpp:169).
+ * Writes *M_out levels into N and P (each with room for `levels`). */
+int pc_greedy_partition(const int *block_sizes, int levels, int filter_length, int *N, int *P,
+                        int *M_out);
+
 /* ----------------------------------------------------------------------------
  * PartConvMulti
  * ------------------------------------------------------------------------- */

@@ -47,6 +47,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "pc_last_error": (ctypes.c_char_p, []),
     "pc_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "pc_greedy_partition": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]),
     "pcm_create": (_vp, [ctypes.c_int, ctypes.c_int]),
     "pcm_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_setup": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),

@@ -214,6 +214,7 @@ struct Engine {
     unsigned long long *d_flag_map = nullptr;
     unsigned long long *h_wg = nullptr, *d_wg_map = nullptr;  // per-workgroup flags (direct mix)
     int nwg = 0;
+    bool use_wgflags = false;   // direct mix with few instances: one host flag per workgroup
     uint64_t frame = 0;
     // kernel probe: HIP events around one kernel of one global level
     int probe_g = -1;
@@ -658,7 +659,8 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_out_map, E.h_out, 0));
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_flag_map, E.h_flag, 0));
     E.nwg = (E.mode == PC_BATCHED) ? (int)E.inst.size() : 0;
-    if (E.mix == 1) {
+    E.use_wgflags = (E.mix == 1 && E.nwg <= 32);
+    if (E.use_wgflags) {
         HIPCHK(hipHostMalloc(&E.h_wg, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long),
                              hipHostMallocCoherent | hipHostMallocMapped));
         memset(E.h_wg, 0, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long));
@@ -727,7 +729,7 @@ static void record_latency(Engine &E, double us)
 // single flag raised by the last workgroup / the mix kernel.
 static bool tick_done(const Engine &E, unsigned long long target)
 {
-    if (E.mix == 1) {
+    if (E.use_wgflags) {
         const volatile unsigned long long *w = E.h_wg;
         for (int i = 0; i < E.nwg; i++)
             if (w[i] < target) return false;
@@ -888,7 +890,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         a.mix = E.mix;
         a.y_inst = E.y_inst;
         a.done_flag = E.d_flag_map;
-        a.wg_flags = E.d_wg_map;
+        a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
         LaunchExtra x;
         if (E.probe_g == 0) {
             int rc = probe_take(E, x);
@@ -991,12 +993,15 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
     a.done_flag = E.d_flag_map;
-    a.wg_flags = E.d_wg_map;
+    a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
     a.fence_sys = 0;
     // level jobs are launched (from the host) only after ticks f = 0 mod k_min;
     // those ticks publish the ring with a device-scope release
     a.fence_mask = (E.gsize.size() > 1) ? (unsigned long long)(E.gsize[1] - 1) : ~0ull;
     P.acq_mask = a.fence_mask;
+    // relay through device memory (d_ticket[2..3]) when many workgroups would poll the bus
+    P.dev_go = (a.njobs > 32) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
+    if (P.dev_go) HIPCHK(hipMemsetAsync(P.dev_go, 0, sizeof(unsigned long long), E.s_main));
     P.f0 = f0;
     P.nticks = nframes;
     P.go = E.d_flag_map + 1;
@@ -1086,6 +1091,38 @@ PC_API int pc_device_name(char *buf, int len)
     hipDeviceProp_t p;
     HIPCHK(hipGetDeviceProperties(&p, dev));
     snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+    return PC_OK;
+}
+
+PC_API int pc_greedy_partition(const int *block_sizes, int levels, int filter_length, int *N, int *P,
+                               int *M_out)
+{
+    if (!block_sizes || levels < 1 || !N || !P || !M_out || filter_length < 1)
+        return fail(PC_ERR_ARG, "bad argument");
+    // P_i = 2 N_{i+1} / N_i - 2, stop at the first level whose cumulative span
+    // covers the filter (src/partconv.cpp:712-728); the last count is recounted
+    // by the partition check (src/partconv.cpp:169)
+    int M = levels;
+    std::vector<int> n(block_sizes, block_sizes + levels), p(levels, 0);
+    long long total = 0;
+    for (int i = 0; i < levels - 1; i++) {
+        p[i] = 2 * n[i + 1] / n[i] - 2;
+        total += (long long)n[i] * p[i];
+        if (total >= filter_length) {
+            M = i + 1;
+            break;
+        }
+    }
+    n.resize(M);
+    p.resize(M);
+    std::vector<int> delay, T;
+    int rc = check_partition(n, p, filter_length, delay, T);
+    if (rc) return rc;
+    for (int i = 0; i < M; i++) {
+        N[i] = n[i];
+        P[i] = p[i];
+    }
+    *M_out = M;
     return PC_OK;
 }
 

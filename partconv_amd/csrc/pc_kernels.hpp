@@ -452,27 +452,32 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
                 if (j < N) o[(long long)j * A.nOut] = acc[q];
             }
         }
-        __syncthreads();
-        if (tid == 0 && A.wg_flags) {
-            // What the flag publishes: host-memory output (system scope), or the
-            // ring frames that level jobs launched after this tick will read
-            // from other XCDs (device scope, at level-start ticks); otherwise
-            // only this tick's completion (workgroup scope: the stores issued).
-            if (A.fence_sys) __threadfence_system();
-            else if ((f & A.fence_mask) == 0) __threadfence();
-            else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __hip_atomic_store(A.wg_flags + i, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_SYSTEM);
+        if (A.wg_flags) {
+            __syncthreads();
+            if (tid == 0) {
+                // What the flag publishes: host-memory output (system scope), or the
+                // ring frames that level jobs launched after this tick will read
+                // from other XCDs (device scope, at level-start ticks); otherwise
+                // only this tick's completion (workgroup scope: the stores issued).
+                if (A.fence_sys) __threadfence_system();
+                else if ((f & A.fence_mask) == 0) __threadfence();
+                else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __hip_atomic_store(A.wg_flags + i, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            return;
         }
-        return;
-    }
-    float *yi = A.y_inst + (size_t)i * N;
+    } else {
+        float *yi = A.y_inst + (size_t)i * N;
 #pragma unroll
-    for (int q = 0; q < IPT; q++) {
-        const int j = tid + q * NT;
-        if (j < N) yi[j] = acc[q];
+        for (int q = 0; q < IPT; q++) {
+            const int j = tid + q * NT;
+            if (j < N) yi[j] = acc[q];
+        }
+        if (A.mix == 0) return;
     }
-    if (A.mix == 0) return;
+    // device ticket: the last workgroup to arrive sums the channels (reduce
+    // mode, instance order as src/partconvMulti.cpp:906-921) and raises done_flag
     __syncthreads();
     if (tid == 0) {
         __threadfence();
@@ -482,12 +487,14 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     __syncthreads();
     if (!*s_last) return;
     __threadfence();
-    for (int e = tid; e < A.nOut * N; e += NT) {
-        const int c = e / N, j = e - c * N;
-        float sum = 0.f;
-        for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++)
-            sum += A.y_inst[(size_t)A.src_list[q] * N + j];
-        A.out[(long long)(f - A.fbase) * A.out_stride + c + (long long)j * A.nOut] = sum;
+    if (A.mix == 2) {
+        for (int e = tid; e < A.nOut * N; e += NT) {
+            const int c = e / N, j = e - c * N;
+            float sum = 0.f;
+            for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++)
+                sum += A.y_inst[(size_t)A.src_list[q] * N + j];
+            A.out[(long long)(f - A.fbase) * A.out_stride + c + (long long)j * A.nOut] = sum;
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -535,7 +542,8 @@ struct PersistArgs {
     TickArgs A;
     uint64_t f0;
     long long nticks;
-    const unsigned long long *go;
+    const unsigned long long *go;  // host-written release counter (pinned)
+    unsigned long long *dev_go;    // relay: workgroup 0 copies go here for the others (or null)
     unsigned long long spin_limit;
     unsigned long long acq_mask;   // device-scope acquire at ticks whose (f [+1]) & mask == 0
 };
@@ -564,9 +572,15 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         if (tid == 0) {
             const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + P.spin_limit;
             int go = 0;
+            // with many workgroups only workgroup 0 polls host memory over the bus;
+            // it relays the counter through device memory to the others
+            const bool host_poll = (P.dev_go == nullptr) || blockIdx.x == 0;
             for (;;) {
                 const unsigned long long g =
-                    __hip_atomic_load(P.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    host_poll ? __hip_atomic_load(P.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                              : __hip_atomic_load(P.dev_go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (host_poll && P.dev_go && (g >= f + 1 || (g & PC_GO_STOP)))
+                    __hip_atomic_store(P.dev_go, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (g & PC_GO_STOP) break;
                 if (g >= f + 1) {
                     go = 1;

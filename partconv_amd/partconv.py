@@ -77,6 +77,21 @@ def _as_int_list(x, n=None):
     return x if n is None else x[:n]
 
 
+def greedy_partition(block_sizes, filter_length):
+    """PartConvMax's partition rule (src/partconv.cpp:712-728): returns (N, P)
+    with the last count recounted for filter_length taps."""
+    bs = [int(b) for b in block_sizes]
+    n = len(bs)
+    B = (ctypes.c_int * n)(*bs)
+    N = (ctypes.c_int * n)()
+    P = (ctypes.c_int * n)()
+    M = ctypes.c_int()
+    rc = lib().pc_greedy_partition(B, n, int(filter_length), N, P, ctypes.byref(M))
+    if rc != 0:
+        raise ValueError(last_error())
+    return list(N)[:M.value], list(P)[:M.value]
+
+
 class PartConvFilter:
     """PartConvFilter (src/partconv.h:40-114): one instance, caller-driven levels."""
 

@@ -73,3 +73,17 @@ def test_python_surface_shapes():
     d = pc.doubleArray(4)
     d[3] = 2.5
     assert d[3] == 2.5
+
+
+def test_greedy_partition_matches_rules():
+    """The product's greedy rule (C ABI) against the oracle's restatement."""
+    import numpy as np
+    from oracle import rules as R
+    from partconv_amd import greedy_partition
+    bs = [128 << i for i in range(9)]
+    rng = np.random.default_rng(31)
+    for T in [4096, 5000, 1 << 20, 70000] + [int(v) for v in rng.integers(300, 1 << 20, size=200)]:
+        N, P = greedy_partition(bs, T)
+        g = R.greedy_partition(bs, T)
+        assert (N, P) == (g.N, g.P), T
+    assert greedy_partition(bs, 4096) == ([128, 256, 512, 1024, 2048], [2, 2, 2, 2, 1])

@@ -248,3 +248,37 @@ def test_persistent_tick_kernel_matches_launch_mode():
             r = np.concatenate([np.zeros(64), r[:-64]])
         assert rel(yp.cpu().numpy()[:, 2], r) < TOL
         m.cleanup()
+
+
+def test_uneven_batched_instances_greedy_partitions():
+    """Configuration-3 shape at reduced size: uneven IR lengths (log-uniform),
+    each instance with its own greedy partition over 128..32768, one batched
+    engine (persistent tick kernel), every channel against fp64."""
+    say("case uneven batched (cfg3 shape)")
+    import torch
+    from partconv_amd import greedy_partition
+    from partconv_amd.workloads import cfg3_irs, cfg3_lengths
+    n = 40
+    lens = cfg3_lengths(seed=3, instances=n, min_log2=12, max_log2=17)
+    irs = cfg3_irs(lens, seed=3)
+    bs = [128 << i for i in range(9)]
+    insts = []
+    for i, T in enumerate(lens):
+        N, P = greedy_partition(bs, int(T))
+        insts.append(dict(N=N, P=P, in_ch=i, out_ch=i))
+    nfr = 2 * 256 + 30
+    rng = np.random.default_rng(33)
+    x = rng.standard_normal((nfr * 128, n)).astype(np.float32)
+    m = make_multi(insts, irs, 128, n, n)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros_like(xd)
+    torch.cuda.synchronize()
+    assert m.run_device(yd.data_ptr(), xd.data_ptr(), nfr, True, persistent=True) == 0, last_error()
+    y = yd.cpu().numpy()
+    worst = 0.0
+    for i in range(n):
+        r = O.direct_conv_f64(x[:, i], irs[i])
+        worst = max(worst, rel(y[:, i], r))
+    say(f"worst channel rel err {worst:.2e}; persistent ticks {m.stats().persistent_ticks}")
+    assert worst < TOL
+    m.cleanup()
