@@ -232,6 +232,36 @@ int pcf_get_levels(const pc_filter *h, int *M, int *N, int *num_parts, int cap);
 int pcf_get_fs(const pc_filter *h);
 int pcf_set_fs(pc_filter *h, int fs);
 
+/* ----------------------------------------------------------------------------
+ * Offline batched partitioned convolution and its adjoint (differentiable
+ * form).  No reference counterpart: the reference has no batch or gradient
+ * path.  It uses the reference's partition rules (src/partconv.cpp:86-182) and
+ * filter-spectrum definition (src/partconv.cpp:199-230): y[c] = conv(x[c], h)
+ * truncated to T samples, with every block of every level of every clip
+ * computed at once.  Device pointers; `stream` is a hipStream_t (may be null).
+ * ------------------------------------------------------------------------- */
+typedef struct pc_conv pc_conv;
+
+/* Partition N[0..M), P[0..M) (last count recounted for `taps`). */
+pc_conv *pcc_create(int M, const int *N, const int *P, int taps);
+/* Levels after recounting; returns M or -1. */
+int pcc_levels(const pc_conv *c, int *N, int *P, int cap);
+/* H = r2c([0_N | taps]) / 2N per partition, from h[0..taps) on the device. */
+int pcc_set_filter(pc_conv *c, const float *d_h, void *stream);
+/* y[C][T] = partitioned convolution of x[C][T] with the filter. */
+int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream);
+/* Adjoint for upstream gradient g[C][T]: dx[C][T] = correlation of g with h,
+ * dh[taps] = sum over clips of the correlation of g with x (either output may
+ * be null). */
+int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
+                 void *stream);
+/* Memory budget (bytes) for keeping every level's input spectra from forward
+ * for backward; above it backward recomputes them from the stored input. */
+int pcc_set_budget(pc_conv *c, long long bytes);
+/* Number of level spectra recomputed by backward so far. */
+long long pcc_recomputed_levels(const pc_conv *c);
+void pcc_destroy(pc_conv *c);
+
 #ifdef __cplusplus
 }
 #endif

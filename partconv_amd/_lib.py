@@ -83,6 +83,14 @@ SIGNATURES = {
     "pcf_get_levels": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.c_int]),
     "pcf_get_fs": (ctypes.c_int, [_vp]),
     "pcf_set_fs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcc_create": (_vp, [ctypes.c_int, _ip, _ip, ctypes.c_int]),
+    "pcc_levels": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int]),
+    "pcc_set_filter": (ctypes.c_int, [_vp, _vp, _vp]),
+    "pcc_forward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pcc_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pcc_set_budget": (ctypes.c_int, [_vp, ctypes.c_longlong]),
+    "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),
+    "pcc_destroy": (None, [_vp]),
 }
 
 

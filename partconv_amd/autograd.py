@@ -1,0 +1,119 @@
+"""Differentiable partitioned convolution on the GPU (torch.autograd.Function
+over the pcc_* entry points of libpartconv_amd.so).
+
+    conv = PartitionedConv([256, 2048, 16384], [14, 14, 0], taps=65536)
+    y = conv(x, h)          # x: [C, T] float32 cuda, h: [taps] float32 cuda
+    loss = ((y - target) ** 2).mean(); loss.backward()
+
+Forward: y[c] = conv(x[c], h) truncated to T samples, computed as the
+reference's multi-level partitioned convolution (partition rules of
+src/partconv.cpp:86-182, filter spectra of src/partconv.cpp:199-230) over every
+block of every clip at once.  Backward is the hand-written adjoint (pcc_backward):
+dL/dx is the delay-line product with conjugated partition spectra run in
+reverse block order, dL/dh_p the conjugated input spectrum times the output-
+gradient spectrum accumulated over blocks and clips.  Input spectra are kept
+from forward for backward while they fit `budget_bytes`, else recomputed from
+the stored time-domain input.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import last_error, lib
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class PartitionedConv:
+    """Handle on one partition / tap count; call it like a function."""
+
+    def __init__(self, N, P, taps, budget_bytes=1 << 30):
+        n = len(N)
+        Na = (ctypes.c_int * n)(*[int(v) for v in N])
+        Pa = (ctypes.c_int * n)(*[int(v) for v in P])
+        self._h = lib().pcc_create(n, Na, Pa, int(taps))
+        if not self._h:
+            raise ValueError(last_error())
+        self.taps = int(taps)
+        self.set_budget(budget_bytes)
+        No = (ctypes.c_int * 16)()
+        Po = (ctypes.c_int * 16)()
+        M = lib().pcc_levels(self._h, No, Po, 16)
+        self.N, self.P = list(No)[:M], list(Po)[:M]
+        self._filter_key = None
+
+    def set_budget(self, nbytes):
+        lib().pcc_set_budget(self._h, int(nbytes))
+
+    @property
+    def recomputed_levels(self):
+        return lib().pcc_recomputed_levels(self._h)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(last_error())
+
+    def set_filter(self, h):
+        key = (h.data_ptr(), h._version)
+        if key != self._filter_key:
+            self._check(lib().pcc_set_filter(self._h, _p(h), _stream()))
+            self._filter_key = key
+
+    def forward_raw(self, x, h, y):
+        self.set_filter(h)
+        self._check(lib().pcc_forward(self._h, _p(x), _p(y), x.shape[0], x.shape[1], _stream()))
+
+    def backward_raw(self, x, h, g, dx, dh):
+        self.set_filter(h)
+        self._check(lib().pcc_backward(self._h, _p(x), _p(g), None if dx is None else _p(dx),
+                                       None if dh is None else _p(dh), x.shape[0], x.shape[1], _stream()))
+
+    def __call__(self, x, h):
+        return PartConvFunction.apply(x, h, self)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pcc_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _prep(t, shape_msg):
+    if not (t.is_cuda and t.dtype == torch.float32):
+        raise TypeError(f"{shape_msg}: float32 CUDA tensor required")
+    return t.contiguous()
+
+
+class PartConvFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, h, conv):
+        x = _prep(x, "x")
+        h = _prep(h, "h")
+        if x.dim() != 2 or h.dim() != 1 or h.numel() != conv.taps:
+            raise ValueError("x must be [C, T] and h [taps]")
+        y = torch.empty_like(x)
+        conv.forward_raw(x, h, y)
+        ctx.save_for_backward(x, h)
+        ctx.conv = conv
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        x, h = ctx.saved_tensors
+        conv = ctx.conv
+        g = g.contiguous()
+        dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
+        dh = torch.empty_like(h) if ctx.needs_input_grad[1] else None
+        if dx is not None or dh is not None:
+            conv.backward_raw(x, h, g, dx, dh)
+        return dx, dh, None

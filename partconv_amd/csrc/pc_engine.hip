@@ -1679,3 +1679,383 @@ PC_API int pcf_set_fs(pc_filter *h, int fs)
     h->E.FS = fs;
     return PC_OK;
 }
+
+// ============================================================================
+// Offline batched convolution and adjoint (pcc_*): see pc_offline.hpp.
+// ============================================================================
+#include "pc_offline.hpp"
+
+__global__ void k_fill_zero(float *p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+struct ConvLevel {
+    int N, P, T;
+    bool big;
+    BigShape sh;
+    cf *H = nullptr;
+};
+
+struct pc_conv {
+    int dev = 0;
+    int taps = 0;
+    std::vector<ConvLevel> lv;
+    cf *tw = nullptr;
+    // filter state
+    const float *h_src = nullptr;
+    bool filter_ready = false;
+    // work space (grown on demand)
+    cf *work = nullptr, *scr = nullptr, *dpart = nullptr, *dsum = nullptr;
+    size_t work_cap = 0, scr_cap = 0, dpart_cap = 0, dsum_cap = 0;
+    // spectra kept from forward for backward (per level), when within budget
+    std::vector<cf *> keep;
+    std::vector<size_t> keep_cap;
+    long long budget = 1ll << 30;
+    const float *keep_x = nullptr;
+    int keep_C = 0, keep_T = 0;
+    bool keep_valid = false;
+    long long recomputed_levels = 0;
+};
+
+static int conv_grow(cf **p, size_t *cap, size_t need)
+{
+    if (need <= *cap) return PC_OK;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    HIPCHK(hipMalloc(p, need * sizeof(cf)));
+    *cap = need;
+    return PC_OK;
+}
+
+template <int LOGN>
+static void off_rfft_t(const WinSrc &W, int ntrans, cf *X, const cf *tw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_off_rfft_small<LOGN>, dim3(ntrans), dim3(SmallCfg<LOGN>::NT), 0, s, W, ntrans, X, tw);
+}
+template <int LOGN>
+static void off_irfft_t(const cf *Y, const InvDst &D, int ntrans, const cf *tw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_off_irfft_small<LOGN>, dim3(ntrans), dim3(SmallCfg<LOGN>::NT), 0, s, Y, D, tw);
+}
+
+// batched r2c of `rows * W.nb` windows into X (N float2 each)
+static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, cf *X, hipStream_t s)
+{
+    const int nt = rows * W.nb;
+    if (nt == 0) return PC_OK;
+    if (L.big) {
+        int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)nt * L.N);
+        if (rc) return rc;
+        const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
+        hipLaunchKernelGGL(k_off_big_fwd1, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, W, L.sh,
+                           c->scr, c->tw);
+        hipLaunchKernelGGL(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, c->scr,
+                           X, W.scale, c->tw);
+    } else {
+        switch (ilog2(L.N)) {
+        case 4: off_rfft_t<4>(W, nt, X, c->tw, s); break;
+        case 5: off_rfft_t<5>(W, nt, X, c->tw, s); break;
+        case 6: off_rfft_t<6>(W, nt, X, c->tw, s); break;
+        case 7: off_rfft_t<7>(W, nt, X, c->tw, s); break;
+        case 8: off_rfft_t<8>(W, nt, X, c->tw, s); break;
+        case 9: off_rfft_t<9>(W, nt, X, c->tw, s); break;
+        case 10: off_rfft_t<10>(W, nt, X, c->tw, s); break;
+        case 11: off_rfft_t<11>(W, nt, X, c->tw, s); break;
+        default: return fail(PC_ERR_ARG, "block size");
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return PC_OK;
+}
+
+// batched c2r of `rows * D.nb` spectra (all blocks, or one parity class)
+static int off_irfft(pc_conv *c, const ConvLevel &L, const cf *Y, const InvDst &D, int rows, hipStream_t s)
+{
+    const int nt = (D.parity >= 0) ? rows * ((D.nb + 1 - D.parity) / 2) : rows * D.nb;
+    if (nt == 0) return PC_OK;
+    if (L.big) {
+        int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)rows * D.nb * L.N);
+        if (rc) return rc;
+        const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
+        hipLaunchKernelGGL(k_off_big_inv1, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, Y, c->scr,
+                           c->tw, D.parity, D.nb);
+        hipLaunchKernelGGL(k_off_big_inv2, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, L.sh, c->scr,
+                           D, c->tw);
+    } else {
+        switch (ilog2(L.N)) {
+        case 4: off_irfft_t<4>(Y, D, nt, c->tw, s); break;
+        case 5: off_irfft_t<5>(Y, D, nt, c->tw, s); break;
+        case 6: off_irfft_t<6>(Y, D, nt, c->tw, s); break;
+        case 7: off_irfft_t<7>(Y, D, nt, c->tw, s); break;
+        case 8: off_irfft_t<8>(Y, D, nt, c->tw, s); break;
+        case 9: off_irfft_t<9>(Y, D, nt, c->tw, s); break;
+        case 10: off_irfft_t<10>(Y, D, nt, c->tw, s); break;
+        case 11: off_irfft_t<11>(Y, D, nt, c->tw, s); break;
+        default: return fail(PC_ERR_ARG, "block size");
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return PC_OK;
+}
+
+static int conv_fill_zero(float *p, long long n, hipStream_t s)
+{
+    if (n <= 0) return PC_OK;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    HIPCHK(hipGetLastError());
+    return PC_OK;
+}
+
+PC_API pc_conv *pcc_create(int M, const int *N, const int *P, int taps)
+{
+    if (M < 1 || !N || !P || taps < 1) {
+        fail(PC_ERR_ARG, "bad argument");
+        return nullptr;
+    }
+    std::vector<int> n(N, N + M), p(P, P + M), delay, T;
+    if (check_partition(n, p, taps, delay, T)) return nullptr;
+    for (int L = 0; L < M; L++)
+        if (n[L] < PC_MIN_BLOCK || n[L] > PC_MAX_BLOCK || (n[L] > 2048 && n[L] < PC_BIG_MIN)) {
+            fail(PC_ERR_PARTITION, "block size outside the supported range");
+            return nullptr;
+        }
+    pc_conv *c = new pc_conv();
+    if (hipGetDevice(&c->dev) != hipSuccess || get_twiddles(c->dev, &c->tw) != PC_OK) {
+        std::string msg = g_err.empty() ? "no HIP device" : g_err;
+        delete c;
+        fail(PC_ERR_HIP, msg);
+        return nullptr;
+    }
+    c->taps = taps;
+    for (int L = 0; L < M; L++) {
+        ConvLevel cl;
+        cl.N = n[L];
+        cl.P = p[L];
+        cl.T = T[L];
+        cl.big = n[L] >= PC_BIG_MIN;
+        if (cl.big) cl.sh = big_shape(n[L]);
+        if (hipMalloc(&cl.H, (size_t)cl.P * cl.N * sizeof(cf)) != hipSuccess) {
+            fail(PC_ERR_HIP, "hipMalloc H");
+            for (ConvLevel &q : c->lv) hipFree(q.H);
+            delete c;
+            return nullptr;
+        }
+        c->lv.push_back(cl);
+    }
+    c->keep.assign(M, nullptr);
+    c->keep_cap.assign(M, 0);
+    return c;
+}
+
+PC_API int pcc_levels(const pc_conv *c, int *N, int *P, int cap)
+{
+    if (!c) return -1;
+    for (int L = 0; L < (int)c->lv.size() && L < cap; L++) {
+        if (N) N[L] = c->lv[L].N;
+        if (P) P[L] = c->lv[L].P;
+    }
+    return (int)c->lv.size();
+}
+
+PC_API int pcc_set_budget(pc_conv *c, long long bytes)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    c->budget = bytes;
+    return PC_OK;
+}
+
+PC_API long long pcc_recomputed_levels(const pc_conv *c) { return c ? c->recomputed_levels : -1; }
+
+// H_p = r2c([0_N | h[T_L + pN, T_L + (p+1)N)]) / 2N for every level
+PC_API int pcc_set_filter(pc_conv *c, const float *d_h, void *stream)
+{
+    if (!c || !d_h) return fail(PC_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    for (ConvLevel &L : c->lv) {
+        WinSrc W{};
+        W.src = d_h;
+        W.row_stride = 0;
+        W.len = c->taps;
+        W.base0 = (long long)L.T - L.N;
+        W.base_step = L.N;
+        W.nb = L.P;
+        W.ulo = L.N;
+        W.uhi = 2 * L.N;
+        W.scale = 1.0f / (float)(2 * L.N);
+        int rc = off_rfft(c, L, W, 1, L.H, s);
+        if (rc) return rc;
+    }
+    c->h_src = d_h;
+    c->filter_ready = true;
+    return PC_OK;
+}
+
+static WinSrc conv_xwin(const ConvLevel &L, const float *x, int T)
+{
+    WinSrc W{};
+    W.src = x;
+    W.row_stride = T;
+    W.len = T;
+    W.base0 = -(long long)L.N - L.T;
+    W.base_step = L.N;
+    W.nb = (T + L.N - 1) / L.N;
+    W.ulo = 0;
+    W.uhi = 2 * L.N;
+    W.scale = 1.f;
+    return W;
+}
+
+// y = sum over levels of the partitioned convolution of every clip (x, y: [C][T])
+PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
+{
+    if (!c || !d_x || !d_y || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "forward before set_filter");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = conv_fill_zero(d_y, (long long)C * T, s);
+    if (rc) return rc;
+    size_t need_keep = 0;
+    for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
+    const bool keep = (long long)need_keep <= c->budget;
+    c->keep_valid = false;
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        ConvLevel &L = c->lv[li];
+        WinSrc W = conv_xwin(L, d_x, T);
+        const int M = W.nb;
+        const size_t n = (size_t)C * M * L.N;
+        cf *X;
+        if (keep) {
+            if ((rc = conv_grow(&c->keep[li], &c->keep_cap[li], n))) return rc;
+            X = c->keep[li];
+        } else {
+            if ((rc = conv_grow(&c->dsum, &c->dsum_cap, n))) return rc;
+            X = c->dsum;
+        }
+        if ((rc = conv_grow(&c->work, &c->work_cap, n))) return rc;
+        if ((rc = off_rfft(c, L, W, C, X, s))) return rc;
+        const long long nthr = (long long)C * M * (L.N / 2);
+        hipLaunchKernelGGL(k_off_mac, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, X, L.H, c->work, C, M,
+                           L.N, L.P, 1);
+        HIPCHK(hipGetLastError());
+        InvDst D{};
+        D.dst = d_y;
+        D.row_stride = T;
+        D.len = T;
+        D.base0 = 0;
+        D.base_step = L.N;
+        D.nb = M;
+        D.ulo = 0;
+        D.uhi = L.N;
+        D.scale = 1.f;
+        D.accumulate = 1;
+        D.parity = -1;
+        if ((rc = off_irfft(c, L, c->work, D, C, s))) return rc;
+    }
+    if (keep) {
+        c->keep_valid = true;
+        c->keep_x = d_x;
+        c->keep_C = C;
+        c->keep_T = T;
+    }
+    return PC_OK;
+}
+
+// Adjoint of pcc_forward for upstream gradient g: dx (may be null) and dh (may
+// be null).  Uses the forward's kept spectra when they are for the same x and
+// within the budget, else recomputes them from x.
+PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
+                        void *stream)
+{
+    if (!c || !d_x || !d_g || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "backward before set_filter");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (d_dx && (rc = conv_fill_zero(d_dx, (long long)C * T, s))) return rc;
+    const bool kept = c->keep_valid && c->keep_x == d_x && c->keep_C == C && c->keep_T == T;
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        ConvLevel &L = c->lv[li];
+        const int M = (T + L.N - 1) / L.N;
+        const size_t n = (size_t)C * M * L.N;
+        // G = r2c([g block | 0_N]); second half of work: W (dx) or D_p (dh)
+        if ((rc = conv_grow(&c->work, &c->work_cap, n + std::max(n, (size_t)L.P * L.N)))) return rc;
+        cf *G = c->work, *Wsp = c->work + n;
+        WinSrc Wg{};
+        Wg.src = d_g;
+        Wg.row_stride = T;
+        Wg.len = T;
+        Wg.base0 = 0;
+        Wg.base_step = L.N;
+        Wg.nb = M;
+        Wg.ulo = 0;
+        Wg.uhi = L.N;
+        Wg.scale = 1.f;
+        if ((rc = off_rfft(c, L, Wg, C, G, s))) return rc;
+        if (d_dx) {
+            const long long nthr = (long long)C * M * (L.N / 2);
+            hipLaunchKernelGGL(k_off_mac, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, G, L.H, Wsp, C,
+                               M, L.N, L.P, -1);
+            HIPCHK(hipGetLastError());
+            InvDst D{};
+            D.dst = d_dx;
+            D.row_stride = T;
+            D.len = T;
+            D.base0 = -(long long)L.N - L.T;
+            D.base_step = L.N;
+            D.nb = M;
+            D.ulo = 0;
+            D.uhi = 2 * L.N;
+            D.scale = 1.f;
+            D.accumulate = 1;
+            for (int par = 0; par < 2; par++) {
+                D.parity = par;
+                if ((rc = off_irfft(c, L, Wsp, D, C, s))) return rc;
+            }
+        }
+        if (d_dh) {
+            cf *X;
+            if (kept) X = c->keep[li];
+            else {
+                // recompute the delay-line spectra from the stored time-domain input
+                if ((rc = conv_grow(&c->dsum, &c->dsum_cap, n))) return rc;
+                X = c->dsum;
+                if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, s))) return rc;
+                c->recomputed_levels++;
+            }
+            if ((rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * L.N))) return rc;
+            const int kb = (L.N / 2 + 255) / 256;
+            hipLaunchKernelGGL(k_off_dh_part, dim3(L.P * C * kb), dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P);
+            const long long nd = (long long)L.P * L.N;
+            hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, c->dpart, Wsp,
+                               C, L.N, L.P);
+            HIPCHK(hipGetLastError());
+            InvDst D{};
+            D.dst = d_dh;
+            D.row_stride = 0;
+            D.len = c->taps;
+            D.base0 = (long long)L.T - L.N;
+            D.base_step = L.N;
+            D.nb = L.P;
+            D.ulo = L.N;
+            D.uhi = 2 * L.N;
+            D.scale = 1.0f / (float)(2 * L.N);
+            D.accumulate = 0;
+            D.parity = -1;
+            if ((rc = off_irfft(c, L, Wsp, D, 1, s))) return rc;
+        }
+    }
+    return PC_OK;
+}
+
+PC_API void pcc_destroy(pc_conv *c)
+{
+    if (!c) return;
+    hipDeviceSynchronize();
+    for (ConvLevel &L : c->lv) hipFree(L.H);
+    for (cf *p : c->keep) hipFree(p);
+    hipFree(c->work);
+    hipFree(c->scr);
+    hipFree(c->dpart);
+    hipFree(c->dsum);
+    delete c;
+}

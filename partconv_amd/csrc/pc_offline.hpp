@@ -1,0 +1,386 @@
+// pc_offline.hpp -- batched offline partitioned convolution and its adjoint
+// (the differentiable form): whole clips [C, T] are resident, every block of
+// every level is computed at once, and the same FFT building blocks and
+// partition rules as the real-time engine are used.
+//
+// Per level L (block N, P partitions, T_L taps before it), output block m
+// covers samples [mN, (m+1)N):
+//   X_{c,m} = r2c( x[c, (m-1)N - T_L + u], u in [0,2N) )          (zero outside [0,T))
+//   Y_{c,m} = sum_{p=0}^{min(P-1,m)} X_{c,m-p} H_p                 (H_p as the engine's)
+//   y[c, mN+n] += c2r(Y_{c,m})[n],  n < N
+// Adjoint, with G_{c,m} = r2c([ g[c, mN .. mN+N) | 0_N ]):
+//   dx: W_{c,m} = sum_{p: m+p<M} G_{c,m+p} conj(H_p);
+//       dx[c, (m-1)N - T_L + u] += c2r(W_{c,m})[u], u in [0,2N)     (overlap-add)
+//   dh: D_p = sum_{c} sum_{m>=p} G_{c,m} conj(X_{c,m-p});
+//       dh[T_L + pN + j] = c2r(D_p)[N + j] / (2N)
+// (circular convolution with [0_N | h_p] and its two adjoints: circular
+// correlation with the kernel, and with the window.)
+#pragma once
+#include "pc_kernels.hpp"
+
+// Window source of a batched r2c: transform t = r * nb + m reads
+//   w[u] = src[r*row_stride + base0 + m*base_step + u]  for ulo <= u < uhi and
+//          0 <= base0 + m*base_step + u < len, else 0;   X *= scale.
+struct WinSrc {
+    const float *src;
+    long long row_stride;
+    long long len;
+    long long base0, base_step;
+    int nb;
+    int ulo, uhi;
+    float scale;
+};
+
+// Destination of a batched c2r: transform t = r * nb + m writes samples u of
+// [ulo, uhi) (scaled) to dst[r*row_stride + base0 + m*base_step + u] when that
+// index lies in [0, len); accumulate adds.  parity >= 0 restricts the launch
+// to blocks m with m % 2 == parity (overlapping windows never race).
+struct InvDst {
+    float *dst;
+    long long row_stride;
+    long long len;
+    long long base0, base_step;
+    int nb;
+    int ulo, uhi;
+    float scale;
+    int accumulate;
+    int parity;
+};
+
+__device__ __forceinline__ float win_sample(const WinSrc &W, int r, int m, int u)
+{
+    if (u < W.ulo || u >= W.uhi) return 0.f;
+    const long long i = W.base0 + (long long)m * W.base_step + u;
+    return (i >= 0 && i < W.len) ? W.src[(long long)r * W.row_stride + i] : 0.f;
+}
+
+__device__ __forceinline__ void inv_store(const InvDst &D, int r, int m, int u, float v)
+{
+    if (u < D.ulo || u >= D.uhi) return;
+    const long long i = D.base0 + (long long)m * D.base_step + u;
+    if (i < 0 || i >= D.len) return;
+    float *p = D.dst + (long long)r * D.row_stride + i;
+    v *= D.scale;
+    if (D.accumulate) *p += v;
+    else *p = v;
+}
+
+// ---- small N (<= 2048): one workgroup per transform -----------------------
+template <int LOGN>
+__global__ void __launch_bounds__(SmallCfg<LOGN>::NT)
+k_off_rfft_small(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict__ tw)
+{
+    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
+    __shared__ cf bufA[N];
+    __shared__ cf bufB[N];
+    const int t = blockIdx.x;
+    const int r = t / W.nb, m = t - r * W.nb;
+    const int tid = threadIdx.x;
+    for (int n = tid; n < N; n += NT)
+        bufA[n] = make_float2(win_sample(W, r, m, 2 * n), win_sample(W, r, m, 2 * n + 1));
+    __syncthreads();
+    cf *Z = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
+    cf *o = X + (size_t)t * N;
+    for (int k = tid; k < N; k += NT) o[k] = cscale(Z[k], W.scale);
+    (void)ntrans;
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(SmallCfg<LOGN>::NT)
+k_off_irfft_small(const cf *__restrict__ Y, InvDst D, const cf *__restrict__ tw)
+{
+    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
+    __shared__ cf bufA[N];
+    __shared__ cf bufB[N];
+    int t = blockIdx.x;
+    if (D.parity >= 0) {
+        // blockIdx enumerates (row, m) with m of the requested parity
+        const int half = (D.nb + 1 - D.parity) / 2;
+        const int r = t / half, mi = t - r * half;
+        t = r * D.nb + 2 * mi + D.parity;
+    }
+    const int r = t / D.nb, m = t - r * D.nb;
+    const int tid = threadIdx.x;
+    const cf *y = Y + (size_t)t * N;
+    for (int k = tid; k < N; k += NT) bufA[k] = y[k];
+    __syncthreads();
+    cf *z = small_inverse<LOGN, NT>(bufA, bufB, tw, tid);
+    const float *zf = reinterpret_cast<const float *>(z);
+    for (int u = tid; u < 2 * N; u += NT) inv_store(D, r, m, u, zf[u]);
+}
+
+// ---- big N (four-step, as k_big1..3) ---------------------------------------
+// S (scratch) holds N float2 per transform.
+__global__ void __launch_bounds__(BIG_NT)
+k_off_big_fwd1(WinSrc W, BigShape sh, cf *__restrict__ S, const cf *__restrict__ tw)
+{
+    extern __shared__ cf lds[];
+    const int A = 1 << sh.logA, B = 1 << sh.logB, N = A * B;
+    const int sst = B + 1;
+    cf *bufA = lds, *bufB = lds + BIG_G * sst;
+    const int groups = A / BIG_G;
+    const int t = blockIdx.x / groups, g = blockIdx.x - t * groups;
+    const int r = t / W.nb, m = t - r * W.nb;
+    const int tid = threadIdx.x;
+    const int a0 = g * BIG_G;
+    for (int e = tid; e < BIG_G * B; e += BIG_NT) {
+        const int b = e / BIG_G, ai = e - b * BIG_G;
+        const int n = a0 + ai + A * b;
+        bufA[ai * sst + b] = make_float2(win_sample(W, r, m, 2 * n), win_sample(W, r, m, 2 * n + 1));
+    }
+    __syncthreads();
+    cf *R = fft_lds<false>(bufA, bufB, sh.logB, BIG_G, sst, tw, tid, BIG_NT);
+    cf *St = S + (size_t)t * N;
+    for (int e = tid; e < BIG_G * B; e += BIG_NT) {
+        const int c = e / BIG_G, ai = e - c * BIG_G;
+        const int a = a0 + ai;
+        St[(size_t)c * A + a] = cmul(R[ai * sst + c], tw[N + ((a * c) & (N - 1))]);
+    }
+}
+
+// column-pair DFTs + real split -> X (four-step slot order), scaled
+__global__ void __launch_bounds__(BIG_NT)
+k_off_big_fwd2(BigShape sh, const cf *__restrict__ S, cf *__restrict__ X, float scale,
+               const cf *__restrict__ tw)
+{
+    extern __shared__ cf lds[];
+    const int A = 1 << sh.logA, B = 1 << sh.logB, N = A * B;
+    const int sst = A + 1;
+    cf *bufA = lds, *bufB = lds + 2 * sst;
+    const int pairs = B / 2;
+    const int t = blockIdx.x / pairs, u = blockIdx.x - t * pairs;
+    const int tid = threadIdx.x;
+    const int c1 = u, c2 = u ? (B - u) : (B / 2);
+    const cf *St = S + (size_t)t * N;
+    for (int e = tid; e < 2 * A; e += BIG_NT) {
+        const int q = e >> sh.logA, a = e & (A - 1);
+        bufA[q * sst + a] = St[(size_t)(q ? c2 : c1) * A + a];
+    }
+    __syncthreads();
+    cf *Z = fft_lds<false>(bufA, bufB, sh.logA, 2, sst, tw, tid, BIG_NT);
+    cf *W = (Z == bufA) ? bufB : bufA;
+    if (u != 0) {
+        for (int d = tid; d < A; d += BIG_NT) {
+            cf xk, xnk;
+            rsplit_fwd(Z[d], Z[sst + (A - 1 - d)], tw[2 * N + c1 + B * d], &xk, &xnk);
+            W[d] = xk;
+            W[sst + (A - 1 - d)] = xnk;
+        }
+    } else {
+        for (int e = tid; e < A + 1; e += BIG_NT) {
+            if (e <= A / 2) {
+                const int d = e;
+                if (d == 0) {
+                    const cf zz = Z[0];
+                    W[0] = make_float2(zz.x + zz.y, zz.x - zz.y);
+                } else {
+                    cf xk, xnk;
+                    rsplit_fwd(Z[d], Z[A - d], tw[2 * N + B * d], &xk, &xnk);
+                    W[d] = xk;
+                    W[A - d] = xnk;
+                }
+            } else {
+                const int d = e - (A / 2 + 1);
+                cf xk, xnk;
+                rsplit_fwd(Z[sst + d], Z[sst + (A - 1 - d)], tw[2 * N + B / 2 + B * d], &xk, &xnk);
+                W[sst + d] = xk;
+                W[sst + (A - 1 - d)] = xnk;
+            }
+        }
+    }
+    __syncthreads();
+    cf *Xt = X + (size_t)t * N + (size_t)u * 2 * A;
+    for (int e = tid; e < 2 * A; e += BIG_NT) {
+        const int q = e >> sh.logA, d = e & (A - 1);
+        Xt[e] = cscale(W[q * sst + d], scale);
+    }
+}
+
+// inverse split + inverse column DFTs + twiddle -> S
+__global__ void __launch_bounds__(BIG_NT)
+k_off_big_inv1(BigShape sh, const cf *__restrict__ Y, cf *__restrict__ S, const cf *__restrict__ tw, int parity,
+               int nb)
+{
+    extern __shared__ cf lds[];
+    const int A = 1 << sh.logA, B = 1 << sh.logB, N = A * B;
+    const int sst = A + 1;
+    cf *bufA = lds, *bufB = lds + 2 * sst;
+    const int pairs = B / 2;
+    int t = blockIdx.x / pairs;
+    const int u = blockIdx.x - t * pairs;
+    if (parity >= 0) {
+        const int half = (nb + 1 - parity) / 2;
+        const int r = t / half, mi = t - r * half;
+        t = r * nb + 2 * mi + parity;
+    }
+    const int tid = threadIdx.x;
+    const int c1 = u, c2 = u ? (B - u) : (B / 2);
+    const cf *Yt = Y + (size_t)t * N + (size_t)u * 2 * A;
+    cf *W = bufB;
+    for (int e = tid; e < 2 * A; e += BIG_NT) {
+        const int q = e >> sh.logA, d = e & (A - 1);
+        W[q * sst + d] = Yt[e];
+    }
+    __syncthreads();
+    cf *Z = bufA;
+    if (u != 0) {
+        for (int d = tid; d < A; d += BIG_NT) {
+            cf ck, cnk;
+            rsplit_inv(W[d], W[sst + (A - 1 - d)], tw[2 * N + c1 + B * d], &ck, &cnk);
+            Z[d] = ck;
+            Z[sst + (A - 1 - d)] = cnk;
+        }
+    } else {
+        for (int e = tid; e < A + 1; e += BIG_NT) {
+            if (e <= A / 2) {
+                const int d = e;
+                if (d == 0) {
+                    const cf y = W[0];
+                    Z[0] = make_float2(y.x + y.y, y.x - y.y);
+                } else {
+                    cf ck, cnk;
+                    rsplit_inv(W[d], W[A - d], tw[2 * N + B * d], &ck, &cnk);
+                    Z[d] = ck;
+                    Z[A - d] = cnk;
+                }
+            } else {
+                const int d = e - (A / 2 + 1);
+                cf ck, cnk;
+                rsplit_inv(W[sst + d], W[sst + (A - 1 - d)], tw[2 * N + B / 2 + B * d], &ck, &cnk);
+                Z[sst + d] = ck;
+                Z[sst + (A - 1 - d)] = cnk;
+            }
+        }
+    }
+    __syncthreads();
+    cf *U = fft_lds<true>(Z, W, sh.logA, 2, sst, tw, tid, BIG_NT);
+    cf *St = S + (size_t)t * N;
+    for (int e = tid; e < 2 * A; e += BIG_NT) {
+        const int q = e >> sh.logA, a = e & (A - 1);
+        const int c = q ? c2 : c1;
+        St[(size_t)c * A + a] = cmulc(U[q * sst + a], tw[N + ((a * c) & (N - 1))]);
+    }
+}
+
+// inverse row DFTs -> samples u = 2n, 2n+1 of z'[n], n = a + A b
+__global__ void __launch_bounds__(BIG_NT)
+k_off_big_inv2(BigShape sh, const cf *__restrict__ S, InvDst D, const cf *__restrict__ tw)
+{
+    extern __shared__ cf lds[];
+    const int A = 1 << sh.logA, B = 1 << sh.logB, N = A * B;
+    const int sst = B + 1;
+    cf *bufA = lds, *bufB = lds + BIG_G * sst;
+    const int groups = A / BIG_G;
+    int t = blockIdx.x / groups;
+    const int g = blockIdx.x - t * groups;
+    if (D.parity >= 0) {
+        const int half = (D.nb + 1 - D.parity) / 2;
+        const int r = t / half, mi = t - r * half;
+        t = r * D.nb + 2 * mi + D.parity;
+    }
+    const int r = t / D.nb, m = t - r * D.nb;
+    const int tid = threadIdx.x;
+    const int a0 = g * BIG_G;
+    const cf *St = S + (size_t)t * N;
+    for (int e = tid; e < BIG_G * B; e += BIG_NT) {
+        const int c = e / BIG_G, ai = e - c * BIG_G;
+        bufA[ai * sst + c] = St[(size_t)c * A + a0 + ai];
+    }
+    __syncthreads();
+    cf *R = fft_lds<true>(bufA, bufB, sh.logB, BIG_G, sst, tw, tid, BIG_NT);
+    for (int e = tid; e < BIG_G * B; e += BIG_NT) {
+        const int b = e / BIG_G, ai = e - b * BIG_G;
+        const int n = a0 + ai + A * b;
+        const cf v = R[ai * sst + b];
+        inv_store(D, r, m, 2 * n, v.x);
+        inv_store(D, r, m, 2 * n + 1, v.y);
+    }
+}
+
+// ---- spectral products ------------------------------------------------------
+// Y[c,m] = sum_p X[c, m - p] H_p          (dir = +1, causal)
+// Y[c,m] = sum_p X[c, m + p] conj(H_p)    (dir = -1, adjoint)
+// spectra laid out [C][M][N] float2; each thread owns two bins of one (c, m).
+__global__ void __launch_bounds__(256)
+k_off_mac(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y, int C, int M, int N, int P,
+          int dir)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int half = N / 2;
+    if (i >= (long long)C * M * half) return;
+    const int k2 = (int)(i % half);
+    const long long cm = i / half;
+    const int m = (int)(cm % M), c = (int)(cm / M);
+    const float4 *Xr = reinterpret_cast<const float4 *>(X) + (size_t)c * M * half;
+    const float4 *Hr = reinterpret_cast<const float4 *>(H);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int p = 0; p < P; p++) {
+        const int mm = m - dir * p;
+        if (mm < 0 || mm >= M) break;
+        const float4 a = Xr[(size_t)mm * half + k2];
+        float4 h = Hr[(size_t)p * half + k2];
+        if (dir < 0) {
+            h.y = -h.y;
+            h.w = -h.w;
+        }
+        if (k2 == 0) {
+            acc.x += a.x * h.x;
+            acc.y += a.y * (dir < 0 ? -h.y : h.y);
+        } else {
+            acc.x += a.x * h.x - a.y * h.y;
+            acc.y += a.x * h.y + a.y * h.x;
+        }
+        acc.z += a.z * h.z - a.w * h.w;
+        acc.w += a.z * h.w + a.w * h.z;
+    }
+    reinterpret_cast<float4 *>(Y)[i] = acc;
+}
+
+// D partial sums: part[p][j][k] = sum_{(c,m) in chunk j, m >= p} G[c,m] conj(X[c,m-p]);
+// chunk j = clip j (grid: P x C x ceil(N/2/256)).
+__global__ void __launch_bounds__(256)
+k_off_dh_part(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict__ part, int C, int M, int N,
+              int P)
+{
+    const int half = N / 2;
+    const int kb = (half + 255) / 256;
+    const int b = blockIdx.x;
+    const int p = b / (C * kb);
+    const int rem = b - p * C * kb;
+    const int c = rem / kb;
+    const int k2 = (rem - c * kb) * 256 + threadIdx.x;
+    if (k2 >= half) return;
+    const float4 *Gr = reinterpret_cast<const float4 *>(G) + (size_t)c * M * half;
+    const float4 *Xr = reinterpret_cast<const float4 *>(X) + (size_t)c * M * half;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int m = p; m < M; m++) {
+        const float4 g = Gr[(size_t)m * half + k2];
+        const float4 x = Xr[(size_t)(m - p) * half + k2];
+        if (k2 == 0) {
+            acc.x += g.x * x.x;
+            acc.y += g.y * x.y;
+        } else {
+            acc.x += g.x * x.x + g.y * x.y;
+            acc.y += g.y * x.x - g.x * x.y;
+        }
+        acc.z += g.z * x.z + g.w * x.w;
+        acc.w += g.w * x.z - g.z * x.w;
+    }
+    reinterpret_cast<float4 *>(part)[((size_t)p * C + c) * half + k2] = acc;
+}
+
+// D[p][k] = sum_j part[p][j][k] (clip order)
+__global__ void __launch_bounds__(256)
+k_off_dh_reduce(const cf *__restrict__ part, cf *__restrict__ D, int C, int N, int P)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)P * N) return;
+    const int p = (int)(i / N), k = (int)(i % N);
+    cf acc = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; c++) {
+        const cf v = part[((size_t)p * C + c) * N + k];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    D[i] = acc;
+}
