@@ -262,6 +262,22 @@ int pcc_set_budget(pc_conv *c, long long bytes);
 long long pcc_recomputed_levels(const pc_conv *c);
 void pcc_destroy(pc_conv *c);
 
+/* ----------------------------------------------------------------------------
+ * MIMO throughput form: y[o] = sum_i conv(x[i], h[o][i]) for up to 32 x 32
+ * channels over whole device-resident signals.  In the reference this is
+ * n_in * n_out PartConvMulti instances summed into the outputs
+ * (src/partconvMulti.cpp:906-921); here every frequency bin's complex
+ * (n_out x n_in) matrix product over all output blocks runs on the fp32
+ * matrix cores.  (The real-time form is pcm_* with one instance per pair.)
+ * ------------------------------------------------------------------------- */
+typedef struct pc_mimo pc_mimo;
+pc_mimo *pcx_create(int M, const int *N, const int *P, int taps, int n_in, int n_out);
+/* h: [n_out][n_in][taps] on the device */
+int pcx_set_filters(pc_mimo *x, const float *d_h, void *stream);
+/* x: [n_in][T], y: [n_out][T] on the device */
+int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *stream);
+void pcx_destroy(pc_mimo *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,10 @@ SIGNATURES = {
     "pcc_set_budget": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),
     "pcc_destroy": (None, [_vp]),
+    "pcx_create": (_vp, [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pcx_set_filters": (ctypes.c_int, [_vp, _vp, _vp]),
+    "pcx_forward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
+    "pcx_destroy": (None, [_vp]),
 }
 
 

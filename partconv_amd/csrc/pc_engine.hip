@@ -208,6 +208,9 @@ struct Engine {
     float *y_inst = nullptr;
     int *d_src_off = nullptr, *d_src_list = nullptr;
     unsigned *d_ticket = nullptr;
+    unsigned *d_ticket_ch = nullptr;     // per output channel (reduce mix)
+    int *d_empty = nullptr;              // output channels without a source
+    int nactive = 0, nempty = 0;
     float *h_in = nullptr, *h_out = nullptr;
     unsigned long long *h_flag = nullptr;
     float *d_in_map = nullptr, *d_out_map = nullptr;   // device views of the pinned buffers
@@ -249,6 +252,10 @@ static void engine_free(Engine &E)
     hipFree(E.d_src_off);
     hipFree(E.d_src_list);
     hipFree(E.d_ticket);
+    hipFree(E.d_ticket_ch);
+    hipFree(E.d_empty);
+    E.d_ticket_ch = nullptr;
+    E.d_empty = nullptr;
     E.y_inst = nullptr;
     E.d_src_off = E.d_src_list = nullptr;
     E.d_ticket = nullptr;
@@ -615,8 +622,9 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         E.groups.push_back(Gr);
     }
     // ---- output mix routing
-    std::vector<int> src_off(nOut + 1, 0), src_list;
+    std::vector<int> src_off(nOut + 1, 0), src_list, empty;
     bool direct = (E.mode == PC_BATCHED);
+    E.nactive = 0;
     for (int c = 0; c < nOut; c++) {
         int cnt = 0;
         for (size_t i = 0; i < E.inst.size(); i++)
@@ -626,8 +634,17 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
             }
         src_off[c + 1] = (int)src_list.size();
         if (cnt != 1) direct = false;
+        if (cnt) E.nactive++;
+        else empty.push_back(c);
     }
-    E.mix = (E.mode == PC_PLAIN) ? 0 : (direct ? 1 : 2);
+    E.nempty = (int)empty.size();
+    HIPCHK(hipMalloc(&E.d_ticket_ch, std::max(nOut, 1) * sizeof(unsigned)));
+    HIPCHK(hipMemset(E.d_ticket_ch, 0, std::max(nOut, 1) * sizeof(unsigned)));
+    HIPCHK(hipMalloc(&E.d_empty, std::max<size_t>(empty.size(), 1) * sizeof(int)));
+    if (!empty.empty())
+        HIPCHK(hipMemcpy(E.d_empty, empty.data(), empty.size() * sizeof(int), hipMemcpyHostToDevice));
+    // 0: per-instance outputs + k_mix (plain engine, or nothing unmuted to mix)
+    E.mix = (E.mode == PC_PLAIN) ? 0 : (direct ? 1 : (E.nactive > 0 ? 2 : 0));
     HIPCHK(hipMalloc(&E.y_inst, E.inst.size() * (size_t)N0 * sizeof(float)));
     HIPCHK(hipMalloc(&E.d_src_off, src_off.size() * sizeof(int)));
     HIPCHK(hipMemcpy(E.d_src_off, src_off.data(), src_off.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -703,6 +720,7 @@ static int engine_reset(Engine &E)
         }
     }
     HIPCHK(hipMemsetAsync(E.d_ticket, 0, 64, E.s_main));
+    HIPCHK(hipMemsetAsync(E.d_ticket_ch, 0, std::max(E.nOut, 1) * sizeof(unsigned), E.s_main));
     HIPCHK(hipStreamSynchronize(E.s_main));
     E.frame = 0;
     std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
@@ -879,6 +897,10 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     a.src_off = E.d_src_off;
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
+    a.ticket_ch = E.d_ticket_ch;
+    a.nactive = E.nactive;
+    a.empty_ch = E.d_empty;
+    a.nempty = E.nempty;
     // host-memory output needs a system-scope release before the flag; device
     // output is published to later kernels by the kernel boundary itself
     a.fence_sys = (out == E.d_out_map) ? 1 : 0;
@@ -898,6 +920,12 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         }
         int rc = launch_tick(E.N0, a, E.tw, E.s_main, x);
         if (rc) return rc;
+        if (E.mix == 0) {
+            const int n = E.nOut * E.N0;
+            hipLaunchKernelGGL(k_mix, dim3((n + 255) / 256), dim3(256), 0, E.s_main, E.y_inst, E.d_src_off,
+                               E.d_src_list, E.N0, E.nOut, out, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
+                               E.d_flag_map, f);
+        }
     } else {
         for (Group &Gr : E.groups) {
             a.jobs = Gr.d_tick;
@@ -973,6 +1001,7 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     const uint64_t f0 = E.frame;
     const long long in_stride = (long long)E.N0 * E.nIn, out_stride = (long long)E.N0 * E.nOut;
     *persistent_ticks = 0;
+    if (E.mix == 0) return 1;  // separate mix kernel per tick: per-tick launches
     PersistArgs P{};
     TickArgs &a = P.A;
     Group &Gr = E.groups[0];
@@ -992,6 +1021,10 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.src_off = E.d_src_off;
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
+    a.ticket_ch = E.d_ticket_ch;
+    a.nactive = E.nactive;
+    a.empty_ch = E.d_empty;
+    a.nempty = E.nempty;
     a.done_flag = E.d_flag_map;
     a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
     a.fence_sys = 0;
@@ -2058,4 +2091,121 @@ PC_API void pcc_destroy(pc_conv *c)
     hipFree(c->dpart);
     hipFree(c->dsum);
     delete c;
+}
+
+// ---------------------------------------------------------------------------
+// MIMO throughput form (pcm-free, offline): y[o] = sum_i conv(x[i], h[o][i]).
+// ---------------------------------------------------------------------------
+struct pc_mimo {
+    pc_conv *c = nullptr;  // partition, twiddles, transforms and work space
+    int n_in = 0, n_out = 0;
+    std::vector<cf *> Ht;  // per level: [N][P][n_in][32]
+    cf *tmp = nullptr;
+    size_t tmp_cap = 0;
+};
+
+PC_API pc_mimo *pcx_create(int M, const int *N, const int *P, int taps, int n_in, int n_out)
+{
+    if (n_in < 1 || n_in > 32 || n_out < 1 || n_out > 32) {
+        fail(PC_ERR_ARG, "MIMO sizes must be 1..32");
+        return nullptr;
+    }
+    pc_conv *c = pcc_create(M, N, P, taps);
+    if (!c) return nullptr;
+    pc_mimo *x = new pc_mimo();
+    x->c = c;
+    x->n_in = n_in;
+    x->n_out = n_out;
+    for (ConvLevel &L : c->lv) {
+        cf *h = nullptr;
+        if (hipMalloc(&h, (size_t)L.N * L.P * n_in * 32 * sizeof(cf)) != hipSuccess) {
+            fail(PC_ERR_HIP, "hipMalloc MIMO spectra");
+            for (cf *q : x->Ht) hipFree(q);
+            pcc_destroy(c);
+            delete x;
+            return nullptr;
+        }
+        x->Ht.push_back(h);
+    }
+    return x;
+}
+
+// d_h: [n_out][n_in][taps] on the device
+PC_API int pcx_set_filters(pc_mimo *x, const float *d_h, void *stream)
+{
+    if (!x || !d_h) return fail(PC_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    pc_conv *c = x->c;
+    const int R = x->n_in * x->n_out;
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        ConvLevel &L = c->lv[li];
+        int rc = conv_grow(&x->tmp, &x->tmp_cap, (size_t)R * L.P * L.N);
+        if (rc) return rc;
+        WinSrc W{};
+        W.src = d_h;
+        W.row_stride = c->taps;
+        W.len = c->taps;
+        W.base0 = (long long)L.T - L.N;
+        W.base_step = L.N;
+        W.nb = L.P;
+        W.ulo = L.N;
+        W.uhi = 2 * L.N;
+        W.scale = 1.0f / (float)(2 * L.N);
+        if ((rc = off_rfft(c, L, W, R, x->tmp, s))) return rc;
+        const long long n = (long long)L.N * L.P * x->n_in * 32;
+        hipLaunchKernelGGL(k_h_transpose, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x->tmp, x->Ht[li],
+                           x->n_in, x->n_out, L.P, L.N);
+        HIPCHK(hipGetLastError());
+    }
+    return PC_OK;
+}
+
+// y[n_out][T] = sum_i conv(x[i], h[o][i]) (x: [n_in][T], device)
+PC_API int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *stream)
+{
+    if (!x || !d_x || !d_y || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    pc_conv *c = x->c;
+    int rc = conv_fill_zero(d_y, (long long)x->n_out * T, s);
+    if (rc) return rc;
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        ConvLevel &L = c->lv[li];
+        WinSrc W = conv_xwin(L, d_x, T);
+        const int M = W.nb;
+        const size_t nx = (size_t)x->n_in * M * L.N, ny = (size_t)x->n_out * M * L.N;
+        if ((rc = conv_grow(&c->dsum, &c->dsum_cap, nx))) return rc;
+        if ((rc = conv_grow(&x->tmp, &x->tmp_cap, nx))) return rc;
+        if ((rc = conv_grow(&c->work, &c->work_cap, ny))) return rc;
+        if ((rc = off_rfft(c, L, W, x->n_in, c->dsum, s))) return rc;
+        const long long n = (long long)nx;
+        hipLaunchKernelGGL(k_spec_transpose, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dsum, x->tmp,
+                           x->n_in, M, L.N);
+        hipLaunchKernelGGL(k_mimo_mfma, dim3(L.N, (M + 31) / 32), dim3(64), 0, s, x->Ht[li], x->tmp, c->work, L.N,
+                           M, L.P, x->n_in, x->n_out);
+        HIPCHK(hipGetLastError());
+        InvDst D{};
+        D.dst = d_y;
+        D.row_stride = T;
+        D.len = T;
+        D.base0 = 0;
+        D.base_step = L.N;
+        D.nb = M;
+        D.ulo = 0;
+        D.uhi = L.N;
+        D.scale = 1.f;
+        D.accumulate = 1;
+        D.parity = -1;
+        if ((rc = off_irfft(c, L, c->work, D, x->n_out, s))) return rc;
+    }
+    return PC_OK;
+}
+
+PC_API void pcx_destroy(pc_mimo *x)
+{
+    if (!x) return;
+    hipDeviceSynchronize();
+    for (cf *q : x->Ht) hipFree(q);
+    hipFree(x->tmp);
+    pcc_destroy(x->c);
+    delete x;
 }

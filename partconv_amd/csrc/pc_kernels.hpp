@@ -75,6 +75,10 @@ struct TickArgs {
     unsigned long long *wg_flags;   // direct: one host-visible completion flag per workgroup
     int fence_sys;                  // output lives in host memory: system-scope release per tick
     unsigned long long fence_mask;  // else: device-scope release at ticks with (f & mask) == 0
+    unsigned *ticket_ch;            // reduce: arrival counter per output channel
+    int nactive;                    // reduce: channels with at least one unmuted source
+    const int *empty_ch;            // reduce: channels with no source (zeroed each tick)
+    int nempty;
 };
 
 // ---------------------------------------------------------------------------
@@ -476,25 +480,59 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         }
         if (A.mix == 0) return;
     }
-    // device ticket: the last workgroup to arrive sums the channels (reduce
-    // mode, instance order as src/partconvMulti.cpp:906-921) and raises done_flag
+    if (A.mix == 1) {
+        // direct mix without per-workgroup flags: device ticket, last one raises done_flag
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            const unsigned t = atomicAdd(A.ticket, 1u);
+            *s_last = (t == (unsigned)(A.njobs - 1));
+        }
+        __syncthreads();
+        if (!*s_last) return;
+        if (tid == 0) {
+            *A.ticket = 0u;
+            if (A.done_flag) {
+                __threadfence_system();
+                __hip_atomic_store(A.done_flag, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        return;
+    }
+    // reduce mix: the last source of each output channel sums that channel in
+    // instance order (src/partconvMulti.cpp:906-921); the last channel to be
+    // finished zeroes the channels without a source and raises done_flag
+    if (TJ.mute) return;
+    const int c = TJ.out_ch;
+    const int cnt = A.src_off[c + 1] - A.src_off[c];
     __syncthreads();
     if (tid == 0) {
         __threadfence();
-        const unsigned t = atomicAdd(A.ticket, 1u);
-        *s_last = (t == (unsigned)(A.njobs - 1));
+        const unsigned t = atomicAdd(A.ticket_ch + c, 1u);
+        *s_last = (t == (unsigned)(cnt - 1));
     }
     __syncthreads();
     if (!*s_last) return;
     __threadfence();
-    if (A.mix == 2) {
-        for (int e = tid; e < A.nOut * N; e += NT) {
-            const int c = e / N, j = e - c * N;
-            float sum = 0.f;
-            for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++)
-                sum += A.y_inst[(size_t)A.src_list[q] * N + j];
-            A.out[(long long)(f - A.fbase) * A.out_stride + c + (long long)j * A.nOut] = sum;
-        }
+    float *oc = A.out + (long long)(f - A.fbase) * A.out_stride + c;
+    for (int j = tid; j < N; j += NT) {
+        float sum = 0.f;
+        for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++) sum += A.y_inst[(size_t)A.src_list[q] * N + j];
+        oc[(long long)j * A.nOut] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        A.ticket_ch[c] = 0u;
+        __threadfence();
+        const unsigned t = atomicAdd(A.ticket, 1u);
+        *s_last = (t == (unsigned)(A.nactive - 1));
+    }
+    __syncthreads();
+    if (!*s_last) return;
+    for (int e = tid; e < A.nempty * N; e += NT) {
+        const int ce = A.empty_ch[e / N], j = e % N;
+        A.out[(long long)(f - A.fbase) * A.out_stride + ce + (long long)j * A.nOut] = 0.f;
     }
     __syncthreads();
     if (tid == 0) {

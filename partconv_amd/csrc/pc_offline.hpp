@@ -384,3 +384,86 @@ k_off_dh_reduce(const cf *__restrict__ part, cf *__restrict__ D, int C, int N, i
     }
     D[i] = acc;
 }
+
+// ---------------------------------------------------------------------------
+// MIMO (throughput form): for every bin k, with M output blocks at once,
+//   Y_k[o][m] = sum_p sum_i H_k,p[o][i] X_k[i][m - p]
+// is a complex (32 x 32P) x (32P x M) product, run on the fp32 matrix cores
+// (v_mfma_f32_32x32x2f32) as two real 32x32 tiles per wave:
+//   Yr = sum_i Hr Xr - Hi Xi,  Yi = sum_i Hr Xi + Hi Xr,
+// with K ordered (i, re/im): lane l supplies A[o = l%32][k = l/32] and
+// B[k = l/32][m = l%32]; accumulator register v of lane l holds
+// C[8(v/4) + 4(l/32) + v%4][l%32].  Packed bin 0 holds the real DC and
+// Nyquist bins, which multiply separately (Yr = sum Hr Xr, Yi = sum Hi Xi).
+// Layouts: Ht[k][p][i][o] (o contiguous, 32 wide), Xt[k][i][m] (m contiguous),
+// Y[o][m][k] (bins contiguous, the inverse transforms' input).
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(64)
+k_mimo_mfma(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__restrict__ Y, int N, int M, int P,
+            int n_in, int n_out)
+{
+    const int k = blockIdx.x;
+    const int m0 = blockIdx.y * 32;
+    const int l = threadIdx.x;
+    const int lo = l & 31, hi = l >> 5;
+    f32x16 accr, acci;
+    for (int v = 0; v < 16; v++) {
+        accr[v] = 0.f;
+        acci[v] = 0.f;
+    }
+    const bool bin0 = (k == 0);
+    const int m = m0 + lo;
+    for (int p = 0; p < P; p++) {
+        const cf *Hp = Ht + ((size_t)k * P + p) * n_in * 32;
+        const int mm = m - p;
+        const bool mv = (mm >= 0 && mm < M);
+        const cf *Xk = Xt + (size_t)k * n_in * M;
+        for (int j = 0; j < n_in; j++) {
+            const cf h = Hp[j * 32 + lo];
+            const cf x = mv ? Xk[(size_t)j * M + mm] : make_float2(0.f, 0.f);
+            float ar, ai;
+            if (!bin0) {
+                ar = hi ? -h.y : h.x;
+                ai = hi ? h.x : h.y;
+            } else {
+                ar = hi ? 0.f : h.x;
+                ai = hi ? h.y : 0.f;
+            }
+            const float b = hi ? x.y : x.x;
+            accr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, b, accr, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, b, acci, 0, 0, 0);
+        }
+    }
+    if (m >= M) return;
+    for (int v = 0; v < 16; v++) {
+        const int o = 8 * (v >> 2) + 4 * hi + (v & 3);
+        if (o < n_out) Y[((size_t)o * M + m) * N + k] = make_float2(accr[v], acci[v]);
+    }
+}
+
+// [R][M][N] -> [N][R][M] (spectra of R signals, M blocks, N bins)
+__global__ void k_spec_transpose(const cf *__restrict__ src, cf *__restrict__ dst, int R, int M, int N)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)R * M * N) return;
+    const int k = (int)(i % N);
+    const long long rm = i / N;
+    const int m = (int)(rm % M), r = (int)(rm / M);
+    dst[((size_t)k * R + r) * M + m] = src[i];
+}
+
+// H [(o*n_in + i)][p][k] -> Ht[k][p][i][o32] (o padded to 32 with zeros)
+__global__ void k_h_transpose(const cf *__restrict__ src, cf *__restrict__ dst, int n_in, int n_out, int P, int N)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)N * P * n_in * 32) return;
+    const int o = (int)(i % 32);
+    long long r = i / 32;
+    const int ii = (int)(r % n_in);
+    r /= n_in;
+    const int p = (int)(r % P);
+    const int k = (int)(r / P);
+    dst[i] = (o < n_out) ? src[(((size_t)o * n_in + ii) * P + p) * N + k] : make_float2(0.f, 0.f);
+}

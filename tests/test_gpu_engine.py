@@ -282,3 +282,37 @@ def test_uneven_batched_instances_greedy_partitions():
     say(f"worst channel rel err {worst:.2e}; persistent ticks {m.stats().persistent_ticks}")
     assert worst < TOL
     m.cleanup()
+
+
+def test_mimo_as_pair_instances():
+    """MIMO in the reference's own terms: one instance per (input i -> output o)
+    pair, outputs summed (src/partconvMulti.cpp:911-921).  8x8 at reduced size;
+    every output against the fp64 sum of per-pair convolutions; launch and
+    persistent modes bitwise equal."""
+    say("case MIMO pairs")
+    import torch
+    from partconv_amd.workloads import cfg4_irs
+    n_in = n_out = 8
+    irs = cfg4_irs(seed=4, n_in=n_in, n_out=n_out, taps=9000)
+    insts, hs = [], []
+    for o in range(n_out):
+        for i in range(n_in):
+            insts.append(dict(N=[256, 4096], P=[30, 0], in_ch=i, out_ch=o))
+            hs.append(irs[o][i])
+    nfr = 2 * 16 + 10
+    rng = np.random.default_rng(44)
+    x = rng.standard_normal((nfr * 256, n_in)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    outs = []
+    for persistent in (False, True):
+        m = make_multi(insts, hs, 256, n_in, n_out)
+        yd = torch.zeros((nfr * 256, n_out), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        assert m.run_device(yd.data_ptr(), xd.data_ptr(), nfr, True, persistent=persistent) == 0, last_error()
+        outs.append(yd.cpu().numpy())
+        m.cleanup()
+    assert np.array_equal(outs[0], outs[1])
+    y = outs[0]
+    for o in range(n_out):
+        r = sum(O.direct_conv_f64(x[:, i], irs[o][i]) for i in range(n_in))
+        assert rel(y[:, o], r) < TOL, o
