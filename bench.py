@@ -151,6 +151,159 @@ def bench_uneven(args):
         "max_rel_err_vs_fp64": err}), flush=True)
 
 
+def bench_mimo(args):
+    """Configuration 4: MIMO 32x32, 65536-tap IRs (RT60 1 s), partition
+    [256x30, 4096x15], buffer 256 at 48 kHz (5.333 ms period).
+    Real time: 1024 pair instances (the reference's own formulation) on the
+    batched engine with the persistent tick kernel.  Throughput: the same
+    filters over 10 s of input at once on the fp32 matrix cores."""
+    import torch
+    from partconv_amd import PartConvMulti, last_error
+    from partconv_amd.mimo import MimoConv
+    from partconv_amd.workloads import cfg4_irs
+    n, n0, taps = 32, 256, 65536
+    irs = cfg4_irs(seed=4, n_in=n, n_out=n, taps=taps)
+    insts = [dict(N=[256, 4096], P=[30, 0], ir=irs[o][i], in_ch=i, out_ch=o) for o in range(n) for i in range(n)]
+    m = PartConvMulti(0)
+    if m.setup_mem(FS, n0, n, n, insts) != 0:
+        raise RuntimeError(last_error())
+    tps = 16
+    total = (args.warmup + args.steps) * tps
+    g = torch.Generator(device="cuda").manual_seed(4)
+    xd = torch.randn((total * n0, n), dtype=torch.float32, device="cuda", generator=g)
+    yd = torch.zeros_like(xd)
+    torch.cuda.synchronize()
+    stride = tps * n0 * n
+    for w in range(args.warmup):
+        if m.run_device(yd.data_ptr(), xd.data_ptr() + 4 * w * stride, tps, True, persistent=True) != 0:
+            raise RuntimeError(last_error())
+    m.clear_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s_ in range(args.steps):
+        off = (args.warmup + s_) * stride
+        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, tps, True, persistent=True) != 0:
+            raise RuntimeError(last_error())
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    st = m.stats()
+    m.cleanup()
+    # throughput form on the matrix cores: same filters, same input
+    h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n)]).astype(np.float32)).cuda()
+    mx = MimoConv([256, 4096], [30, 0], taps, n, n)
+    mx.set_filters(h)
+    xt = xd.t().contiguous()
+    ymx = mx(xt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 3
+    for _ in range(reps):
+        ymx = mx(xt)
+    e1.record()
+    torch.cuda.synchronize()
+    tput_s = e0.elapsed_time(e1) / 1e3 / reps
+    T = xt.shape[1]
+    flops = 0.0
+    for Nl, Pl in ((256, 30), (4096, 15)):
+        M = -(-T // Nl)
+        flops += Nl * Pl * M * 8.0 * n * n
+    # fp64 check of one output over the run: real-time and throughput forms
+    x = xd.cpu().numpy().astype(np.float64)
+    o = 7
+    from numpy.fft import irfft, rfft
+    L = 1 << int(np.ceil(np.log2(x.shape[0] + taps)))
+    r = np.zeros(x.shape[0])
+    for i in range(n):
+        r += irfft(rfft(x[:, i], L) * rfft(irs[o][i].astype(np.float64), L), L)[:x.shape[0]]
+    y = yd.cpu().numpy()
+    lo = args.warmup * tps * n0
+    err_rt = float(np.max(np.abs(y[lo:, o] - r[lo:])) / np.sqrt(np.mean(r[lo:] ** 2)))
+    err_mx = float(np.max(np.abs(ymx[o].cpu().numpy() - r)) / np.sqrt(np.mean(r ** 2)))
+    print(json.dumps({
+        "metric": "channel-samples/s at p99 block latency < block period (MIMO 32x32)",
+        "value": n * args.steps * tps * n0 / secs, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded white noise, seeded decaying-noise IRs)",
+        "config": {"workload": "MIMO 32x32, 65536-tap IRs, [256x30, 4096x15], buffer 256 at 48 kHz; real time as "
+                               "1024 pair instances (batched, persistent tick kernel)"},
+        "latency_us": {"p50": st.p50_us, "p99": st.p99_us, "max": st.max_us, "period": st.period_us,
+                       "misses": int(st.deadline_misses), "ticks": int(st.ticks)},
+        "p99_below_period": bool(st.p99_us < st.period_us),
+        "max_rel_err_vs_fp64": err_rt,
+        "throughput_form": {"output_channel_samples_per_s": n * T / tput_s, "seconds_per_call": tput_s,
+                            "samples": T, "matrix_core_tflops_effective": flops / tput_s / 1e12,
+                            "mfma_peak_tflops_fp32": 157.3, "max_rel_err_vs_fp64": err_mx}}), flush=True)
+
+
+def bench_diff(args):
+    """Configuration 5: 64 clips x 131072 samples, learnable 65536-tap IR
+    (RT60 0.8 s), partition [256x14, 2048x14, 16384x3], MSE loss against targets
+    from a hidden IR (RT60 1.5 s) + N(0, (1e-3)^2); forward + hand-written
+    adjoint per step.  Also the 2^20-tap IR in recompute-spectra mode."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    from partconv_amd.workloads import decaying_noise_ir
+    C, T, taps = 64, 131072, 65536
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn((C, T), dtype=torch.float32, device="cuda", generator=g)
+    h0 = torch.from_numpy(decaying_noise_ir(taps, 0.8, FS, 50)).cuda().requires_grad_(True)
+    hs = torch.from_numpy(decaying_noise_ir(taps, 1.5, FS, 51)).cuda()
+    conv = PartitionedConv([256, 2048, 16384], [14, 14, 0], taps)
+    with torch.no_grad():
+        tgt = conv(x, hs) + 1e-3 * torch.randn((C, T), dtype=torch.float32, device="cuda", generator=g)
+
+    def step():
+        h0.grad = None
+        loss = torch.mean((conv(x, h0) - tgt) ** 2)
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    # long IR, spectra recomputed in backward
+    taps2 = 1 << 20
+    conv2 = PartitionedConv([256, 2048, 16384], [14, 14, 0], taps2, budget_bytes=0)
+    h2 = torch.from_numpy(decaying_noise_ir(taps2, 3.0, FS, 52)).cuda().requires_grad_(True)
+    for _ in range(2):
+        h2.grad = None
+        torch.mean(conv2(x, h2) ** 2).backward()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        h2.grad = None
+        torch.mean(conv2(x, h2) ** 2).backward()
+    torch.cuda.synchronize()
+    secs2 = time.perf_counter() - t1
+    # fp64 check of one clip's forward
+    from numpy.fft import irfft, rfft
+    xc = x[3].cpu().numpy().astype(np.float64)
+    hc = h0.detach().cpu().numpy().astype(np.float64)
+    L = 1 << int(np.ceil(np.log2(T + taps)))
+    r = irfft(rfft(xc, L) * rfft(hc, L), L)[:T]
+    with torch.no_grad():
+        yc = conv(x, h0)[3].cpu().numpy()
+    err = float(np.max(np.abs(yc - r)) / np.sqrt(np.mean(r ** 2)))
+    print(json.dumps({
+        "metric": "clip-samples/s, forward + adjoint (differentiable partitioned convolution)",
+        "value": C * T * args.steps / secs, "unit": "clip-samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded white noise, seeded decaying-noise IRs)",
+        "config": {"workload": "64 clips x 131072, 65536-tap learnable IR, [256x14, 2048x14, 16384x3], MSE"},
+        "max_rel_err_vs_fp64": err,
+        "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
+                              "ms_per_step": 1e3 * secs2 / args.steps,
+                              "recomputed_levels": int(conv2.recomputed_levels)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,8 +313,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["cfg2", "cfg3"], default="cfg2",
-                    help="cfg2: 16-channel drop-in (default, headline); cfg3: uneven batched instances")
+    ap.add_argument("--workload", choices=["cfg2", "cfg3", "cfg4", "cfg5"], default="cfg2",
+                    help="cfg2: 16-channel drop-in (default, headline); cfg3: uneven batched instances; "
+                         "cfg4: MIMO 32x32; cfg5: differentiable forward + adjoint")
     ap.add_argument("--instances", type=int, default=512, help="cfg3 instance count")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 64 MiB device copy first (HBM-counter calibration under a profiler)")
@@ -173,6 +327,12 @@ def main():
 
     if args.workload == "cfg3":
         bench_uneven(args)
+        return
+    if args.workload == "cfg4":
+        bench_mimo(args)
+        return
+    if args.workload == "cfg5":
+        bench_diff(args)
         return
     import torch
     from partconv_amd.workloads import cfg2_irs
