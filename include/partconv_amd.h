@@ -179,6 +179,14 @@ int pcm_probe(pc_multi *h, int global_level);
  * instance 8(N0+1)(2P0+1) + 12*N0 for the tick kernel. */
 int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, double *bytes_per_launch);
 
+/* Persistent-kernel trace (instrumentation, no reference counterpart): for
+ * the next persistent run, workgroup 0 records per tick the GPU clock
+ * (s_memrealtime, 100 MHz) when it sees the release and when its output is
+ * published, for up to nticks ticks (0 disables).  pcm_trace_read copies
+ * 2*n values (n returned). */
+int pcm_trace(pc_multi *h, long long nticks);
+long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
+
 /* ----------------------------------------------------------------------------
  * PartConvFilter (single instance, caller-driven levels)
  * ------------------------------------------------------------------------- */

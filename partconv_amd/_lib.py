@@ -64,6 +64,8 @@ SIGNATURES = {
     "pcm_get_latencies": (ctypes.c_longlong, [_vp, _dp, ctypes.c_longlong]),
     "pcm_instance_partition": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int]),
     "pcm_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_trace": (ctypes.c_int, [_vp, ctypes.c_longlong]),
+    "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),
     "pcf_setup": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _fp, ctypes.c_int, ctypes.c_int,

@@ -221,6 +221,8 @@ struct Engine {
     uint64_t frame = 0;
     // kernel probe: HIP events around one kernel of one global level
     int probe_g = -1;
+    unsigned long long *d_trace = nullptr;   // persistent-kernel timestamps (optional)
+    long long trace_cap = 0;
     std::vector<hipEvent_t> pa, pb;
     std::vector<bool> pused;
     int pnext = 0;
@@ -268,6 +270,9 @@ static void engine_free(Engine &E)
     E.h_flag = nullptr;
     for (hipEvent_t e : E.ev_lvl) hipEventDestroy(e);
     E.ev_lvl.clear();
+    hipFree(E.d_trace);
+    E.d_trace = nullptr;
+    E.trace_cap = 0;
     for (hipEvent_t e : E.pa) hipEventDestroy(e);
     for (hipEvent_t e : E.pb) hipEventDestroy(e);
     E.pa.clear();
@@ -1035,6 +1040,8 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     // relay through device memory (d_ticket[2..3]) when many workgroups would poll the bus
     P.dev_go = (a.njobs > 32) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
     if (P.dev_go) HIPCHK(hipMemsetAsync(P.dev_go, 0, sizeof(unsigned long long), E.s_main));
+    P.trace = E.d_trace;
+    P.trace_cap = E.trace_cap;
     P.f0 = f0;
     P.nticks = nframes;
     P.go = E.d_flag_map + 1;
@@ -1426,6 +1433,34 @@ PC_API int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, doub
     if (launches) *launches = E.probe_n;
     if (bytes_per_launch) *bytes_per_launch = E.probe_bytes;
     return PC_OK;
+}
+
+PC_API int pcm_trace(pc_multi *h, long long nticks)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    Engine &E = h->E;
+    HIPCHK(hipDeviceSynchronize());
+    hipFree(E.d_trace);
+    E.d_trace = nullptr;
+    E.trace_cap = 0;
+    if (nticks > 0) {
+        HIPCHK(hipMalloc(&E.d_trace, (size_t)nticks * 2 * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(E.d_trace, 0, (size_t)nticks * 2 * sizeof(unsigned long long)));
+        E.trace_cap = nticks;
+    }
+    return PC_OK;
+}
+
+PC_API long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap)
+{
+    if (!h || !out) return -1;
+    Engine &E = h->E;
+    const long long n = std::min(cap, E.trace_cap);
+    if (n <= 0) return 0;
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    if (hipMemcpy(out, E.d_trace, (size_t)n * 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    return n;
 }
 
 // ----------------------------------------------------------------------------

@@ -545,6 +545,177 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     }
 }
 
+// ---------------------------------------------------------------------------
+// buffer_size 64: level 0 of a tick as one wavefront with the 64-point complex
+// transform held in registers (one value per lane, radix-2 stages exchanged
+// with lane shuffles; no LDS passes or barriers).  The forward transform is
+// decimation in frequency (natural input, bit-reversed output: lane l holds
+// bin k = brev6(l)); the inverse is decimation in time (bit-reversed input,
+// natural output).  Every lane keeps its stage twiddles, its split twiddle and
+// its bin's H_0 / S in registers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int brev6(int x) { return (int)(__builtin_bitreverse32((unsigned)x) >> 26); }
+__device__ __forceinline__ cf shfl_xor_cf(cf v, int m) { return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m)); }
+__device__ __forceinline__ cf shfl_cf(cf v, int src) { return make_float2(__shfl(v.x, src), __shfl(v.y, src)); }
+
+struct W64 {
+    cf tw[6];       // stage d = 32,16,..,1: W_{2d}^{lane & (d-1)}
+    cf ws;          // split twiddle exp(-i pi k / 64) of this lane's bin k
+    int k;          // bin held by this lane after the forward transform
+    int partner;    // lane holding bin (64 - k) mod 64
+};
+
+__device__ __forceinline__ void w64_setup(W64 &w, const cf *__restrict__ tw, int lane)
+{
+    int s = 0;
+    for (int d = 32; d >= 1; d >>= 1, s++) w.tw[s] = tw[2 * d + (lane & (d - 1))];
+    w.k = brev6(lane);
+    w.partner = brev6((64 - w.k) & 63);
+    w.ws = tw[128 + w.k];
+}
+
+__device__ __forceinline__ cf w64_fwd(cf v, const W64 &w, int lane)
+{
+    int s = 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1, s++) {
+        const cf o = shfl_xor_cf(v, d);
+        v = (lane & d) ? cmul(csub(o, v), w.tw[s]) : cadd(v, o);
+    }
+    return v;
+}
+
+__device__ __forceinline__ cf w64_inv(cf v, const W64 &w, int lane)
+{
+    int s = 5;
+#pragma unroll
+    for (int d = 1; d <= 32; d <<= 1, s--) {
+        const cf t = (lane & d) ? cmulc(v, w.tw[s]) : v;
+        const cf o = shfl_xor_cf(t, d);
+        v = (lane & d) ? csub(o, t) : cadd(t, o);
+    }
+    return v;
+}
+
+struct TickState64 {
+    W64 w;
+    cf h0, sp;       // H_0 and S of bin w.k
+    float2 pv;       // previous frame samples 2*lane, 2*lane+1 (lanes < 32)
+};
+
+__device__ __forceinline__ void tick64_load_state(const LevelJob &J, uint64_t f, int lane, const cf *__restrict__ tw,
+                                                  TickState64 &st)
+{
+    w64_setup(st.w, tw, lane);
+    st.h0 = J.H[st.w.k];
+    st.sp = J.S[st.w.k];
+    const float *prev = J.ring + (((long long)f * 64 - 64) & (long long)J.ring_mask);
+    st.pv = (lane < 32) ? make_float2(prev[2 * lane], prev[2 * lane + 1]) : make_float2(0.f, 0.f);
+}
+
+// Phase A for buffer_size 64; leaves the instance's output sample `lane` in
+// *acc (through LDS) and X_f of bin st.w.k in *xr.
+__device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, float *obuf, int lane,
+                                              TickState64 &st, float *acc, cf *xr)
+{
+    const LevelJob &J = TJ.l0;
+    const long long mask = J.ring_mask;
+    const long long R = mask + 1;
+    const long long t0 = (long long)f * 64;
+    const float *inf = A.in + (long long)(f - A.fbase) * A.in_stride;
+    const uint64_t fr = A.relaxed ? f : f + 1;
+    const int lo = lane & 31;
+    // loads: lanes 32..63 fetch the new frame's pairs, lanes 0..31 the level blocks due
+    float2 xv = make_float2(0.f, 0.f), lev = make_float2(0.f, 0.f), y0p = make_float2(0.f, 0.f);
+    if (lane >= 32) {
+        xv.x = inf[TJ.in_ch + (long long)(2 * lo) * A.nIn];
+        xv.y = inf[TJ.in_ch + (long long)(2 * lo + 1) * A.nIn];
+    } else {
+        if (A.relaxed && f > 0) y0p = reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64)[lo];
+        for (int L = 0; L < TJ.nlev; L++) {
+            const int ks = TJ.kshift[L];
+            const uint64_t qb = fr >> ks;
+            if (qb == 0) continue;
+            const uint64_t mb = qb - 1;
+            const uint64_t sub = fr & ((1ull << ks) - 1);
+            const float2 v = reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)(mb & 1) * TJ.NL[L] + sub * 64)[lo];
+            lev.x += v.x;
+            lev.y += v.y;
+        }
+    }
+    // scale (src/partconvMulti.cpp:839-847), ring write; window pair of this lane
+    cf z;
+    if (lane >= 32) {
+        const float a = xv.x * TJ.scale, b = xv.y * TJ.scale;
+        float *ring = const_cast<float *>(J.ring);
+        const long long p = (t0 + 2 * lo) & mask;
+        ring[p] = a;
+        ring[p + 1] = b;
+        ring[p + R] = a;
+        ring[p + R + 1] = b;
+        z = make_float2(a, b);
+    } else {
+        z = make_float2(st.pv.x, st.pv.y);
+    }
+    // the new frame is the next tick's previous frame
+    {
+        const cf nz = shfl_cf(z, lo + 32);
+        if (lane < 32) st.pv = make_float2(nz.x, nz.y);
+    }
+    cf Z = w64_fwd(z, st.w, lane);
+    const cf Zp = shfl_cf(Z, st.w.partner);
+    cf X;
+    if (st.w.k == 0) {
+        X = make_float2(Z.x + Z.y, Z.x - Z.y);
+    } else {
+        cf xnk;
+        rsplit_fwd(Z, Zp, st.w.ws, &X, &xnk);
+    }
+    *xr = X;
+    cf Y = pmul(X, st.h0, st.w.k == 0);
+    Y.x += st.sp.x;
+    Y.y += st.sp.y;
+    const cf Yp = shfl_cf(Y, st.w.partner);
+    cf C;
+    if (st.w.k == 0) {
+        C = make_float2(Y.x + Y.y, Y.x - Y.y);
+    } else {
+        cf cnk;
+        rsplit_inv(Y, Yp, st.w.ws, &C, &cnk);
+    }
+    const cf zo = w64_inv(C, st.w, lane);   // lane n: z'[n] = (y[2n], y[2n+1])
+    if (lane < 32) {
+        float2 o;
+        if (A.relaxed) {
+            reinterpret_cast<float2 *>(J.ybuf + (size_t)(f & 1) * 64)[lo] = make_float2(zo.x, zo.y);
+            o = make_float2(y0p.x + lev.x, y0p.y + lev.y);
+        } else {
+            o = make_float2(zo.x + lev.x, zo.y + lev.y);
+        }
+        reinterpret_cast<float2 *>(obuf)[lo] = o;
+    }
+    __syncthreads();
+    *acc = obuf[lane];
+}
+
+// Phase B for buffer_size 64: FDL row f % P <- X_f; S for tick f+1 (bin st.w.k).
+__device__ __forceinline__ void tick64_phaseB(const TickJob &TJ, uint64_t f, cf xr, TickState64 &st)
+{
+    const LevelJob &J = TJ.l0;
+    const int P = J.P;
+    const int k = st.w.k;
+    const int slot = (int)(f % (uint64_t)P);
+    J.fdl[(size_t)slot * 64 + k] = xr;
+    cf s = (P > 1) ? pmul(xr, J.H[64 + k], k == 0) : make_float2(0.f, 0.f);
+    int r = slot;
+    for (int p = 2; p < P; p++) {
+        r = (r == 0) ? P - 1 : r - 1;
+        s = pmac(s, J.fdl[(size_t)r * 64 + k], J.H[(size_t)p * 64 + k], k == 0);
+    }
+    J.S[k] = s;
+    st.sp = s;
+}
+
 template <int LOGN>
 __global__ void __launch_bounds__(TickCfg<LOGN>::NT)
 k_tick(TickArgs A, const cf *__restrict__ tw)
@@ -555,13 +726,22 @@ k_tick(TickArgs A, const cf *__restrict__ tw)
     __shared__ unsigned s_last;
     const int tid = threadIdx.x;
     const TickJob &TJ = A.jobs[blockIdx.x];
-    TickState<LOGN> st;
-    tick_load_state<LOGN>(TJ.l0, A.f, tid, st);
     float acc[IPT];
-    cf xr[IPT];
-    tick_phaseA<LOGN>(TJ, A, A.f, bufA, bufB, tw, tid, st, acc, xr);
-    tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
-    tick_phaseB<LOGN>(TJ, A.f, tid, xr, st);
+    if constexpr (LOGN == 6) {
+        TickState64 st;
+        tick64_load_state(TJ.l0, A.f, tid, tw, st);
+        cf xr;
+        tick64_phaseA(TJ, A, A.f, reinterpret_cast<float *>(bufA), tid, st, &acc[0], &xr);
+        tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
+        tick64_phaseB(TJ, A.f, xr, st);
+    } else {
+        TickState<LOGN> st;
+        tick_load_state<LOGN>(TJ.l0, A.f, tid, st);
+        cf xr[IPT];
+        tick_phaseA<LOGN>(TJ, A, A.f, bufA, bufB, tw, tid, st, acc, xr);
+        tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
+        tick_phaseB<LOGN>(TJ, A.f, tid, xr, st);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -584,6 +764,8 @@ struct PersistArgs {
     unsigned long long *dev_go;    // relay: workgroup 0 copies go here for the others (or null)
     unsigned long long spin_limit;
     unsigned long long acq_mask;   // device-scope acquire at ticks whose (f [+1]) & mask == 0
+    unsigned long long *trace;     // optional: per tick {release seen, output published} (s_memrealtime)
+    long long trace_cap;
 };
 
 template <int LOGN>
@@ -604,7 +786,9 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     }
     __syncthreads();
     TickState<LOGN> st;
-    tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
+    TickState64 st64;
+    if constexpr (LOGN == 6) tick64_load_state(sTJ.l0, P.f0, tid, tw, st64);
+    else tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
         if (tid == 0) {
@@ -635,11 +819,22 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         }
         __syncthreads();
         if (!s_go) return;
+        const bool tr = P.trace && blockIdx.x == 0 && tid == 0 && t < P.trace_cap;
+        if (tr) P.trace[2 * t] = __builtin_amdgcn_s_memrealtime();
         float acc[IPT];
-        cf xr[IPT];
-        tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr);
-        tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
-        tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
+        if constexpr (LOGN == 6) {
+            cf xr;
+            tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr);
+            tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
+            if (tr) P.trace[2 * t + 1] = __builtin_amdgcn_s_memrealtime();
+            tick64_phaseB(sTJ, f, xr, st64);
+        } else {
+            cf xr[IPT];
+            tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr);
+            tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
+            if (tr) P.trace[2 * t + 1] = __builtin_amdgcn_s_memrealtime();
+            tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
+        }
         __syncthreads();
     }
 }

@@ -340,6 +340,16 @@ class PartConvMulti:
             raise RuntimeError(last_error())
         return a.value, n.value, b.value
 
+    def trace(self, nticks):
+        """Record GPU timestamps of the next persistent run (see pcm_trace)."""
+        return _report(lib().pcm_trace(self._h, int(nticks)), "trace")
+
+    def trace_read(self, cap=1 << 20):
+        """[(release_seen, published)] GPU clock (100 MHz ticks) per tick."""
+        buf = (ctypes.c_ulonglong * (2 * cap))()
+        n = lib().pcm_trace_read(self._h, buf, cap)
+        return np.frombuffer(buf, dtype=np.uint64)[:2 * max(n, 0)].reshape(-1, 2)
+
     def instance_partition(self, i):
         N = (ctypes.c_int * 16)()
         P = (ctypes.c_int * 16)()
