@@ -180,12 +180,24 @@ int pcm_probe(pc_multi *h, int global_level);
 int pcm_probe_read(pc_multi *h, double *avg_us, long long *launches, double *bytes_per_launch);
 
 /* Persistent-kernel trace (instrumentation, no reference counterpart): for
- * the next persistent run, workgroup 0 records per tick the GPU clock
- * (s_memrealtime, 100 MHz) when it sees the release and when its output is
- * published, for up to nticks ticks (0 disables).  pcm_trace_read copies
- * 2*n values (n returned). */
+ * the next persistent run, up to nticks ticks (0 disables), 8 GPU clock
+ * values (s_memrealtime, 100 MHz) per tick: release seen / output published
+ * by workgroups 0, njobs/2 and njobs-1; completion raised (ticket paths);
+ * workgroup 0 ready for the next release.  pcm_trace_read copies 8*n values
+ * (n returned). */
 int pcm_trace(pc_multi *h, long long nticks);
 long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
+
+/* Level workers of persistent runs (tuning switch, no reference counterpart):
+ * on (default), the blocks of levels with period <= 8 ticks and N <= 2048 run
+ * inside the persistent kernel -- in worker workgroups of their own (one per
+ * instance and level) when those fit in half of the device, else in each
+ * instance's tick workgroup after its tick ("self mode"); off, every level
+ * block is a host launch on the level's stream.  Output is identical in all
+ * cases.  pcm_workers_info returns the number of worker workgroups and sets
+ * *self_mode (may be null). */
+int pcm_set_persistent_workers(pc_multi *h, int on);
+int pcm_workers_info(pc_multi *h, int *self_mode);
 
 /* ----------------------------------------------------------------------------
  * PartConvFilter (single instance, caller-driven levels)

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pcm_instance_partition": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int]),
     "pcm_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_trace": (ctypes.c_int, [_vp, ctypes.c_longlong]),
+    "pcm_set_persistent_workers": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_workers_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),

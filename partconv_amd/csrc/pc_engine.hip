@@ -228,6 +228,22 @@ struct Engine {
     int pnext = 0;
     double probe_ms = 0, probe_bytes = 0;
     long long probe_n = 0;
+    // level workers of persistent runs (short-period small levels, batched engine)
+    std::vector<char> g_worker;       // per global level: served by workers in persistent runs
+    struct WRef {
+        int g, batch, j;
+    };
+    std::vector<WRef> wref;           // per worker slot
+    std::vector<WorkJob> wjobs_h;
+    WorkJob *d_wjobs = nullptr;
+    int nwjobs = 0, lds_w = 0;        // worker-level jobs, largest worker block size
+    int nworkers = -1;                // worker workgroups (-1: not planned yet)
+    bool self_levels = false;         // no room for workers: tick workgroups run those levels
+    int2 *d_wrange = nullptr;
+    unsigned long long *d_lvl_done = nullptr, *d_dev_tick = nullptr;
+    std::vector<unsigned long long> h_lvl;   // staging for lvl_done
+    bool workers_active = false;      // a persistent run with workers owns those levels
+    int persist_workers = 1;          // 0: persistent runs launch every level from the host
     // stats
     std::vector<double> lat;
     size_t lat_pos = 0;
@@ -242,6 +258,18 @@ static void engine_free(Engine &E)
 {
     if (E.s_main) hipStreamSynchronize(E.s_main);
     for (hipStream_t s : E.s_lvl) hipStreamSynchronize(s);
+    hipFree(E.d_wjobs);
+    hipFree(E.d_lvl_done);
+    hipFree(E.d_dev_tick);
+    E.d_wjobs = nullptr;
+    E.d_lvl_done = E.d_dev_tick = nullptr;
+    hipFree(E.d_wrange);
+    E.d_wrange = nullptr;
+    E.nworkers = -1;
+    E.nwjobs = 0;
+    E.wref.clear();
+    E.wjobs_h.clear();
+    E.g_worker.clear();
     for (Group &G : E.groups) {
         hipFree(G.d_tick);
         hipFree(G.d_l0);
@@ -581,9 +609,50 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         J.ring_mask = I.R - 1;
         return J;
     };
+    // levels served by in-kernel workers during persistent runs: period <= 8
+    // ticks and a single-workgroup block (N <= 2048)
+    E.g_worker.assign(G, 0);
+    for (int g = 1; g < G; g++) E.g_worker[g] = (E.gsize[g] <= 8 && (N0 << E.gshift[g]) < PC_BIG_MIN) ? 1 : 0;
     for (auto &mem : members) {
         Group Gr;
         Gr.members = mem;
+        const bool batched = (E.mode == PC_BATCHED);
+        std::vector<std::vector<int>> wslot(E.inst.size(), std::vector<int>(G, -1));
+        for (int g = 1; g < G; g++) {
+            std::vector<LevelJob> js;
+            std::vector<int> pos;
+            for (size_t q = 0; q < mem.size(); q++) {
+                Inst &I = E.inst[mem[q]];
+                if (I.gmap[g] >= 0) {
+                    js.push_back(mk_level_job(I, I.lv[I.gmap[g]]));
+                    pos.push_back((int)q);
+                }
+            }
+            if (js.empty()) continue;
+            Batch b;
+            b.g = g;
+            b.N = N0 << E.gshift[g];
+            b.big = b.N >= PC_BIG_MIN;
+            if (b.big) b.sh = big_shape(b.N);
+            b.njobs = (int)js.size();
+            HIPCHK(hipMalloc(&b.d_jobs, js.size() * sizeof(LevelJob)));
+            HIPCHK(hipMemcpy(b.d_jobs, js.data(), js.size() * sizeof(LevelJob), hipMemcpyHostToDevice));
+            if (batched && E.g_worker[g]) {
+                for (size_t j = 0; j < js.size(); j++) {
+                    WorkJob w{};
+                    w.J = js[j];
+                    w.inst = pos[j];
+                    w.kshift = E.gshift[g];
+                    w.logN = ilog2(b.N);
+                    w.slot = (int)E.wjobs_h.size();
+                    wslot[mem[pos[j]]][g] = w.slot;
+                    E.wjobs_h.push_back(w);
+                    E.wref.push_back({g, (int)Gr.batches.size(), (int)j});
+                    E.lds_w = std::max(E.lds_w, b.N);
+                }
+            }
+            Gr.batches.push_back(b);
+        }
         std::vector<TickJob> tj;
         std::vector<LevelJob> l0;
         for (int i : mem) {
@@ -595,10 +664,14 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
             t.mute = I.d.mute;
             t.scale = I.d.scale;
             t.nlev = (int)I.lv.size() - 1;
-            for (int L = 1; L < (int)I.lv.size(); L++) {
+            for (int L = 0; L < PC_MAX_LEVELS; L++) t.wslot[L] = -1;
+            for (int g = 1; g < G; g++) {
+                const int L = I.gmap[g];
+                if (L < 1) continue;
                 t.kshift[L - 1] = I.lv[L].kshift;
                 t.NL[L - 1] = I.lv[L].N;
                 t.ybuf[L - 1] = reinterpret_cast<const float *>(E.arena + I.lv[L].off_ybuf);
+                t.wslot[L - 1] = wslot[i][g];
             }
             tj.push_back(t);
             l0.push_back(t.l0);
@@ -607,24 +680,16 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         HIPCHK(hipMemcpy(Gr.d_tick, tj.data(), tj.size() * sizeof(TickJob), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&Gr.d_l0, l0.size() * sizeof(LevelJob)));
         HIPCHK(hipMemcpy(Gr.d_l0, l0.data(), l0.size() * sizeof(LevelJob), hipMemcpyHostToDevice));
-        for (int g = 1; g < G; g++) {
-            std::vector<LevelJob> js;
-            for (int i : mem) {
-                Inst &I = E.inst[i];
-                if (I.gmap[g] >= 0) js.push_back(mk_level_job(I, I.lv[I.gmap[g]]));
-            }
-            if (js.empty()) continue;
-            Batch b;
-            b.g = g;
-            b.N = N0 << E.gshift[g];
-            b.big = b.N >= PC_BIG_MIN;
-            if (b.big) b.sh = big_shape(b.N);
-            b.njobs = (int)js.size();
-            HIPCHK(hipMalloc(&b.d_jobs, js.size() * sizeof(LevelJob)));
-            HIPCHK(hipMemcpy(b.d_jobs, js.data(), js.size() * sizeof(LevelJob), hipMemcpyHostToDevice));
-            Gr.batches.push_back(b);
-        }
         E.groups.push_back(Gr);
+    }
+    E.nwjobs = (int)E.wjobs_h.size();
+    E.nworkers = -1;
+    if (E.nwjobs > 0) {
+        HIPCHK(hipMalloc(&E.d_wjobs, E.wjobs_h.size() * sizeof(WorkJob)));
+        HIPCHK(hipMemcpy(E.d_wjobs, E.wjobs_h.data(), E.wjobs_h.size() * sizeof(WorkJob), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&E.d_lvl_done, E.wjobs_h.size() * sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&E.d_dev_tick, E.inst.size() * sizeof(unsigned long long)));
+        E.h_lvl.assign(E.wjobs_h.size(), 0ull);
     }
     // ---- output mix routing
     std::vector<int> src_off(nOut + 1, 0), src_list, empty;
@@ -660,10 +725,17 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     HIPCHK(hipMemset(E.d_ticket, 0, 64));
 
     // ---- streams, events, pinned staging
+    // Level streams get hardware queues of their own (a stream with a CU mask,
+    // here every CU, is never placed on a shared queue): level blocks launched
+    // while the persistent tick kernel occupies s_main must not queue behind it.
     const int nls = std::min(PC_LEVEL_STREAMS, std::max(0, G - 1));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, E.dev));
+    std::vector<uint32_t> cu_mask((prop.multiProcessorCount + 31) / 32, 0u);
+    for (int c = 0; c < prop.multiProcessorCount; c++) cu_mask[c / 32] |= 1u << (c % 32);
     for (int q = 0; q < nls; q++) {
         hipStream_t s;
-        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)cu_mask.size(), cu_mask.data()));
         E.s_lvl.push_back(s);
     }
     E.g_stream.assign(G, 0);
@@ -844,6 +916,7 @@ static int start_levels(Engine &E, uint64_t f, bool use_event)
     for (int g = 1; g < G; g++) {
         const uint64_t k = (uint64_t)E.gsize[g];
         if (f % k != 0) continue;
+        if (E.workers_active && E.g_worker[g]) continue;  // the persistent kernel's workers run it
         hipStream_t s = level_stream(E, g);
         if (use_event) {
             if (!recorded) {
@@ -962,29 +1035,120 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     return PC_OK;
 }
 
+// Grid: one tick workgroup per instance, then one workgroup per level worker.
+// Dynamic LDS: two float2 buffers of max(N0, largest worker block).
 template <int LOGN>
-static hipError_t launch_persist_t(const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks)
+static hipError_t launch_persist_t(const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks, size_t lds)
 {
     PersistArgs pa = p;
     const cf *t = tw;
     void *args[] = {(void *)&pa, (void *)&t};
     return hipLaunchCooperativeKernel((const void *)k_persist<LOGN>, dim3(nblocks), dim3(TickCfg<LOGN>::NT),
-                                      args, 0, s);
+                                      args, (unsigned)lds, s);
 }
 
-static hipError_t launch_persist(int N0, const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks)
+static hipError_t launch_persist(int N0, const PersistArgs &p, const cf *tw, hipStream_t s, int lds_n)
 {
+    const int nblocks = p.A.njobs + p.nworkers;
+    const size_t lds = (size_t)2 * lds_n * sizeof(cf);
     switch (ilog2(N0)) {
-    case 4: return launch_persist_t<4>(p, tw, s, nblocks);
-    case 5: return launch_persist_t<5>(p, tw, s, nblocks);
-    case 6: return launch_persist_t<6>(p, tw, s, nblocks);
-    case 7: return launch_persist_t<7>(p, tw, s, nblocks);
-    case 8: return launch_persist_t<8>(p, tw, s, nblocks);
-    case 9: return launch_persist_t<9>(p, tw, s, nblocks);
-    case 10: return launch_persist_t<10>(p, tw, s, nblocks);
-    case 11: return launch_persist_t<11>(p, tw, s, nblocks);
+    case 4: return launch_persist_t<4>(p, tw, s, nblocks, lds);
+    case 5: return launch_persist_t<5>(p, tw, s, nblocks, lds);
+    case 6: return launch_persist_t<6>(p, tw, s, nblocks, lds);
+    case 7: return launch_persist_t<7>(p, tw, s, nblocks, lds);
+    case 8: return launch_persist_t<8>(p, tw, s, nblocks, lds);
+    case 9: return launch_persist_t<9>(p, tw, s, nblocks, lds);
+    case 10: return launch_persist_t<10>(p, tw, s, nblocks, lds);
+    case 11: return launch_persist_t<11>(p, tw, s, nblocks, lds);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int LOGN>
+static hipError_t persist_occupancy_t(int *nb, size_t lds)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_persist<LOGN>, TickCfg<LOGN>::NT, lds);
+}
+
+static hipError_t persist_occupancy(int N0, int *nb, size_t lds)
+{
+    switch (ilog2(N0)) {
+    case 4: return persist_occupancy_t<4>(nb, lds);
+    case 5: return persist_occupancy_t<5>(nb, lds);
+    case 6: return persist_occupancy_t<6>(nb, lds);
+    case 7: return persist_occupancy_t<7>(nb, lds);
+    case 8: return persist_occupancy_t<8>(nb, lds);
+    case 9: return persist_occupancy_t<9>(nb, lds);
+    case 10: return persist_occupancy_t<10>(nb, lds);
+    case 11: return persist_occupancy_t<11>(nb, lds);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Who runs the worker levels in persistent runs.  Tick and worker workgroups
+// together may take at most half of the workgroups the device holds at once,
+// so the level kernels launched from the host (the long-period levels) still
+// find room beside the persistent kernel.  If one worker per job fits, each
+// job gets its own worker workgroup; otherwise every tick workgroup runs its
+// own instance's worker-level blocks after its tick (self mode), which costs
+// no extra workgroups and needs no handshake.
+static int plan_workers(Engine &E)
+{
+    if (E.nworkers >= 0) return PC_OK;
+    E.nworkers = 0;
+    E.self_levels = false;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, E.dev));
+    int nb = 0;
+    HIPCHK(persist_occupancy(E.N0, &nb, (size_t)2 * std::max(E.N0, E.lds_w) * sizeof(cf)));
+    const long long budget = (long long)prop.multiProcessorCount * nb / 2;
+    const long long ntick = (long long)E.groups[0].members.size();
+    if (ntick + E.nwjobs > budget) {
+        E.self_levels = true;
+    } else {
+        std::vector<int2> ranges;
+        for (int q = 0; q < E.nwjobs; q++) ranges.push_back(make_int2(q, 1));
+        HIPCHK(hipMalloc(&E.d_wrange, ranges.size() * sizeof(int2)));
+        HIPCHK(hipMemcpy(E.d_wrange, ranges.data(), ranges.size() * sizeof(int2), hipMemcpyHostToDevice));
+        E.nworkers = (int)ranges.size();
+    }
+    if (E.verbosity > 0)
+        printf("persistent kernel: %lld tick + %d worker workgroups for %d worker-level jobs%s (%d per CU fit)\n",
+               ntick, E.nworkers, E.nwjobs, E.self_levels ? ", run by the tick workgroups" : "", nb);
+    return PC_OK;
+}
+
+// After a persistent run with level workers: every worker-level block whose
+// start tick precedes tick `fend` (and its delay-line sum for the next block)
+// must be complete.  Blocks a worker did not finish (a stopped or timed-out
+// run) are launched here, per job, on the level's stream.
+static int finish_worker_levels(Engine &E, uint64_t fend)
+{
+    HIPCHK(hipMemcpyAsync(E.h_lvl.data(), E.d_lvl_done, E.h_lvl.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, E.s_main));
+    HIPCHK(hipStreamSynchronize(E.s_main));
+    Group &Gr = E.groups[0];
+    for (int w = 0; w < E.nwjobs; w++) {
+        const Engine::WRef &r = E.wref[w];
+        const Batch &b = Gr.batches[r.batch];
+        const uint64_t k = (uint64_t)E.gsize[r.g];
+        const uint64_t m_end = (fend + k - 1) / k;   // blocks 0 .. m_end-1 are due
+        const unsigned long long d = E.h_lvl[w];
+        uint64_t next = (d + 1) / 2;                 // first block not written
+        if (next >= m_end && !(d & 1)) continue;
+        hipStream_t s = level_stream(E, r.g);
+        const LevelJob *J = b.d_jobs + r.j;
+        if (d & 1) launch_partial(b.N, J, 1, next, s, LaunchExtra());  // block next-1 out, its S missing
+        for (uint64_t m = next; m < m_end; m++) {
+            int rc = launch_small(b.N, J, 1, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
+            if (rc) return rc;
+            launch_partial(b.N, J, 1, m + 1, s, LaunchExtra());
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E.ev_lvl[r.g], s));
+        E.lvl_pending[r.g] = true;
+    }
+    return PC_OK;
 }
 
 static void store_go(Engine &E, unsigned long long v)
@@ -1033,15 +1197,28 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.done_flag = E.d_flag_map;
     a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
     a.fence_sys = 0;
-    // level jobs are launched (from the host) only after ticks f = 0 mod k_min;
-    // those ticks publish the ring with a device-scope release
-    a.fence_mask = (E.gsize.size() > 1) ? (unsigned long long)(E.gsize[1] - 1) : ~0ull;
-    P.acq_mask = a.fence_mask;
+    // Level jobs launched from the host start only at ticks f = 0 mod k_min
+    // (k_min over the host-launched levels); those ticks publish the ring with
+    // a device-scope release, and the ticks that first read such a level's
+    // block acquire at device scope.  Separate level workers get the frames at
+    // ticks f = 0 mod (their k_min).
+    auto set_masks = [&](bool workers, bool separate) {
+        unsigned long long kh = 0, kw = 0;
+        for (int g = 1; g < (int)E.gsize.size(); g++) {
+            const unsigned long long k = (unsigned long long)E.gsize[g];
+            if (workers && E.g_worker[g]) kw = kw ? std::min(kw, k) : k;
+            else kh = kh ? std::min(kh, k) : k;
+        }
+        a.fence_mask = kh ? kh - 1 : ~0ull;
+        P.acq_mask = a.fence_mask;
+        P.pub_mask = (separate && kw) ? kw - 1 : ~0ull;
+    };
+    set_masks(false, false);
     // relay through device memory (d_ticket[2..3]) when many workgroups would poll the bus
     P.dev_go = (a.njobs > 32) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
     if (P.dev_go) HIPCHK(hipMemsetAsync(P.dev_go, 0, sizeof(unsigned long long), E.s_main));
-    P.trace = E.d_trace;
-    P.trace_cap = E.trace_cap;
+    a.trace = E.d_trace;
+    a.trace_cap = E.trace_cap;
     P.f0 = f0;
     P.nticks = nframes;
     P.go = E.d_flag_map + 1;
@@ -1051,7 +1228,41 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     for (int g = 1; g < G; g++)
         if (E.lvl_pending[g]) HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
     store_go(E, (unsigned long long)f0);
-    hipError_t le = launch_persist(E.N0, P, E.tw, E.s_main, a.njobs);
+    hipError_t le = hipErrorInvalidValue;
+    if (E.nwjobs > 0 && E.persist_workers) {
+        int rc = plan_workers(E);
+        if (rc) return rc;
+    }
+    if (E.nwjobs > 0 && E.persist_workers && (E.nworkers > 0 || E.self_levels)) {
+        // worker slot w has completed every block whose start tick precedes f0
+        for (int w = 0; w < E.nwjobs; w++) {
+            const uint64_t k = (uint64_t)E.gsize[E.wref[w].g];
+            E.h_lvl[w] = 2 * ((f0 + k - 1) / k);
+        }
+        HIPCHK(hipMemcpyAsync(E.d_lvl_done, E.h_lvl.data(), E.h_lvl.size() * sizeof(unsigned long long),
+                              hipMemcpyHostToDevice, E.s_main));
+        HIPCHK(hipMemsetAsync(E.d_dev_tick, 0, E.inst.size() * sizeof(unsigned long long), E.s_main));
+        HIPCHK(hipStreamSynchronize(E.s_main));
+        P.wjobs = E.d_wjobs;
+        P.wrange = E.d_wrange;
+        P.nworkers = E.nworkers;
+        P.self_levels = E.self_levels ? 1 : 0;
+        P.dev_tick = E.d_dev_tick;
+        P.lvl_done = E.d_lvl_done;
+        set_masks(true, E.nworkers > 0);
+        le = launch_persist(E.N0, P, E.tw, E.s_main, std::max(E.N0, E.lds_w));
+        if (le != hipSuccess) (void)hipGetLastError();  // too many workgroups to co-reside: no workers
+        else E.workers_active = true;
+    }
+    if (le != hipSuccess) {
+        P.wjobs = nullptr;
+        P.wrange = nullptr;
+        P.nworkers = 0;
+        P.self_levels = 0;
+        P.dev_tick = P.lvl_done = nullptr;
+        set_masks(false, false);
+        le = launch_persist(E.N0, P, E.tw, E.s_main, E.N0);
+    }
     if (le != hipSuccess) {
         (void)hipGetLastError();
         return 1;  // caller falls back to per-tick launches
@@ -1096,6 +1307,11 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         t++;
     }
     *persistent_ticks = t;
+    if (E.workers_active) {
+        E.workers_active = false;
+        int rc = finish_worker_levels(E, f0 + (uint64_t)t);
+        if (rc) return rc;
+    }
     if (stalled) {
         for (; t < nframes; t++) {
             int rc = engine_tick(E, d_in, in_stride, (long long)f0, d_out, out_stride, true);
@@ -1444,11 +1660,28 @@ PC_API int pcm_trace(pc_multi *h, long long nticks)
     E.d_trace = nullptr;
     E.trace_cap = 0;
     if (nticks > 0) {
-        HIPCHK(hipMalloc(&E.d_trace, (size_t)nticks * 2 * sizeof(unsigned long long)));
-        HIPCHK(hipMemset(E.d_trace, 0, (size_t)nticks * 2 * sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&E.d_trace, (size_t)nticks * PC_TRACE_W * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(E.d_trace, 0, (size_t)nticks * PC_TRACE_W * sizeof(unsigned long long)));
         E.trace_cap = nticks;
     }
     return PC_OK;
+}
+
+PC_API int pcm_set_persistent_workers(pc_multi *h, int on)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    h->E.persist_workers = on ? 1 : 0;
+    return PC_OK;
+}
+
+PC_API int pcm_workers_info(pc_multi *h, int *self_mode)
+{
+    if (!h) return -1;
+    if (self_mode) *self_mode = 0;
+    if (!h->E.ready || h->E.nwjobs == 0) return 0;
+    if (plan_workers(h->E)) return -1;
+    if (self_mode) *self_mode = h->E.self_levels ? 1 : 0;
+    return h->E.nworkers;
 }
 
 PC_API long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap)
@@ -1458,7 +1691,8 @@ PC_API long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long 
     const long long n = std::min(cap, E.trace_cap);
     if (n <= 0) return 0;
     if (hipDeviceSynchronize() != hipSuccess) return -1;
-    if (hipMemcpy(out, E.d_trace, (size_t)n * 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(out, E.d_trace, (size_t)n * PC_TRACE_W * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
+        hipSuccess)
         return -1;
     return n;
 }

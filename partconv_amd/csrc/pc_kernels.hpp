@@ -51,6 +51,7 @@ struct TickJob {
     int kshift[PC_MAX_LEVELS];             // log2(N_L / N0)
     int NL[PC_MAX_LEVELS];
     const float *ybuf[PC_MAX_LEVELS];
+    int wslot[PC_MAX_LEVELS];              // persistent runs: level-worker slot, -1 = host-launched level
 };
 
 struct TickArgs {
@@ -79,7 +80,20 @@ struct TickArgs {
     int nactive;                    // reduce: channels with at least one unmuted source
     const int *empty_ch;            // reduce: channels with no source (zeroed each tick)
     int nempty;
+    unsigned long long *trace;      // optional timestamps (persistent runs), PC_TRACE_W per tick
+    long long trace_cap;
 };
+
+// Persistent-run trace, per tick (s_memrealtime, 100 MHz): release seen and
+// output published by workgroups 0, njobs/2 and njobs-1; completion raised;
+// workgroup 0 ready for the next release.
+#define PC_TRACE_W 8
+
+__device__ __forceinline__ void trace_stamp(const TickArgs &A, uint64_t f, int slot)
+{
+    const long long t = (long long)(f - (uint64_t)A.fbase);
+    if (A.trace && t < A.trace_cap) A.trace[PC_TRACE_W * t + slot] = __builtin_amdgcn_s_memrealtime();
+}
 
 // ---------------------------------------------------------------------------
 // Small levels: one workgroup runs the whole job in LDS (N <= 2048).
@@ -324,18 +338,29 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
             lev[q] = 0.f;
         }
     }
-    for (int L = 0; L < TJ.nlev; L++) {
-        const int ks = TJ.kshift[L];
-        const uint64_t qb = fr >> ks;
-        if (qb == 0) continue;
-        const uint64_t mb = qb - 1;
-        const uint64_t sub = fr & ((1ull << ks) - 1);
-        const float *src = TJ.ybuf[L] + (size_t)(mb & 1) * TJ.NL[L] + sub * N;
+    // level blocks four at a time: all loads of a group in flight together,
+    // then summed in level order
+    for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
+        float v[4][IPT];
 #pragma unroll
-        for (int q = 0; q < IPT; q++) {
-            const int j = tid + q * NT;
-            if (j < N) lev[q] += src[j];
+        for (int c = 0; c < 4; c++) {
+            const int L = L0 + c;
+            const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
+            const float *src = nullptr;
+            if (qb != 0) {
+                const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
+                src = TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] + sub * N;
+            }
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const int j = tid + q * NT;
+                v[c][q] = (src && j < N) ? src[j] : 0.f;
+            }
         }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < IPT; q++) lev[q] += v[c][q];
     }
     // ---- scale (src/partconvMulti.cpp:839-847), ring write, window in LDS
     float *ring = const_cast<float *>(J.ring);
@@ -434,6 +459,15 @@ __device__ __forceinline__ void tick_phaseB(const TickJob &TJ, uint64_t f, int t
     }
 }
 
+// Every thread's outstanding vector stores complete (reached L2) before the
+// barrier, so one thread's device-scope release after it covers the whole
+// workgroup's stores.
+__device__ __forceinline__ void wg_stores_done()
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+}
+
 // Publish this instance's output.
 //   direct mix (each output channel fed by exactly one unmuted instance): every
 //     workgroup writes its channel and then its own completion flag
@@ -481,10 +515,12 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         if (A.mix == 0) return;
     }
     if (A.mix == 1) {
-        // direct mix without per-workgroup flags: device ticket, last one raises done_flag
-        __syncthreads();
+        // direct mix without per-workgroup flags: device ticket, last one raises done_flag.
+        // Device-scope release only where a reader on another XCD follows: the
+        // ring at level-start ticks, or host-memory output
+        wg_stores_done();
         if (tid == 0) {
-            __threadfence();
+            if (A.fence_sys || (f & A.fence_mask) == 0) __threadfence();
             const unsigned t = atomicAdd(A.ticket, 1u);
             *s_last = (t == (unsigned)(A.njobs - 1));
         }
@@ -492,6 +528,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         if (!*s_last) return;
         if (tid == 0) {
             *A.ticket = 0u;
+            trace_stamp(A, f, 6);
             if (A.done_flag) {
                 __threadfence_system();
                 __hip_atomic_store(A.done_flag, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
@@ -537,6 +574,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     __syncthreads();
     if (tid == 0) {
         *A.ticket = 0u;
+        trace_stamp(A, f, 6);
         if (A.done_flag) {
             __threadfence_system();
             __hip_atomic_store(A.done_flag, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
@@ -632,15 +670,25 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
         xv.y = inf[TJ.in_ch + (long long)(2 * lo + 1) * A.nIn];
     } else {
         if (A.relaxed && f > 0) y0p = reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64)[lo];
-        for (int L = 0; L < TJ.nlev; L++) {
-            const int ks = TJ.kshift[L];
-            const uint64_t qb = fr >> ks;
-            if (qb == 0) continue;
-            const uint64_t mb = qb - 1;
-            const uint64_t sub = fr & ((1ull << ks) - 1);
-            const float2 v = reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)(mb & 1) * TJ.NL[L] + sub * 64)[lo];
-            lev.x += v.x;
-            lev.y += v.y;
+        // level blocks four at a time: loads in flight together, summed in level order
+        for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
+            float2 v[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int L = L0 + c;
+                const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
+                v[c] = make_float2(0.f, 0.f);
+                if (qb != 0) {
+                    const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
+                    v[c] = reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] +
+                                                            sub * 64)[lo];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                lev.x += v[c].x;
+                lev.y += v[c].y;
+            }
         }
     }
     // scale (src/partconvMulti.cpp:839-847), ring write; window pair of this lane
@@ -756,6 +804,30 @@ k_tick(TickArgs A, const cf *__restrict__ tw)
 // ---------------------------------------------------------------------------
 #define PC_GO_STOP (1ull << 62)
 
+// Level workers: workgroups of the persistent kernel that run the blocks of
+// the short-period levels (k = N_L / N0 <= 8, N_L <= 2048), so those blocks
+// need no host launch between the tick that completes their input and the
+// tick that reads their output.  Each worker owns a run of jobs (instances) of
+// one level.  Handshake (device memory, agent scope; every wait bounded):
+//   tick workgroup i, at ticks f = 0 mod k_min:  ring stores -> release ->
+//       dev_tick[i] = f + 1
+//   worker, at its level's start ticks f = m k, for each of its jobs: wait
+//       dev_tick[i] >= f + 1, acquire, block m (window -> r2c -> FDL slot ->
+//       X H_0 + S -> c2r -> double-buffer half m & 1), release,
+//       lvl_done[job] = 2m + 1; then, for each job, the delay-line sum S for
+//       block m + 1 and lvl_done[job] = 2m + 2
+//   tick workgroup i, at the first tick that reads block mb of the level:
+//       wait lvl_done[job] >= 2 mb + 1, acquire.
+// A tick workgroup that stops (PC_GO_STOP or a timeout) writes the stop bit
+// into dev_tick[i] so its workers end too.
+struct WorkJob {
+    LevelJob J;
+    int inst;      // tick workgroup (instance) whose frames feed the level
+    int kshift;    // log2(N_L / N0)
+    int logN;
+    int slot;      // index into lvl_done
+};
+
 struct PersistArgs {
     TickArgs A;
     uint64_t f0;
@@ -764,21 +836,188 @@ struct PersistArgs {
     unsigned long long *dev_go;    // relay: workgroup 0 copies go here for the others (or null)
     unsigned long long spin_limit;
     unsigned long long acq_mask;   // device-scope acquire at ticks whose (f [+1]) & mask == 0
-    unsigned long long *trace;     // optional: per tick {release seen, output published} (s_memrealtime)
-    long long trace_cap;
+    unsigned long long pub_mask;   // workers: frames handed over at ticks f & mask == 0
+    const WorkJob *wjobs;          // worker-level jobs (lvl_done slot = index)
+    const int2 *wrange;            // per worker workgroup: first job, job count (one level)
+    int nworkers;                  // workgroups njobs .. njobs + nworkers - 1
+    int self_levels;               // no workers: each tick workgroup runs its own worker-level blocks
+    unsigned long long *dev_tick;  // per tick workgroup: frames published to the workers
+    unsigned long long *lvl_done;  // per worker slot: 2m + 1 (block m out), 2m + 2 (S of m + 1 too)
 };
+
+
+// S[k] = sum_{p=1}^{P-1} FDL[(b - p) % P][k] H_p[k] for one job, by one workgroup
+// (two bins per thread and step, as k_partial).
+template <int NT>
+__device__ void wg_partial(const LevelJob &J, int N, uint64_t b, int tid)
+{
+    const int P = J.P;
+    const float4 *fdl = reinterpret_cast<const float4 *>(J.fdl);
+    const float4 *H = reinterpret_cast<const float4 *>(J.H);
+    const size_t row = (size_t)(N / 2);
+    const int s0 = (int)(b % (uint64_t)P);
+    for (int k2 = tid; k2 < N / 2; k2 += NT) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        int s = s0;
+#pragma unroll 4
+        for (int p = 1; p < P; p++) {
+            s = (s == 0) ? P - 1 : s - 1;
+            const float4 a = fdl[(size_t)s * row + k2];
+            const float4 h = H[(size_t)p * row + k2];
+            if (k2 == 0) {
+                acc.x += a.x * h.x;
+                acc.y += a.y * h.y;
+            } else {
+                acc.x += a.x * h.x - a.y * h.y;
+                acc.y += a.x * h.y + a.y * h.x;
+            }
+            acc.z += a.z * h.z - a.w * h.w;
+            acc.w += a.z * h.w + a.w * h.z;
+        }
+        reinterpret_cast<float4 *>(J.S)[k2] = acc;
+    }
+}
+
+// Bounded spin of one thread until *p >= target.  Returns 1 when reached, 0 on
+// the stop bit or the time limit.
+__device__ __forceinline__ int spin_geq(const unsigned long long *p, unsigned long long target,
+                                        unsigned long long limit, int sleep_rounds)
+{
+    const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + limit;
+    for (;;) {
+        const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v & PC_GO_STOP) return 0;
+        if (v >= target) return 1;
+        if (__builtin_amdgcn_s_memrealtime() > t_end) return 0;
+        for (int r = 0; r < sleep_rounds; r++) __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Block m of one worker-level job (window -> r2c -> FDL slot -> X H_0 + S ->
+// c2r -> double-buffer half m & 1), all in LDS, by one workgroup.
+template <int LOGNL, int NT>
+__device__ void level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf *bufB, const cf *__restrict__ tw,
+                            int tid)
+{
+    constexpr int N = 1 << LOGNL, IPT = (N + NT - 1) / NT;
+    cf h[IPT], sv[IPT];
+    small_prefetch<LOGNL, NT>(J, tid, h, sv);
+    __syncthreads();  // the LDS buffers may still be read by the previous user
+    small_load_window<LOGNL, NT>(bufA, J, PC_MODE_RUN, (long long)m * N - N0 - J.T, tid);
+    cf *X = small_forward<LOGNL, NT>(bufA, bufB, tw, tid);
+    cf *other = (X == bufA) ? bufB : bufA;
+    small_mac<LOGNL, NT>(X, J, m, tid, h, sv);
+    cf *z = small_inverse<LOGNL, NT>(X, other, tw, tid);
+    cf *yo = reinterpret_cast<cf *>(J.ybuf + (size_t)(m & 1) * N);
+    for (int n = tid; n < N / 2; n += NT) yo[n] = z[n];
+}
+
+// One level worker: every block of its jobs whose start tick lies in the run.
+template <int LOGNL, int NT>
+__device__ void level_worker(const PersistArgs &P, int2 range, cf *bufA, cf *bufB, const cf *__restrict__ tw,
+                             int tid, int *s_ok)
+{
+    constexpr int N = 1 << LOGNL;
+    const int ks = P.wjobs[range.x].kshift;
+    const uint64_t k = 1ull << ks;
+    const uint64_t fend = P.f0 + (uint64_t)P.nticks;
+    for (uint64_t f = (P.f0 + k - 1) & ~(k - 1); f < fend; f += k) {
+        const uint64_t m = f >> ks;
+        for (int q = range.x; q < range.x + range.y; q++) {
+            const WorkJob &W = P.wjobs[q];
+            if (tid == 0) {
+                const int ok = spin_geq(P.dev_tick + W.inst, f + 1, P.spin_limit, 4);
+                if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                *s_ok = ok;
+            }
+            __syncthreads();
+            if (!*s_ok) return;
+            level_block<LOGNL, NT>(W.J, m, P.A.N0, bufA, bufB, tw, tid);
+            wg_stores_done();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(P.lvl_done + q, 2 * m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        // delay-line sums for the next blocks, off the deadline path
+        for (int q = range.x; q < range.x + range.y; q++) {
+            wg_partial<NT>(P.wjobs[q].J, N, m + 1, tid);
+            wg_stores_done();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(P.lvl_done + q, 2 * m + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+// Self mode: a tick workgroup runs block m of its own worker-level job q right
+// after tick f = m k (the consumer is itself, so no handshake is needed; the
+// lvl_done stores only report progress to the host after the kernel ends).
+template <int LOGN>
+__device__ void self_level(const PersistArgs &P, int q, uint64_t m, cf *lds, const cf *__restrict__ tw, int tid)
+{
+    constexpr int NT = TickCfg<LOGN>::NT;
+    const WorkJob &W = P.wjobs[q];
+    const int d = W.logN - LOGN;
+    if constexpr (LOGN + 1 <= 11) {
+        if (d == 1) {
+            level_block<LOGN + 1, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 1)), tw, tid);
+        }
+    }
+    if constexpr (LOGN + 2 <= 11) {
+        if (d == 2) {
+            level_block<LOGN + 2, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 2)), tw, tid);
+        }
+    }
+    if constexpr (LOGN + 3 <= 11) {
+        if (d == 3) {
+            level_block<LOGN + 3, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 3)), tw, tid);
+        }
+    }
+    if (tid == 0) __hip_atomic_store(P.lvl_done + q, 2 * m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    wg_partial<NT>(W.J, 1 << W.logN, m + 1, tid);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(P.lvl_done + q, 2 * m + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int LOGN>
+__device__ void level_worker_dispatch(const PersistArgs &P, int2 range, cf *bufA, cf *bufB,
+                                      const cf *__restrict__ tw, int tid, int *s_ok)
+{
+    constexpr int NT = TickCfg<LOGN>::NT;
+    const int d = P.wjobs[range.x].logN - LOGN;
+    if constexpr (LOGN + 1 <= 11) {
+        if (d == 1) return level_worker<LOGN + 1, NT>(P, range, bufA, bufB, tw, tid, s_ok);
+    }
+    if constexpr (LOGN + 2 <= 11) {
+        if (d == 2) return level_worker<LOGN + 2, NT>(P, range, bufA, bufB, tw, tid, s_ok);
+    }
+    if constexpr (LOGN + 3 <= 11) {
+        if (d == 3) return level_worker<LOGN + 3, NT>(P, range, bufA, bufB, tw, tid, s_ok);
+    }
+}
 
 template <int LOGN>
 __global__ void __launch_bounds__(TickCfg<LOGN>::NT)
 k_persist(PersistArgs P, const cf *__restrict__ tw)
 {
     constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
-    __shared__ cf bufA[N];
-    __shared__ cf bufB[N];
+    // 2 x max(N0, largest worker block) float2, sized at launch
+    extern __shared__ cf pc_lds[];
     __shared__ unsigned s_last;
     __shared__ int s_go;
     __shared__ TickJob sTJ;
     const int tid = threadIdx.x;
+    if ((int)blockIdx.x >= P.A.njobs) {
+        const int2 range = P.wrange[blockIdx.x - P.A.njobs];
+        const int lw = P.wjobs[range.x].logN;
+        level_worker_dispatch<LOGN>(P, range, pc_lds, pc_lds + (1 << lw), tw, tid, &s_go);
+        return;
+    }
+    cf *bufA = pc_lds;
+    cf *bufB = pc_lds + N;
     {
         const unsigned *src = reinterpret_cast<const unsigned *>(P.A.jobs + blockIdx.x);
         unsigned *dst = reinterpret_cast<unsigned *>(&sTJ);
@@ -811,30 +1050,60 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
                 if (__builtin_amdgcn_s_memrealtime() > t_end) break;
                 __builtin_amdgcn_s_sleep(1);
             }
-            // level blocks written by other kernels (other XCDs) are first read
-            // at deadline ticks: acquire at device scope there
-            if (go && (((P.A.relaxed ? f : f + 1) & P.acq_mask) == 0))
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint64_t fr = P.A.relaxed ? f : f + 1;
+            // blocks of worker levels first read at this tick
+            bool waited = false;
+            for (int L = 0; go && P.nworkers > 0 && L < sTJ.nlev; L++) {
+                const int slot = sTJ.wslot[L];
+                if (slot < 0) continue;
+                const int ks = sTJ.kshift[L];
+                if ((fr & ((1ull << ks) - 1)) != 0 || (fr >> ks) == 0) continue;
+                const unsigned long long need = 2 * ((fr >> ks) - 1) + 1;
+                go = spin_geq(P.lvl_done + slot, need, P.spin_limit, 0);
+                waited = true;
+            }
+            // level blocks written by other workgroups or kernels (other XCDs) are
+            // first read at deadline ticks: acquire at device scope there
+            if (go && (waited || (fr & P.acq_mask) == 0)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (!go && P.dev_tick)
+                __hip_atomic_store(P.dev_tick + blockIdx.x, PC_GO_STOP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_go = go;
         }
         __syncthreads();
         if (!s_go) return;
-        const bool tr = P.trace && blockIdx.x == 0 && tid == 0 && t < P.trace_cap;
-        if (tr) P.trace[2 * t] = __builtin_amdgcn_s_memrealtime();
+        const int tslot = (blockIdx.x == 0) ? 0 : (blockIdx.x == P.A.njobs / 2) ? 2 : (blockIdx.x == P.A.njobs - 1) ? 4 : -1;
+        const bool tr = P.A.trace && tslot >= 0 && tid == 0;
+        if (tr) trace_stamp(P.A, f, tslot);
         float acc[IPT];
         if constexpr (LOGN == 6) {
             cf xr;
             tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
-            if (tr) P.trace[2 * t + 1] = __builtin_amdgcn_s_memrealtime();
+            if (tr) trace_stamp(P.A, f, tslot + 1);
             tick64_phaseB(sTJ, f, xr, st64);
         } else {
             cf xr[IPT];
             tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
-            if (tr) P.trace[2 * t + 1] = __builtin_amdgcn_s_memrealtime();
+            if (tr) trace_stamp(P.A, f, tslot + 1);
             tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
         }
+        if (P.self_levels) {
+            for (int L = 0; L < sTJ.nlev; L++) {
+                const int q = sTJ.wslot[L];
+                if (q < 0 || (f & ((1ull << sTJ.kshift[L]) - 1)) != 0) continue;
+                self_level<LOGN>(P, q, f >> sTJ.kshift[L], pc_lds, tw, tid);
+            }
+        }
+        if (P.nworkers > 0 && (f & P.pub_mask) == 0) {
+            // the frame is in the ring: hand it to this instance's level workers
+            wg_stores_done();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(P.dev_tick + blockIdx.x, f + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (tr && tslot == 0) trace_stamp(P.A, f, 7);
         __syncthreads();
     }
 }

@@ -344,11 +344,28 @@ class PartConvMulti:
         """Record GPU timestamps of the next persistent run (see pcm_trace)."""
         return _report(lib().pcm_trace(self._h, int(nticks)), "trace")
 
+    def set_persistent_workers(self, on):
+        """Level workers inside the persistent kernel on/off (identical output)."""
+        return _report(lib().pcm_set_persistent_workers(self._h, 1 if on else 0), "set_persistent_workers")
+
+    @property
+    def workers(self):
+        """Number of level-worker workgroups a persistent run adds (0 if none)."""
+        return lib().pcm_workers_info(self._h, None)
+
+    @property
+    def self_mode(self):
+        """True when the tick workgroups run the short-period levels themselves."""
+        v = ctypes.c_int()
+        lib().pcm_workers_info(self._h, ctypes.byref(v))
+        return bool(v.value)
+
     def trace_read(self, cap=1 << 20):
-        """[(release_seen, published)] GPU clock (100 MHz ticks) per tick."""
-        buf = (ctypes.c_ulonglong * (2 * cap))()
+        """Per tick 8 GPU clock values (100 MHz): release seen / published by
+        workgroups 0, njobs/2, njobs-1; completion raised; workgroup 0 ready."""
+        buf = (ctypes.c_ulonglong * (8 * cap))()
         n = lib().pcm_trace_read(self._h, buf, cap)
-        return np.frombuffer(buf, dtype=np.uint64)[:2 * max(n, 0)].reshape(-1, 2)
+        return np.frombuffer(buf, dtype=np.uint64)[:8 * max(n, 0)].reshape(-1, 8)
 
     def instance_partition(self, i):
         N = (ctypes.c_int * 16)()

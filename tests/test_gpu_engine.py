@@ -250,6 +250,95 @@ def test_persistent_tick_kernel_matches_launch_mode():
         m.cleanup()
 
 
+@pytest.mark.parametrize("relaxed", [False, True])
+def test_persistent_level_workers_match_host_launched_levels(relaxed):
+    """Levels with period 2, 4 and 8 ticks run inside the persistent kernel
+    (level workers); the result must be bitwise the one with every level
+    launched from the host, and the per-tick launch path.  Runs are split at
+    ticks that are not multiples of the level periods."""
+    say(f"case level workers relaxed={relaxed}")
+    import torch
+    from partconv_amd import greedy_partition
+    rng = np.random.default_rng(12)
+    bs = [64 << i for i in range(9)]
+    taps = [30000, 9000, 70000]
+    insts, irs = [], []
+    for i, T in enumerate(taps):
+        N, P = greedy_partition(bs, T)
+        insts.append(dict(N=N, P=P, in_ch=i, out_ch=i))
+        irs.append(decaying_noise_ir(T, 1.0, 48000, 120 + i))
+    nfr = 611
+    x = rng.standard_normal((nfr * 64, 3)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    outs = []
+    for variant in ("launch", "workers", "no-workers"):
+        m = make_multi(insts, irs, 64, 3, 3, relaxed=relaxed)
+        if variant == "workers":
+            assert m.workers == sum(1 for i in insts for n in i["N"][1:] if n <= 8 * 64)
+        if variant == "no-workers":
+            m.set_persistent_workers(False)
+        yd = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        cuts = [0, 1, 302, 303, 500, nfr]
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            off = a * 64 * 3 * 4
+            assert m.run_device(yd.data_ptr() + off, xd.data_ptr() + off, b - a, True,
+                                persistent=(variant != "launch")) == 0, last_error()
+            say(f"  {variant} ticks {a}-{b}: persistent so far {m.stats().persistent_ticks}")
+        if variant != "launch":
+            assert m.stats().persistent_ticks == nfr
+        outs.append(yd.cpu().numpy())
+        m.cleanup()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+    for i in range(3):
+        r = O.direct_conv_f64(x[:, i], irs[i])
+        if relaxed:
+            r = np.concatenate([np.zeros(64), r[:-64]])
+        assert rel(outs[1][:, i], r) < TOL, i
+
+
+def test_many_instances_run_short_levels_in_tick_workgroups():
+    """Many instances: one worker workgroup per job would not fit in half of
+    the device, so each tick workgroup runs its own instance's short-period
+    level blocks (self mode).  Bitwise equal to the per-tick launch path; spot
+    channels against fp64."""
+    say("case self mode (many instances)")
+    import torch
+    from partconv_amd import greedy_partition
+    n = 300
+    rng = np.random.default_rng(21)
+    lens = rng.integers(4096, 8192, size=n)
+    bs = [128 << i for i in range(9)]
+    insts, irs = [], []
+    for i, T in enumerate(lens):
+        N, P = greedy_partition(bs, int(T))
+        insts.append(dict(N=N, P=P, in_ch=i, out_ch=i))
+        irs.append(decaying_noise_ir(int(T), 0.3, 48000, 1000 + i))
+    nfr = 97
+    x = rng.standard_normal((nfr * 128, n)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    outs = []
+    for persistent in (False, True):
+        m = make_multi(insts, irs, 128, n, n)
+        if persistent:
+            say(f"  worker workgroups {m.workers}, self mode {m.self_mode}")
+            assert m.workers == 0 and m.self_mode
+        yd = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        for a, b in ((0, 45), (45, nfr)):
+            off = a * 128 * n * 4
+            assert m.run_device(yd.data_ptr() + off, xd.data_ptr() + off, b - a, True,
+                                persistent=persistent) == 0, last_error()
+        if persistent:
+            assert m.stats().persistent_ticks == nfr
+        outs.append(yd.cpu().numpy())
+        m.cleanup()
+    assert np.array_equal(outs[0], outs[1])
+    for i in (0, 137, n - 1):
+        assert rel(outs[1][:, i], O.direct_conv_f64(x[:, i], irs[i])) < TOL, i
+
+
 def test_uneven_batched_instances_greedy_partitions():
     """Configuration-3 shape at reduced size: uneven IR lengths (log-uniform),
     each instance with its own greedy partition over 128..32768, one batched
