@@ -32,6 +32,7 @@
 #define PC_MAX_BLOCK 65536
 #define PC_MIN_BLOCK 16
 #define PC_TW_MAX (2 * PC_MAX_BLOCK)
+#define PC_GO_WORD 1          // release counter: the word after the completion flag (same line)
 #define PC_TICK_TIMEOUT_S 10.0
 
 static thread_local std::string g_err;
@@ -209,6 +210,8 @@ struct Engine {
     int *d_src_off = nullptr, *d_src_list = nullptr;
     unsigned *d_ticket = nullptr;
     unsigned *d_ticket_ch = nullptr;     // per output channel (reduce mix)
+    unsigned *d_ticket_grp = nullptr;  // per group of PC_TICKET_GROUP workgroups + global (direct mix)
+    size_t ticket_grp_bytes = 0;
     int *d_empty = nullptr;              // output channels without a source
     int nactive = 0, nempty = 0;
     float *h_in = nullptr, *h_out = nullptr;
@@ -230,10 +233,6 @@ struct Engine {
     long long probe_n = 0;
     // level workers of persistent runs (short-period small levels, batched engine)
     std::vector<char> g_worker;       // per global level: served by workers in persistent runs
-    struct WRef {
-        int g, batch, j;
-    };
-    std::vector<WRef> wref;           // per worker slot
     std::vector<WorkJob> wjobs_h;
     WorkJob *d_wjobs = nullptr;
     int nwjobs = 0, lds_w = 0;        // worker-level jobs, largest worker block size
@@ -241,7 +240,6 @@ struct Engine {
     bool self_levels = false;         // no room for workers: tick workgroups run those levels
     int2 *d_wrange = nullptr;
     unsigned long long *d_lvl_done = nullptr, *d_dev_tick = nullptr;
-    std::vector<unsigned long long> h_lvl;   // staging for lvl_done
     bool workers_active = false;      // a persistent run with workers owns those levels
     int persist_workers = 1;          // 0: persistent runs launch every level from the host
     // stats
@@ -267,7 +265,6 @@ static void engine_free(Engine &E)
     E.d_wrange = nullptr;
     E.nworkers = -1;
     E.nwjobs = 0;
-    E.wref.clear();
     E.wjobs_h.clear();
     E.g_worker.clear();
     for (Group &G : E.groups) {
@@ -283,6 +280,8 @@ static void engine_free(Engine &E)
     hipFree(E.d_src_list);
     hipFree(E.d_ticket);
     hipFree(E.d_ticket_ch);
+    hipFree(E.d_ticket_grp);
+    E.d_ticket_grp = nullptr;
     hipFree(E.d_empty);
     E.d_ticket_ch = nullptr;
     E.d_empty = nullptr;
@@ -647,7 +646,6 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
                     w.slot = (int)E.wjobs_h.size();
                     wslot[mem[pos[j]]][g] = w.slot;
                     E.wjobs_h.push_back(w);
-                    E.wref.push_back({g, (int)Gr.batches.size(), (int)j});
                     E.lds_w = std::max(E.lds_w, b.N);
                 }
             }
@@ -689,7 +687,6 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         HIPCHK(hipMemcpy(E.d_wjobs, E.wjobs_h.data(), E.wjobs_h.size() * sizeof(WorkJob), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&E.d_lvl_done, E.wjobs_h.size() * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&E.d_dev_tick, E.inst.size() * sizeof(unsigned long long)));
-        E.h_lvl.assign(E.wjobs_h.size(), 0ull);
     }
     // ---- output mix routing
     std::vector<int> src_off(nOut + 1, 0), src_list, empty;
@@ -723,6 +720,10 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         HIPCHK(hipMemcpy(E.d_src_list, src_list.data(), src_list.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&E.d_ticket, 64));
     HIPCHK(hipMemset(E.d_ticket, 0, 64));
+    E.ticket_grp_bytes = ((E.inst.size() + PC_TICKET_GROUP - 1) / PC_TICKET_GROUP + 1) * PC_TICKET_STRIDE *
+                         sizeof(unsigned);
+    HIPCHK(hipMalloc(&E.d_ticket_grp, E.ticket_grp_bytes));
+    HIPCHK(hipMemset(E.d_ticket_grp, 0, E.ticket_grp_bytes));
 
     // ---- streams, events, pinned staging
     // Level streams get hardware queues of their own (a stream with a CU mask,
@@ -747,6 +748,8 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     HIPCHK(hipHostMalloc(&E.h_in, (size_t)N0 * nIn * sizeof(float), hipHostMallocCoherent | hipHostMallocMapped));
     HIPCHK(hipHostMalloc(&E.h_out, (size_t)N0 * std::max(nOut, 1) * sizeof(float),
                          hipHostMallocCoherent | hipHostMallocMapped));
+    // completion flag at word 0, release counter at word PC_GO_WORD: one cache
+    // line (measured faster than two separate lines)
     HIPCHK(hipHostMalloc(&E.h_flag, 64, hipHostMallocCoherent | hipHostMallocMapped));
     memset(E.h_flag, 0, 64);
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_in_map, E.h_in, 0));
@@ -798,6 +801,7 @@ static int engine_reset(Engine &E)
     }
     HIPCHK(hipMemsetAsync(E.d_ticket, 0, 64, E.s_main));
     HIPCHK(hipMemsetAsync(E.d_ticket_ch, 0, std::max(E.nOut, 1) * sizeof(unsigned), E.s_main));
+    HIPCHK(hipMemsetAsync(E.d_ticket_grp, 0, E.ticket_grp_bytes, E.s_main));
     HIPCHK(hipStreamSynchronize(E.s_main));
     E.frame = 0;
     std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
@@ -976,6 +980,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
     a.ticket_ch = E.d_ticket_ch;
+    a.ticket_grp = E.d_ticket_grp;
     a.nactive = E.nactive;
     a.empty_ch = E.d_empty;
     a.nempty = E.nempty;
@@ -1118,42 +1123,35 @@ static int plan_workers(Engine &E)
     return PC_OK;
 }
 
-// After a persistent run with level workers: every worker-level block whose
-// start tick precedes tick `fend` (and its delay-line sum for the next block)
-// must be complete.  Blocks a worker did not finish (a stopped or timed-out
-// run) are launched here, per job, on the level's stream.
-static int finish_worker_levels(Engine &E, uint64_t fend)
-{
-    HIPCHK(hipMemcpyAsync(E.h_lvl.data(), E.d_lvl_done, E.h_lvl.size() * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, E.s_main));
-    HIPCHK(hipStreamSynchronize(E.s_main));
-    Group &Gr = E.groups[0];
-    for (int w = 0; w < E.nwjobs; w++) {
-        const Engine::WRef &r = E.wref[w];
-        const Batch &b = Gr.batches[r.batch];
-        const uint64_t k = (uint64_t)E.gsize[r.g];
-        const uint64_t m_end = (fend + k - 1) / k;   // blocks 0 .. m_end-1 are due
-        const unsigned long long d = E.h_lvl[w];
-        uint64_t next = (d + 1) / 2;                 // first block not written
-        if (next >= m_end && !(d & 1)) continue;
-        hipStream_t s = level_stream(E, r.g);
-        const LevelJob *J = b.d_jobs + r.j;
-        if (d & 1) launch_partial(b.N, J, 1, next, s, LaunchExtra());  // block next-1 out, its S missing
-        for (uint64_t m = next; m < m_end; m++) {
-            int rc = launch_small(b.N, J, 1, E.N0, m, m, (int)(m & 1), E.tw, PC_MODE_RUN, s);
-            if (rc) return rc;
-            launch_partial(b.N, J, 1, m + 1, s, LaunchExtra());
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(E.ev_lvl[r.g], s));
-        E.lvl_pending[r.g] = true;
-    }
-    return PC_OK;
-}
-
 static void store_go(Engine &E, unsigned long long v)
 {
-    __atomic_store_n(E.h_flag + 1, v, __ATOMIC_RELEASE);
+    __atomic_store_n(E.h_flag + PC_GO_WORD, v, __ATOMIC_RELEASE);
+}
+
+template <int LOGN>
+static void launch_worker_fix_t(Engine &E, uint64_t fend)
+{
+    hipLaunchKernelGGL(k_worker_fix<LOGN>, dim3(E.nwjobs), dim3(TickCfg<LOGN>::NT),
+                       (size_t)2 * E.lds_w * sizeof(cf), E.s_main, E.d_wjobs, E.d_lvl_done, fend, E.N0, E.tw);
+}
+
+// After a persistent run with level workers: complete, on the device, every
+// worker-level block with start tick < fend that the run did not finish
+// (k_worker_fix), ordered on the tick stream before anything that follows.
+static int finish_worker_levels(Engine &E, uint64_t fend)
+{
+    switch (ilog2(E.N0)) {
+    case 4: launch_worker_fix_t<4>(E, fend); break;
+    case 5: launch_worker_fix_t<5>(E, fend); break;
+    case 6: launch_worker_fix_t<6>(E, fend); break;
+    case 7: launch_worker_fix_t<7>(E, fend); break;
+    case 8: launch_worker_fix_t<8>(E, fend); break;
+    case 9: launch_worker_fix_t<9>(E, fend); break;
+    case 10: launch_worker_fix_t<10>(E, fend); break;
+    default: return fail(PC_ERR_ARG, "no worker levels at this buffer size");
+    }
+    HIPCHK(hipGetLastError());
+    return PC_OK;
 }
 
 // Device-resident ticks served by one persistent tick kernel (batched engine).
@@ -1191,6 +1189,7 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.src_list = E.d_src_list;
     a.ticket = E.d_ticket;
     a.ticket_ch = E.d_ticket_ch;
+    a.ticket_grp = E.d_ticket_grp;
     a.nactive = E.nactive;
     a.empty_ch = E.d_empty;
     a.nempty = E.nempty;
@@ -1214,14 +1213,16 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         P.pub_mask = (separate && kw) ? kw - 1 : ~0ull;
     };
     set_masks(false, false);
-    // relay through device memory (d_ticket[2..3]) when many workgroups would poll the bus
+    // with many workgroups only workgroup 0 polls the bus for the release and
+    // relays it through device memory (d_ticket[2..3]); up to 32 each poll the
+    // bus themselves (measured faster than the relay at 16 instances)
     P.dev_go = (a.njobs > 32) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
     if (P.dev_go) HIPCHK(hipMemsetAsync(P.dev_go, 0, sizeof(unsigned long long), E.s_main));
     a.trace = E.d_trace;
     a.trace_cap = E.trace_cap;
     P.f0 = f0;
     P.nticks = nframes;
-    P.go = E.d_flag_map + 1;
+    P.go = E.d_flag_map + PC_GO_WORD;
     P.spin_limit = 200000000ull;  // 2 s at 100 MHz
     // the levels the first tick reads must be complete before the kernel owns the tick stream
     const int G = (int)E.gsize.size();
@@ -1234,15 +1235,11 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         if (rc) return rc;
     }
     if (E.nwjobs > 0 && E.persist_workers && (E.nworkers > 0 || E.self_levels)) {
-        // worker slot w has completed every block whose start tick precedes f0
-        for (int w = 0; w < E.nwjobs; w++) {
-            const uint64_t k = (uint64_t)E.gsize[E.wref[w].g];
-            E.h_lvl[w] = 2 * ((f0 + k - 1) / k);
-        }
-        HIPCHK(hipMemcpyAsync(E.d_lvl_done, E.h_lvl.data(), E.h_lvl.size() * sizeof(unsigned long long),
-                              hipMemcpyHostToDevice, E.s_main));
-        HIPCHK(hipMemsetAsync(E.d_dev_tick, 0, E.inst.size() * sizeof(unsigned long long), E.s_main));
-        HIPCHK(hipStreamSynchronize(E.s_main));
+        // worker-level progress counters for a history served up to tick f0 - 1
+        const int n = std::max(E.nwjobs, (int)E.inst.size());
+        hipLaunchKernelGGL(k_worker_init, dim3((n + 255) / 256), dim3(256), 0, E.s_main, E.d_wjobs, E.nwjobs,
+                           (uint64_t)f0, E.d_lvl_done, E.d_dev_tick, (int)E.inst.size());
+        HIPCHK(hipGetLastError());
         P.wjobs = E.d_wjobs;
         P.wrange = E.d_wrange;
         P.nworkers = E.nworkers;
@@ -1296,8 +1293,15 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         int rc = start_levels(E, f, false);
         if (rc) return rc;
     }
-    if (stalled) store_go(E, PC_GO_STOP);
-    HIPCHK(hipStreamSynchronize(E.s_main));
+    if (stalled) {
+        store_go(E, PC_GO_STOP);
+        HIPCHK(hipStreamSynchronize(E.s_main));
+        // a stopped tick may have counted only some of its arrivals
+        HIPCHK(hipMemsetAsync(E.d_ticket, 0, 64, E.s_main));
+        HIPCHK(hipMemsetAsync(E.d_ticket_ch, 0, std::max(E.nOut, 1) * sizeof(unsigned), E.s_main));
+        HIPCHK(hipMemsetAsync(E.d_ticket_grp, 0, E.ticket_grp_bytes, E.s_main));
+        HIPCHK(hipStreamSynchronize(E.s_main));
+    }
     if (stalled && t < nframes && tick_done(E, f0 + (uint64_t)t + 1)) {
         // the kernel served the tick after all (slow, not stalled): account for it
         const uint64_t f = f0 + (uint64_t)t;

@@ -77,12 +77,25 @@ struct TickArgs {
     int fence_sys;                  // output lives in host memory: system-scope release per tick
     unsigned long long fence_mask;  // else: device-scope release at ticks with (f & mask) == 0
     unsigned *ticket_ch;            // reduce: arrival counter per output channel
+    unsigned *ticket_grp;           // direct: arrival counters per group of PC_TICKET_GROUP workgroups
     int nactive;                    // reduce: channels with at least one unmuted source
     const int *empty_ch;            // reduce: channels with no source (zeroed each tick)
     int nempty;
     unsigned long long *trace;      // optional timestamps (persistent runs), PC_TRACE_W per tick
     long long trace_cap;
 };
+
+// Completion of many workgroups: arrivals are counted per group of
+// PC_TICKET_GROUP workgroups, and the last of each group arrives at a global
+// counter (counters PC_TICKET_STRIDE words apart: own cache lines; the global
+// one after the groups).  The counters only grow, by a power of two per tick
+// (PC_TICKET_GROUP per group: the first member of a short last group adds
+// the missing weight; the next power of two >= groups at the global counter,
+// the first group adding the missing weight), so the arrival that completes a
+// tick is the one that makes its counter a multiple of that power of two --
+// modulo 2^32 as well -- and nothing is reset between ticks.
+#define PC_TICKET_GROUP 16
+#define PC_TICKET_STRIDE 32
 
 // Persistent-run trace, per tick (s_memrealtime, 100 MHz): release seen and
 // output published by workgroups 0, njobs/2 and njobs-1; completion raised;
@@ -338,12 +351,13 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
             lev[q] = 0.f;
         }
     }
-    // level blocks four at a time: all loads of a group in flight together,
-    // then summed in level order
-    for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
-        float v[4][IPT];
+    // level blocks CH at a time: all loads of a group in flight together, then
+    // summed in level order
+    constexpr int CH = (IPT >= 8) ? 2 : 16 / IPT;
+    for (int L0 = 0; L0 < TJ.nlev; L0 += CH) {
+        float v[CH][IPT];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < CH; c++) {
             const int L = L0 + c;
             const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
             const float *src = nullptr;
@@ -358,7 +372,7 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
             }
         }
 #pragma unroll
-        for (int c = 0; c < 4; c++)
+        for (int c = 0; c < CH; c++)
 #pragma unroll
             for (int q = 0; q < IPT; q++) lev[q] += v[c][q];
     }
@@ -521,13 +535,24 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         wg_stores_done();
         if (tid == 0) {
             if (A.fence_sys || (f & A.fence_mask) == 0) __threadfence();
-            const unsigned t = atomicAdd(A.ticket, 1u);
-            *s_last = (t == (unsigned)(A.njobs - 1));
+            const int grp = i / PC_TICKET_GROUP;
+            const int ngrp = (A.njobs + PC_TICKET_GROUP - 1) / PC_TICKET_GROUP;
+            const int gsz = min(PC_TICKET_GROUP, A.njobs - grp * PC_TICKET_GROUP);
+            const unsigned wg = (i == grp * PC_TICKET_GROUP) ? (unsigned)(PC_TICKET_GROUP - gsz + 1) : 1u;
+            const unsigned tg = atomicAdd(A.ticket_grp + (size_t)grp * PC_TICKET_STRIDE, wg) + wg;
+            int last = 0;
+            if ((tg & (PC_TICKET_GROUP - 1)) == 0) {
+                const unsigned p2 = 1u << (32 - __clz(ngrp - 1));   // next power of two >= ngrp (ngrp > 1)
+                const unsigned pw = (ngrp == 1) ? 1u : p2;
+                const unsigned w = (grp == 0) ? pw - (unsigned)ngrp + 1u : 1u;
+                const unsigned t = atomicAdd(A.ticket_grp + (size_t)ngrp * PC_TICKET_STRIDE, w) + w;
+                last = ((t & (pw - 1)) == 0);
+            }
+            *s_last = last;
         }
         __syncthreads();
         if (!*s_last) return;
         if (tid == 0) {
-            *A.ticket = 0u;
             trace_stamp(A, f, 6);
             if (A.done_flag) {
                 __threadfence_system();
@@ -849,7 +874,7 @@ struct PersistArgs {
 // S[k] = sum_{p=1}^{P-1} FDL[(b - p) % P][k] H_p[k] for one job, by one workgroup
 // (two bins per thread and step, as k_partial).
 template <int NT>
-__device__ void wg_partial(const LevelJob &J, int N, uint64_t b, int tid)
+__device__ __forceinline__ void wg_partial(const LevelJob &J, int N, uint64_t b, int tid)
 {
     const int P = J.P;
     const float4 *fdl = reinterpret_cast<const float4 *>(J.fdl);
@@ -896,7 +921,7 @@ __device__ __forceinline__ int spin_geq(const unsigned long long *p, unsigned lo
 // Block m of one worker-level job (window -> r2c -> FDL slot -> X H_0 + S ->
 // c2r -> double-buffer half m & 1), all in LDS, by one workgroup.
 template <int LOGNL, int NT>
-__device__ void level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf *bufB, const cf *__restrict__ tw,
+__device__ __forceinline__ void level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf *bufB, const cf *__restrict__ tw,
                             int tid)
 {
     constexpr int N = 1 << LOGNL, IPT = (N + NT - 1) / NT;
@@ -914,7 +939,7 @@ __device__ void level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf 
 
 // One level worker: every block of its jobs whose start tick lies in the run.
 template <int LOGNL, int NT>
-__device__ void level_worker(const PersistArgs &P, int2 range, cf *bufA, cf *bufB, const cf *__restrict__ tw,
+__device__ __forceinline__ void level_worker(const PersistArgs &P, int2 range, cf *bufA, cf *bufB, const cf *__restrict__ tw,
                              int tid, int *s_ok)
 {
     constexpr int N = 1 << LOGNL;
@@ -951,11 +976,11 @@ __device__ void level_worker(const PersistArgs &P, int2 range, cf *bufA, cf *buf
     }
 }
 
-// Self mode: a tick workgroup runs block m of its own worker-level job q right
-// after tick f = m k (the consumer is itself, so no handshake is needed; the
+// Self mode: a tick workgroup runs block m of its own worker-level job q after
+// one of its ticks (the consumer is itself, so no handshake is needed; the
 // lvl_done stores only report progress to the host after the kernel ends).
 template <int LOGN>
-__device__ void self_level(const PersistArgs &P, int q, uint64_t m, cf *lds, const cf *__restrict__ tw, int tid)
+__device__ __forceinline__ void self_level(const PersistArgs &P, int q, uint64_t m, cf *lds, const cf *__restrict__ tw, int tid)
 {
     constexpr int NT = TickCfg<LOGN>::NT;
     const WorkJob &W = P.wjobs[q];
@@ -983,7 +1008,7 @@ __device__ void self_level(const PersistArgs &P, int q, uint64_t m, cf *lds, con
 }
 
 template <int LOGN>
-__device__ void level_worker_dispatch(const PersistArgs &P, int2 range, cf *bufA, cf *bufB,
+__device__ __forceinline__ void level_worker_dispatch(const PersistArgs &P, int2 range, cf *bufA, cf *bufB,
                                       const cf *__restrict__ tw, int tid, int *s_ok)
 {
     constexpr int NT = TickCfg<LOGN>::NT;
@@ -996,6 +1021,83 @@ __device__ void level_worker_dispatch(const PersistArgs &P, int2 range, cf *bufA
     }
     if constexpr (LOGN + 3 <= 11) {
         if (d == 3) return level_worker<LOGN + 3, NT>(P, range, bufA, bufB, tw, tid, s_ok);
+    }
+}
+
+// Before a persistent run from tick f0: every worker-level block whose start
+// tick precedes f0 is complete (the host launched it), so job q starts at
+// lvl_done = 2 ceil(f0 / k_q).  Also clears the frame hand-over counters.
+__global__ void k_worker_init(const WorkJob *__restrict__ wjobs, int nwjobs, uint64_t f0,
+                              unsigned long long *lvl_done, unsigned long long *dev_tick, int ninst)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nwjobs) {
+        const uint64_t k = 1ull << wjobs[q].kshift;
+        lvl_done[q] = 2 * ((f0 + k - 1) / k);
+    }
+    if (q < ninst) dev_tick[q] = 0;
+}
+
+// After a persistent run that served ticks up to fend - 1: job q (one
+// workgroup) completes every block with start tick < fend that the run did not
+// finish (a stopped run, or blocks whose turn came after the last tick) -- its
+// delay-line sum first if lvl_done is odd -- so that what follows (another
+// run, or per-tick launches) finds the state of a fully served history.
+template <int LOGNL, int NT>
+__device__ __forceinline__ void worker_fix(const WorkJob &W, unsigned long long *done, uint64_t fend, int N0, cf *lds,
+                           const cf *__restrict__ tw, int tid)
+{
+    constexpr int N = 1 << LOGNL;
+    const uint64_t k = 1ull << W.kshift;
+    const uint64_t m_end = (fend + k - 1) / k;
+    const unsigned long long d = *done;
+    uint64_t next = (d + 1) / 2;
+    if (d & 1) {
+        wg_partial<NT>(W.J, N, next, tid);
+        __syncthreads();
+    }
+    for (uint64_t m = next; m < m_end; m++) {
+        level_block<LOGNL, NT>(W.J, m, N0, lds, lds + N, tw, tid);
+        __syncthreads();
+        wg_partial<NT>(W.J, N, m + 1, tid);
+        __syncthreads();
+    }
+    if (tid == 0 && (next < m_end || (d & 1))) *done = 2 * m_end;
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(TickCfg<LOGN>::NT)
+k_worker_fix(const WorkJob *__restrict__ wjobs, unsigned long long *lvl_done, uint64_t fend, int N0,
+             const cf *__restrict__ tw)
+{
+    constexpr int NT = TickCfg<LOGN>::NT;
+    extern __shared__ cf pc_lds[];
+    const WorkJob &W = wjobs[blockIdx.x];
+    unsigned long long *done = lvl_done + blockIdx.x;
+    const int d = W.logN - LOGN;
+    if constexpr (LOGN + 1 <= 11) {
+        if (d == 1) worker_fix<LOGN + 1, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
+    }
+    if constexpr (LOGN + 2 <= 11) {
+        if (d == 2) worker_fix<LOGN + 2, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
+    }
+    if constexpr (LOGN + 3 <= 11) {
+        if (d == 3) worker_fix<LOGN + 3, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
+    }
+}
+
+// Wait for the host's release counter (pinned host memory, read across the
+// bus: about 1.2 us per load on MI355X), one load at a time (more loads in
+// flight per workgroup made the whole round trip slower).  Returns the value
+// that ended the wait (>= target, or with the stop bit), 0 on timeout.
+__device__ __forceinline__ unsigned long long poll_host(const unsigned long long *p, unsigned long long target,
+                                                        unsigned long long t_end)
+{
+    for (;;) {
+        const unsigned long long g = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if ((g & PC_GO_STOP) || g >= target) return g;
+        if (__builtin_amdgcn_s_memrealtime() > t_end) return 0;
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -1036,19 +1138,21 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             // with many workgroups only workgroup 0 polls host memory over the bus;
             // it relays the counter through device memory to the others
             const bool host_poll = (P.dev_go == nullptr) || blockIdx.x == 0;
-            for (;;) {
-                const unsigned long long g =
-                    host_poll ? __hip_atomic_load(P.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
-                              : __hip_atomic_load(P.dev_go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (host_poll && P.dev_go && (g >= f + 1 || (g & PC_GO_STOP)))
-                    __hip_atomic_store(P.dev_go, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (g & PC_GO_STOP) break;
-                if (g >= f + 1) {
-                    go = 1;
-                    break;
+            if (host_poll) {
+                const unsigned long long g = poll_host(P.go, f + 1, t_end);
+                if (P.dev_go && g) __hip_atomic_store(P.dev_go, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                go = (g != 0 && !(g & PC_GO_STOP));
+            } else {
+                for (;;) {
+                    const unsigned long long g = __hip_atomic_load(P.dev_go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (g & PC_GO_STOP) break;
+                    if (g >= f + 1) {
+                        go = 1;
+                        break;
+                    }
+                    if (__builtin_amdgcn_s_memrealtime() > t_end) break;
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                if (__builtin_amdgcn_s_memrealtime() > t_end) break;
-                __builtin_amdgcn_s_sleep(1);
             }
             const uint64_t fr = P.A.relaxed ? f : f + 1;
             // blocks of worker levels first read at this tick
@@ -1089,10 +1193,16 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
         }
         if (P.self_levels) {
+            // block m of a level with period k starts at tick m k and runs at tick
+            // m k + d_k, d_k = k/2 - 1 (0, 1, 3 for k = 2, 4, 8): at most one
+            // block per tick, each done before its first reader (m k + k - 1)
             for (int L = 0; L < sTJ.nlev; L++) {
                 const int q = sTJ.wslot[L];
-                if (q < 0 || (f & ((1ull << sTJ.kshift[L]) - 1)) != 0) continue;
-                self_level<LOGN>(P, q, f >> sTJ.kshift[L], pc_lds, tw, tid);
+                if (q < 0) continue;
+                const int ks = sTJ.kshift[L];
+                const uint64_t d = (1ull << (ks - 1)) - 1;
+                if (f < P.f0 + d || ((f - d) & ((1ull << ks) - 1)) != 0) continue;
+                self_level<LOGN>(P, q, (f - d) >> ks, pc_lds, tw, tid);
             }
         }
         if (P.nworkers > 0 && (f & P.pub_mask) == 0) {
