@@ -238,6 +238,7 @@ struct Engine {
     int nwjobs = 0, lds_w = 0;        // worker-level jobs, largest worker block size
     int nworkers = -1;                // worker workgroups (-1: not planned yet)
     bool self_levels = false;         // no room for workers: tick workgroups run those levels
+    long long persist_budget = 0;     // workgroups a persistent kernel may take (half the device)
     int2 *d_wrange = nullptr;
     unsigned long long *d_lvl_done = nullptr, *d_dev_tick = nullptr;
     bool workers_active = false;      // a persistent run with workers owns those levels
@@ -1108,7 +1109,10 @@ static int plan_workers(Engine &E)
     HIPCHK(persist_occupancy(E.N0, &nb, (size_t)2 * std::max(E.N0, E.lds_w) * sizeof(cf)));
     const long long budget = (long long)prop.multiProcessorCount * nb / 2;
     const long long ntick = (long long)E.groups[0].members.size();
-    if (ntick + E.nwjobs > budget) {
+    E.persist_budget = budget;
+    if (E.nwjobs == 0) {
+        // nothing to plan
+    } else if (ntick + E.nwjobs > budget) {
         E.self_levels = true;
     } else {
         std::vector<int2> ranges;
@@ -1169,6 +1173,13 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     const long long in_stride = (long long)E.N0 * E.nIn, out_stride = (long long)E.N0 * E.nOut;
     *persistent_ticks = 0;
     if (E.mix == 0) return 1;  // separate mix kernel per tick: per-tick launches
+    {
+        int rc = plan_workers(E);
+        if (rc) return rc;
+    }
+    // the tick workgroups alone must leave room for the level kernels launched
+    // beside the persistent kernel: else per-tick launches
+    if ((long long)E.groups[0].members.size() > E.persist_budget) return 1;
     PersistArgs P{};
     TickArgs &a = P.A;
     Group &Gr = E.groups[0];
@@ -1230,10 +1241,6 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         if (E.lvl_pending[g]) HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
     store_go(E, (unsigned long long)f0);
     hipError_t le = hipErrorInvalidValue;
-    if (E.nwjobs > 0 && E.persist_workers) {
-        int rc = plan_workers(E);
-        if (rc) return rc;
-    }
     if (E.nwjobs > 0 && E.persist_workers && (E.nworkers > 0 || E.self_levels)) {
         // worker-level progress counters for a history served up to tick f0 - 1
         const int n = std::max(E.nwjobs, (int)E.inst.size());
