@@ -46,19 +46,25 @@ def build_engine(mode, irs):
 
 
 def timed_run(m, xd, yd, warmup, steps, torch, persistent=False):
-    """warmup + timed steps of TICKS_PER_STEP ticks, every tick waited for;
-    returns (seconds, stats)."""
+    """warmup + timed steps of TICKS_PER_STEP ticks, every tick released and
+    waited for individually; returns (seconds, stats).  With the persistent
+    kernel the warmup and the timed steps are one launch each (a real-time
+    stream keeps its kernel running; relaunching it every step would add a
+    launch, about 0.1-0.15 ms, per 256 ticks); the per-tick launch engines
+    make one call per step."""
     stride = TICKS_PER_STEP * N0 * CHANNELS
-    for w in range(warmup):
-        if m.run_device(yd.data_ptr(), xd.data_ptr() + 4 * w * stride, TICKS_PER_STEP, True,
-                        persistent=persistent) != 0:
+    calls = [(0, warmup)] if persistent else [(w, 1) for w in range(warmup)]
+    for first, n in calls:
+        if n and m.run_device(yd.data_ptr() + 4 * first * stride, xd.data_ptr() + 4 * first * stride,
+                              n * TICKS_PER_STEP, True, persistent=persistent) != 0:
             raise RuntimeError("warmup failed")
     m.clear_stats()
     torch.cuda.synchronize()
+    calls = [(warmup, steps)] if persistent else [(warmup + s, 1) for s in range(steps)]
     t0 = time.perf_counter()
-    for s in range(steps):
-        off = (warmup + s) * stride
-        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, TICKS_PER_STEP, True,
+    for first, n in calls:
+        off = first * stride
+        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, n * TICKS_PER_STEP, True,
                         persistent=persistent) != 0:
             raise RuntimeError("step failed")
     torch.cuda.synchronize()
@@ -66,21 +72,42 @@ def timed_run(m, xd, yd, warmup, steps, torch, persistent=False):
 
 
 def cpu_baseline(irs, x_host, budget_s):
-    """The fp32 oracle (CPU restatement of the reference algorithm, one thread)
-    on the first ticks of the same workload; ticks chosen for ~budget_s."""
+    """The fp32 oracle (CPU restatement of the reference algorithm) on the first
+    ticks of the same workload, one instance per channel, the channels spread
+    over the host's cores (threads; the oracle runs outside the GIL) as the
+    reference spreads its work over pthreads; ticks chosen for ~budget_s."""
+    import threading
     from oracle import oracle as O
-    insts = [dict(N=PART_N, P=PART_P, in_ch=c, out_ch=c) for c in range(CHANNELS)]
-    m = O.OracleMulti(FS, N0, CHANNELS, CHANNELS, insts, irs, prec="f32")
-    _, t_cal = m.process_frames(x_host, TICKS_PER_STEP)
-    m.reset()
+    cores = max(1, min(CHANNELS, len(os.sched_getaffinity(0))))
+    # one mono engine per channel (same partition, same IR, same input column)
+    ms = [O.OracleMulti(FS, N0, 1, 1, [dict(N=PART_N, P=PART_P, in_ch=0, out_ch=0)], [irs[c]], prec="f32")
+          for c in range(CHANNELS)]
+    cols = [np.ascontiguousarray(x_host[:, c:c + 1]) for c in range(CHANNELS)]
+
+    def run(nticks):
+        def work(w):
+            for c in range(w, CHANNELS, cores):
+                ms[c].process_frames(cols[c], nticks)
+        th = [threading.Thread(target=work, args=(w,)) for w in range(cores)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0
+
+    t_cal = run(TICKS_PER_STEP)
+    for m in ms:
+        m.reset()
     steps = max(1, int(budget_s / max(t_cal, 1e-6)))
     steps = min(steps, x_host.shape[0] // (TICKS_PER_STEP * N0))
     ticks = steps * TICKS_PER_STEP
-    _, t = m.process_frames(x_host, ticks)
-    m.close()
-    return dict(value=CHANNELS * ticks * N0 / t, unit="channel-samples/s", cores=1, kind="port",
+    t = run(ticks)
+    for m in ms:
+        m.close()
+    return dict(value=CHANNELS * ticks * N0 / t, unit="channel-samples/s", cores=cores, kind="port",
                 sample=f"{ticks} ticks x {CHANNELS} channels ({ticks * N0 / FS:.2f} s of audio) of the same "
-                       f"workload through the fp32 CPU restatement, single thread")
+                       f"workload through the fp32 CPU restatement, one instance per channel on {cores} threads")
 
 
 def bench_uneven(args):
@@ -111,16 +138,17 @@ def bench_uneven(args):
     yd = torch.zeros_like(xd)
     torch.cuda.synchronize()
     stride = tps * n0 * n
-    for w in range(args.warmup):
-        if m.run_device(yd.data_ptr(), xd.data_ptr() + 4 * w * stride, tps, True, persistent=True) != 0:
-            raise RuntimeError(last_error())
+    # warmup and timed steps: one persistent launch each, every tick released
+    # and waited for individually
+    if args.warmup and m.run_device(yd.data_ptr(), xd.data_ptr(), args.warmup * tps, True, persistent=True) != 0:
+        raise RuntimeError(last_error())
     m.clear_stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s_ in range(args.steps):
-        off = (args.warmup + s_) * stride
-        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, tps, True, persistent=True) != 0:
-            raise RuntimeError(last_error())
+    off = args.warmup * stride
+    if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, args.steps * tps, True,
+                    persistent=True) != 0:
+        raise RuntimeError(last_error())
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     st = m.stats()

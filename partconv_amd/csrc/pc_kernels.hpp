@@ -29,6 +29,31 @@
 
 #define PC_MAX_LEVELS 16
 
+// Global-memory view of a pointer.  Job descriptors are read from LDS, where
+// the compiler can no longer see that their pointers address global memory;
+// without the cast it emits flat loads / stores, which also count against the
+// LDS wait counter and so hold up the LDS work that follows them.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// float2 loads / stores through a global-memory pointer (the HIP float2 class
+// has no address-space-qualified operators; the built-in vector type does).
+typedef float pc_v2f __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 gload2(const float2 *p)
+{
+    const pc_v2f v = *(const __attribute__((address_space(1))) pc_v2f *)p;
+    return make_float2(v.x, v.y);
+}
+
+__device__ __forceinline__ void gstore2(float2 *p, float2 v)
+{
+    *(__attribute__((address_space(1))) pc_v2f *)p = pc_v2f{v.x, v.y};
+}
+
 enum { PC_MODE_RUN = 0, PC_MODE_SPECTRUM = 1 };
 
 struct LevelJob {
@@ -347,7 +372,7 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
         const int j = tid + q * NT;
         if (j < N) {
             xv[q] = inf[inch + (long long)j * A.nIn];
-            y0p[q] = (A.relaxed && f > 0) ? J.ybuf[(size_t)((f - 1) & 1) * N + j] : 0.f;
+            y0p[q] = (A.relaxed && f > 0) ? gmem(J.ybuf)[(size_t)((f - 1) & 1) * N + j] : 0.f;
             lev[q] = 0.f;
         }
     }
@@ -360,10 +385,10 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
         for (int c = 0; c < CH; c++) {
             const int L = L0 + c;
             const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
-            const float *src = nullptr;
+            __attribute__((address_space(1))) const float *src = nullptr;
             if (qb != 0) {
                 const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
-                src = TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] + sub * N;
+                src = gmem(TJ.ybuf[L]) + (size_t)((qb - 1) & 1) * TJ.NL[L] + sub * N;
             }
 #pragma unroll
             for (int q = 0; q < IPT; q++) {
@@ -376,17 +401,13 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
 #pragma unroll
             for (int q = 0; q < IPT; q++) lev[q] += v[c][q];
     }
-    // ---- scale (src/partconvMulti.cpp:839-847), ring write, window in LDS
-    float *ring = const_cast<float *>(J.ring);
+    // ---- scale (src/partconvMulti.cpp:839-847), window in LDS
     float *wf = reinterpret_cast<float *>(bufA);
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const int j = tid + q * NT;
         if (j < N) {
             const float v = xv[q] * scale;
-            const long long p = (t0 + j) & mask;
-            ring[p] = v;
-            ring[p + R] = v;
             wf[j] = st.pv[q];
             wf[N + j] = v;
             st.pv[q] = v;
@@ -410,12 +431,18 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
     __syncthreads();
     cf *z = small_inverse<LOGN, NT>(X, other, tw, tid);
     const float *zf = reinterpret_cast<const float *>(z);
+    // ring write of the scaled frame (st.pv), issued after the transforms so the
+    // stores do not hold up their barriers
+    auto ring = gmem(const_cast<float *>(J.ring));
 #pragma unroll
     for (int q = 0; q < IPT; q++) {
         const int j = tid + q * NT;
         if (j < N) {
+            const long long p = (t0 + j) & mask;
+            ring[p] = st.pv[q];
+            ring[p + R] = st.pv[q];
             if (A.relaxed) {
-                J.ybuf[(size_t)(f & 1) * N + j] = zf[j];   // read back at tick f+1
+                gmem(J.ybuf)[(size_t)(f & 1) * N + j] = zf[j];   // read back at tick f+1
                 acc[q] = y0p[q] + lev[q];
             } else {
                 acc[q] = zf[j] + lev[q];
@@ -610,16 +637,54 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
 
 // ---------------------------------------------------------------------------
 // buffer_size 64: level 0 of a tick as one wavefront with the 64-point complex
-// transform held in registers (one value per lane, radix-2 stages exchanged
-// with lane shuffles; no LDS passes or barriers).  The forward transform is
-// decimation in frequency (natural input, bit-reversed output: lane l holds
-// bin k = brev6(l)); the inverse is decimation in time (bit-reversed input,
-// natural output).  Every lane keeps its stage twiddles, its split twiddle and
-// its bin's H_0 / S in registers.
+// transform held in registers (one value per lane; the radix-2 stages exchange
+// values with DPP / permlane moves in the VALU, no LDS passes or barriers).
+// The forward transform is decimation in frequency (natural input, bit-reversed
+// output: lane l holds bin k = brev6(l)); the inverse is decimation in time
+// (bit-reversed input, natural output).  Every lane keeps its stage twiddles,
+// its split twiddle and its bin's H_0 / S in registers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int brev6(int x) { return (int)(__builtin_bitreverse32((unsigned)x) >> 26); }
-__device__ __forceinline__ cf shfl_xor_cf(cf v, int m) { return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m)); }
 __device__ __forceinline__ cf shfl_cf(cf v, int src) { return make_float2(__shfl(v.x, src), __shfl(v.y, src)); }
+
+// Lane exchange v <- v[lane ^ D] in the VALU, without the LDS crossbar:
+// D = 1, 2: DPP quad permutations; D = 4: quad xor 3 then row half-mirror
+// (xor 7); D = 8: row half-mirror then row mirror (xor 15); D = 16, 32: the
+// gfx950 row / half-wave swaps (v_permlane16_swap, v_permlane32_swap).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+template <int D>
+__device__ __forceinline__ float xor_lane(float v, int lane)
+{
+    if constexpr (D == 1) {
+        return dpp_mov<0xB1>(v);                 // quad_perm [1,0,3,2]
+    } else if constexpr (D == 2) {
+        return dpp_mov<0x4E>(v);                 // quad_perm [2,3,0,1]
+    } else if constexpr (D == 4) {
+        return dpp_mov<0x141>(dpp_mov<0x1B>(v)); // quad_perm [3,2,1,0], row_half_mirror
+    } else if constexpr (D == 8) {
+        return dpp_mov<0x140>(dpp_mov<0x141>(v)); // row_half_mirror, row_mirror
+    } else if constexpr (D == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, v),
+                                                        __builtin_bit_cast(unsigned, v), false, false);
+        return __builtin_bit_cast(float, (lane & 16) ? p[0] : p[1]);
+    } else {
+        static_assert(D == 32, "lane exchange distance");
+        const auto p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, v),
+                                                        __builtin_bit_cast(unsigned, v), false, false);
+        return __builtin_bit_cast(float, (lane & 32) ? p[0] : p[1]);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ cf xor_cf(cf v, int lane)
+{
+    return make_float2(xor_lane<D>(v.x, lane), xor_lane<D>(v.y, lane));
+}
 
 struct W64 {
     cf tw[6];       // stage d = 32,16,..,1: W_{2d}^{lane & (d-1)}
@@ -637,27 +702,40 @@ __device__ __forceinline__ void w64_setup(W64 &w, const cf *__restrict__ tw, int
     w.ws = tw[128 + w.k];
 }
 
+// One radix-2 stage at distance D (stage index S into the lane's twiddles).
+template <int D, int S>
+__device__ __forceinline__ cf w64_fwd_stage(cf v, const W64 &w, int lane)
+{
+    const cf o = xor_cf<D>(v, lane);
+    return (lane & D) ? cmul(csub(o, v), w.tw[S]) : cadd(v, o);
+}
+
+template <int D, int S>
+__device__ __forceinline__ cf w64_inv_stage(cf v, const W64 &w, int lane)
+{
+    const cf t = (lane & D) ? cmulc(v, w.tw[S]) : v;
+    const cf o = xor_cf<D>(t, lane);
+    return (lane & D) ? csub(o, t) : cadd(t, o);
+}
+
 __device__ __forceinline__ cf w64_fwd(cf v, const W64 &w, int lane)
 {
-    int s = 0;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1, s++) {
-        const cf o = shfl_xor_cf(v, d);
-        v = (lane & d) ? cmul(csub(o, v), w.tw[s]) : cadd(v, o);
-    }
-    return v;
+    v = w64_fwd_stage<32, 0>(v, w, lane);
+    v = w64_fwd_stage<16, 1>(v, w, lane);
+    v = w64_fwd_stage<8, 2>(v, w, lane);
+    v = w64_fwd_stage<4, 3>(v, w, lane);
+    v = w64_fwd_stage<2, 4>(v, w, lane);
+    return w64_fwd_stage<1, 5>(v, w, lane);
 }
 
 __device__ __forceinline__ cf w64_inv(cf v, const W64 &w, int lane)
 {
-    int s = 5;
-#pragma unroll
-    for (int d = 1; d <= 32; d <<= 1, s--) {
-        const cf t = (lane & d) ? cmulc(v, w.tw[s]) : v;
-        const cf o = shfl_xor_cf(t, d);
-        v = (lane & d) ? csub(o, t) : cadd(t, o);
-    }
-    return v;
+    v = w64_inv_stage<1, 5>(v, w, lane);
+    v = w64_inv_stage<2, 4>(v, w, lane);
+    v = w64_inv_stage<4, 3>(v, w, lane);
+    v = w64_inv_stage<8, 2>(v, w, lane);
+    v = w64_inv_stage<16, 1>(v, w, lane);
+    return w64_inv_stage<32, 0>(v, w, lane);
 }
 
 struct TickState64 {
@@ -694,7 +772,7 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
         xv.x = inf[TJ.in_ch + (long long)(2 * lo) * A.nIn];
         xv.y = inf[TJ.in_ch + (long long)(2 * lo + 1) * A.nIn];
     } else {
-        if (A.relaxed && f > 0) y0p = reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64)[lo];
+        if (A.relaxed && f > 0) y0p = gload2(reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64) + lo);
         // level blocks four at a time: loads in flight together, summed in level order
         for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
             float2 v[4];
@@ -705,8 +783,8 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
                 v[c] = make_float2(0.f, 0.f);
                 if (qb != 0) {
                     const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
-                    v[c] = reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] +
-                                                            sub * 64)[lo];
+                    v[c] = gload2(reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] +
+                                                                   sub * 64) + lo);
                 }
             }
 #pragma unroll
@@ -716,23 +794,13 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
             }
         }
     }
-    // scale (src/partconvMulti.cpp:839-847), ring write; window pair of this lane
+    // scale (src/partconvMulti.cpp:839-847); window pair of this lane
     cf z;
-    if (lane >= 32) {
-        const float a = xv.x * TJ.scale, b = xv.y * TJ.scale;
-        float *ring = const_cast<float *>(J.ring);
-        const long long p = (t0 + 2 * lo) & mask;
-        ring[p] = a;
-        ring[p + 1] = b;
-        ring[p + R] = a;
-        ring[p + R + 1] = b;
-        z = make_float2(a, b);
-    } else {
-        z = make_float2(st.pv.x, st.pv.y);
-    }
+    if (lane >= 32) z = make_float2(xv.x * TJ.scale, xv.y * TJ.scale);
+    else z = make_float2(st.pv.x, st.pv.y);
     // the new frame is the next tick's previous frame
     {
-        const cf nz = shfl_cf(z, lo + 32);
+        const cf nz = xor_cf<32>(z, lane);
         if (lane < 32) st.pv = make_float2(nz.x, nz.y);
     }
     cf Z = w64_fwd(z, st.w, lane);
@@ -757,10 +825,17 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
         rsplit_inv(Y, Yp, st.w.ws, &C, &cnk);
     }
     const cf zo = w64_inv(C, st.w, lane);   // lane n: z'[n] = (y[2n], y[2n+1])
+    if (lane >= 32) {
+        // ring write of the scaled frame, after the transforms
+        float *ring = const_cast<float *>(J.ring);
+        const long long p = (t0 + 2 * lo) & mask;
+        gstore2(reinterpret_cast<float2 *>(ring + p), make_float2(z.x, z.y));
+        gstore2(reinterpret_cast<float2 *>(ring + p + R), make_float2(z.x, z.y));
+    }
     if (lane < 32) {
         float2 o;
         if (A.relaxed) {
-            reinterpret_cast<float2 *>(J.ybuf + (size_t)(f & 1) * 64)[lo] = make_float2(zo.x, zo.y);
+            gstore2(reinterpret_cast<float2 *>(J.ybuf + (size_t)(f & 1) * 64) + lo, make_float2(zo.x, zo.y));
             o = make_float2(y0p.x + lev.x, y0p.y + lev.y);
         } else {
             o = make_float2(zo.x + lev.x, zo.y + lev.y);
