@@ -189,14 +189,14 @@ int pcm_trace(pc_multi *h, long long nticks);
 long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
 
 /* Level workers of persistent runs (tuning switch, no reference counterpart):
- * on (default), the blocks of levels with period <= 8 ticks and N <= 2048 run
- * inside the persistent kernel -- in worker workgroups of their own (one per
- * instance and level) when those fit in half of the device, else in each
- * instance's tick workgroup after its tick ("self mode"); off, every level
- * block is a host launch on the level's stream.  Output is identical in all
- * cases.  pcm_workers_info returns the number of worker workgroups and sets
- * *self_mode (may be null). */
-int pcm_set_persistent_workers(pc_multi *h, int on);
+ * mode 1 (default): the blocks of levels with period <= 8 ticks and N <= 2048
+ * run inside the persistent kernel -- in worker workgroups of their own (one
+ * per instance and level) when those fit in half of the device, else in each
+ * instance's tick workgroup after its tick ("self mode"); mode 2: self mode
+ * always; mode 0: every level block is a host launch on the level's stream.
+ * Output is identical in all modes.  pcm_workers_info returns the number of
+ * worker workgroups and sets *self_mode (may be null). */
+int pcm_set_persistent_workers(pc_multi *h, int mode);
 int pcm_workers_info(pc_multi *h, int *self_mode);
 
 /* ----------------------------------------------------------------------------

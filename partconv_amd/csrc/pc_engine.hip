@@ -242,7 +242,7 @@ struct Engine {
     int2 *d_wrange = nullptr;
     unsigned long long *d_lvl_done = nullptr, *d_dev_tick = nullptr;
     bool workers_active = false;      // a persistent run with workers owns those levels
-    int persist_workers = 1;          // 0: persistent runs launch every level from the host
+    int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
     // stats
     std::vector<double> lat;
     size_t lat_pos = 0;
@@ -1112,7 +1112,7 @@ static int plan_workers(Engine &E)
     E.persist_budget = budget;
     if (E.nwjobs == 0) {
         // nothing to plan
-    } else if (ntick + E.nwjobs > budget) {
+    } else if (ntick + E.nwjobs > budget || E.persist_workers == 2) {
         E.self_levels = true;
     } else {
         std::vector<int2> ranges;
@@ -1678,10 +1678,14 @@ PC_API int pcm_trace(pc_multi *h, long long nticks)
     return PC_OK;
 }
 
-PC_API int pcm_set_persistent_workers(pc_multi *h, int on)
+PC_API int pcm_set_persistent_workers(pc_multi *h, int mode)
 {
     if (!h) return fail(PC_ERR_ARG, "null handle");
-    h->E.persist_workers = on ? 1 : 0;
+    if (mode < 0 || mode > 2) return fail(PC_ERR_ARG, "workers mode must be 0, 1 or 2");
+    h->E.persist_workers = mode;
+    h->E.nworkers = -1;   // plan again
+    hipFree(h->E.d_wrange);
+    h->E.d_wrange = nullptr;
     return PC_OK;
 }
 

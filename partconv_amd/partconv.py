@@ -344,9 +344,11 @@ class PartConvMulti:
         """Record GPU timestamps of the next persistent run (see pcm_trace)."""
         return _report(lib().pcm_trace(self._h, int(nticks)), "trace")
 
-    def set_persistent_workers(self, on):
-        """Level workers inside the persistent kernel on/off (identical output)."""
-        return _report(lib().pcm_set_persistent_workers(self._h, 1 if on else 0), "set_persistent_workers")
+    def set_persistent_workers(self, mode):
+        """Short-period levels in persistent runs: True/1 inside the kernel (worker
+        workgroups, or self mode when they do not fit), 2 always self mode,
+        False/0 host launches.  Output is identical."""
+        return _report(lib().pcm_set_persistent_workers(self._h, int(mode)), "set_persistent_workers")
 
     @property
     def workers(self):

@@ -298,6 +298,60 @@ def test_persistent_level_workers_match_host_launched_levels(relaxed):
         assert rel(outs[1][:, i], r) < TOL, i
 
 
+WORKER_CASES = [
+    (16, [16, 64, 256, 1024], [6, 6, 6, 0]),
+    (32, [32, 64, 256, 2048], [2, 6, 14, 0]),
+    (128, [128, 512, 1024, 4096], [6, 2, 6, 0]),
+    (256, [256, 512, 2048, 8192], [2, 6, 6, 0]),
+    (512, [512, 1024, 4096], [2, 6, 0]),
+    (1024, [1024, 2048, 8192], [2, 6, 0]),
+]
+
+
+@pytest.mark.parametrize("N0,N,P", WORKER_CASES)
+@pytest.mark.parametrize("relaxed", [False, True])
+def test_worker_modes_every_buffer_size(N0, N, P, relaxed):
+    """Short-period levels (k = 2, 4, 8) at every buffer size, with partition
+    counts above 2: worker workgroups and self mode against host-launched
+    levels and the per-tick launch path, bitwise; one channel against fp64.
+    Runs are split at ticks that are not multiples of the level periods."""
+    say(f"case worker modes N0={N0} N={N} relaxed={relaxed}")
+    import torch
+    taps = 3 * N[-1] + sum(n * p for n, p in zip(N[:-1], P[:-1])) - N0 // 2
+    insts = [dict(N=N, P=P, in_ch=i, out_ch=i) for i in range(3)]
+    irs = [decaying_noise_ir(taps, 1.0, 48000, 300 + i) for i in range(3)]
+    nfr = 3 * (N[-1] // N0) + 7
+    rng = np.random.default_rng(N0)
+    x = rng.standard_normal((nfr * N0, 3)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    cuts = sorted({0, 1, nfr // 3 + 1, nfr // 2 + 3, nfr})
+    outs = []
+    for variant in ("launch", 1, 2, 0):
+        m = make_multi(insts, irs, N0, 3, 3, relaxed=relaxed)
+        if variant != "launch":
+            m.set_persistent_workers(variant)
+            if variant == 2:
+                assert m.self_mode and m.workers == 0
+            if variant == 1:
+                assert not m.self_mode and m.workers > 0
+        yd = torch.zeros_like(xd)
+        torch.cuda.synchronize()
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            off = a * N0 * 3 * 4
+            assert m.run_device(yd.data_ptr() + off, xd.data_ptr() + off, b - a, True,
+                                persistent=(variant != "launch")) == 0, last_error()
+        if variant != "launch":
+            assert m.stats().persistent_ticks == nfr
+        outs.append(yd.cpu().numpy())
+        m.cleanup()
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+    r = O.direct_conv_f64(x[:, 1], irs[1])
+    if relaxed:
+        r = np.concatenate([np.zeros(N0), r[:-N0]])
+    assert rel(outs[0][:, 1], r) < TOL
+
+
 def test_many_instances_run_short_levels_in_tick_workgroups():
     """Many instances: one worker workgroup per job would not fit in half of
     the device, so each tick workgroup runs its own instance's short-period
