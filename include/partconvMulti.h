@@ -1,7 +1,8 @@
 /*
  * partconvMulti.h -- C++ drop-in for the reference's PartConvMulti,
- * PartConvMultiRelaxed (src/partconvMulti.h:52-148) and PartConvFilter
- * (src/partconv.h:40-114), header-only over the C ABI of partconv_amd.h.
+ * PartConvMultiRelaxed (src/partconvMulti.h:52-148), PartConvFilter
+ * (src/partconv.h:40-114) and PartConvMax (src/partconv.h:117-175),
+ * header-only over the C ABI of partconv_amd.h.
  *
  * Same class names, method names, argument meaning and public fields.  Where
  * the reference calls exit(-1) (bad config, unreadable IR, frameCount !=
@@ -153,6 +154,67 @@ public:
 
 private:
     pc_filter *h_;
+};
+
+/* PartConvMax (src/partconv.h:117-175): the in-memory front end.  Channel
+ * modes from (n, m, k), IRs from one array, one greedy partition over
+ * blockSizes, per-channel buffers (float** run). */
+class PartConvMax
+{
+public:
+    PartConvMax(int inVerbosity = 0) : verbosity(inVerbosity), h_(pcm_create(inVerbosity, 0)) {}
+    ~PartConvMax() { pcm_destroy(h_); }
+
+    int setup(int inSampleRate, int n, int m, int k, float *impulses, int filterLength, int stride, int *blockSizes,
+              int levels, int fftwOptLevel = 1, const char *wisdom = nullptr, int max_threads_per_level = 1,
+              int max_level0_threads = 0)
+    {
+        (void)fftwOptLevel;
+        (void)wisdom;
+        (void)max_threads_per_level;
+        (void)max_level0_threads;
+        int rc = pcm_setup_max(h_, inSampleRate, n, m, k, impulses, filterLength, stride, blockSizes, levels);
+        if (rc) {
+            fprintf(stderr, "%s\n", pc_last_error());
+            return 1;
+        }
+        pc_multi_info i;
+        if (pcm_get_info(h_, &i) == PC_OK) {
+            FS = i.FS;
+            buffer_size = i.buffer_size;
+            max_block_size = i.max_block_size;
+            numInputChannels = i.numInputChannels;
+            numOutputChannels = i.numOutputChannels;
+        }
+        return 0;
+    }
+    int cleanup() { return pcm_cleanup(h_); }
+    int run(float **const output, float **const input)
+    {
+        int rc = pcm_run_channels(h_, output, input);
+        if (rc) fprintf(stderr, "%s\n", pc_last_error());
+        frameNum++;
+        return rc;
+    }
+    void reset()
+    {
+        pcm_reset(h_);
+        frameNum = 0;
+    }
+    pc_multi *handle() { return h_; }
+
+    float **outbuffers = nullptr; /* kept for source compatibility */
+    float **inbuffers = nullptr;
+    int FS = 0;
+    int buffer_size = 0;
+    int max_block_size = 0;
+    int verbosity;
+    int numInputChannels = 0;
+    int numOutputChannels = 0;
+    unsigned long frameNum = 0;
+
+private:
+    pc_multi *h_;
 };
 
 #endif /* PARTCONV_AMD_PARTCONVMULTI_H */

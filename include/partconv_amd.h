@@ -124,6 +124,23 @@ int pcm_setup(pc_multi *h, const char *config_file, int max_threads_per_level,
 int pcm_setup_mem(pc_multi *h, int FS, int buffer_size, int nIn, int nOut, int numPCs,
                   const pc_instance *inst);
 
+/* PartConvMax::setup(inSampleRate, n, m, k, impulses, filterLength, stride,
+ * blockSizes, levels, ...) (src/partconv.cpp:653-847): the in-memory front end.
+ * Channel modes: n == m == k parallel filters with distinct IRs (instance i:
+ * input i -> output i); n == m, k == 1 the same IR on every channel; m == n*k
+ * filterbank (IR i on input j -> output i*n + j); otherwise PC_ERR_CONFIG.
+ * IR i, tap l is impulses[stride*(filterLength*i + l)].  One greedy partition
+ * over blockSizes[0..levels) (P_i = 2 N_{i+1}/N_i - 2 until the IR is
+ * covered, last count recounted; src/partconv.cpp:712-728) for every
+ * instance; buffer_size = blockSizes[0].  FFTW options and thread counts of
+ * the reference signature have no counterpart here. */
+int pcm_setup_max(pc_multi *h, int FS, int n, int m, int k, const float *impulses, int filterLength,
+                  int stride, const int *blockSizes, int levels);
+
+/* PartConvMax::run(float** output, float** input) (src/partconv.cpp:1219-1303):
+ * one tick on per-channel host buffers of buffer_size samples. */
+int pcm_run_channels(pc_multi *h, float *const *output, const float *const *input);
+
 /* PartConvMulti::run (src/partconvMulti.cpp:815-928) /
  * PartConvMultiRelaxed::run (src/partconvMulti.cpp:1017-1134): one tick.
  * input: frameCount x numInputChannels interleaved, output: frameCount x

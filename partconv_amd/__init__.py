@@ -3,8 +3,8 @@
 Python surface over libpartconv_amd.so (C ABI in include/partconv_amd.h).
 """
 from ._lib import last_error, lib  # noqa: F401
-from .partconv import (PartConvFilter, PartConvMulti, PartConvMultiRelaxed,  # noqa: F401
+from .partconv import (PartConvFilter, PartConvMax, PartConvMulti, PartConvMultiRelaxed,  # noqa: F401
                        doubleArray, greedy_partition, intArray)
 
-__all__ = ["PartConvFilter", "PartConvMulti", "PartConvMultiRelaxed", "intArray", "doubleArray",
+__all__ = ["PartConvFilter", "PartConvMax", "PartConvMulti", "PartConvMultiRelaxed", "intArray", "doubleArray",
            "greedy_partition", "last_error", "lib"]

@@ -66,6 +66,9 @@ SIGNATURES = {
     "pcm_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_trace": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcm_set_persistent_workers": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_setup_max": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+    "pcm_run_channels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "pcm_workers_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),

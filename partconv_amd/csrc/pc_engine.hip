@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1471,6 +1472,67 @@ PC_API int pcm_setup_mem(pc_multi *h, int FS, int buffer_size, int nIn, int nOut
         d.push_back(std::move(x));
     }
     return multi_setup(h, FS, buffer_size, nIn, nOut, d);
+}
+
+// PartConvMax::setup (src/partconv.cpp:653-847): channel mode from (n, m, k),
+// the IRs from one array (IR i, tap l at impulses[stride*(filterLength*i + l)]),
+// one greedy partition over blockSizes for every instance.
+PC_API int pcm_setup_max(pc_multi *h, int FS, int n, int m, int k, const float *impulses, int filterLength,
+                         int stride, const int *blockSizes, int levels)
+{
+    if (!h || !impulses || !blockSizes || levels < 1 || filterLength < 1 || stride < 1 || n < 1 || m < 1 || k < 1)
+        return fail(PC_ERR_ARG, "bad argument");
+    // (inst IR, input, output) per channel mode (src/partconv.cpp:659-708)
+    std::vector<std::array<int, 3>> map;
+    if (n == m && n == k) {
+        for (int i = 0; i < k; i++) map.push_back({i, i, i});          // parallel, distinct IRs
+    } else if (n == m && k == 1) {
+        for (int j = 0; j < n; j++) map.push_back({0, j, j});          // parallel, one IR
+    } else if (m == n * k) {
+        for (int i = 0; i < k; i++)                                     // filterbank
+            for (int j = 0; j < n; j++) map.push_back({i, j, i * n + j});
+    } else {
+        return fail(PC_ERR_CONFIG, "PartConvMax::setup() does not support requested channel configuration");
+    }
+    int Ng[PC_MAX_LEVELS], Pg[PC_MAX_LEVELS], M = 0;
+    if (levels > PC_MAX_LEVELS) return fail(PC_ERR_ARG, "too many levels");
+    int rc = pc_greedy_partition(blockSizes, levels, filterLength, Ng, Pg, &M);
+    if (rc) return rc;
+    std::vector<std::vector<float>> irs(k, std::vector<float>(filterLength));
+    for (int i = 0; i < k; i++)
+        for (int l = 0; l < filterLength; l++) irs[i][l] = impulses[(size_t)stride * ((size_t)filterLength * i + l)];
+    std::vector<InstDesc> d;
+    for (auto &e : map) {
+        InstDesc x;
+        x.in_ch = e[1];
+        x.out_ch = e[2];
+        x.N.assign(Ng, Ng + M);
+        x.P.assign(Pg, Pg + M);
+        x.ir = irs[e[0]];
+        d.push_back(std::move(x));
+    }
+    return multi_setup(h, FS, blockSizes[0], n, m, d);
+}
+
+// PartConvMax::run (src/partconv.cpp:1219-1303): one tick on per-channel
+// buffers of buffer_size samples (input[c], output[c]).
+PC_API int pcm_run_channels(pc_multi *h, float *const *output, const float *const *input)
+{
+    if (!h || !output || !input) return fail(PC_ERR_ARG, "null argument");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    const int N0 = E.N0;
+    for (int c = 0; c < E.nIn; c++) {
+        if (!input[c]) return fail(PC_ERR_ARG, "null input channel");
+        for (int j = 0; j < N0; j++) E.h_in[(size_t)j * E.nIn + c] = input[c][j];
+    }
+    int rc = engine_tick(E, E.d_in_map, 0, (long long)E.frame, E.d_out_map, 0, true);
+    if (rc) return rc;
+    for (int c = 0; c < E.nOut; c++) {
+        if (!output[c]) return fail(PC_ERR_ARG, "null output channel");
+        for (int j = 0; j < N0; j++) output[c][j] = E.h_out[(size_t)j * E.nOut + c];
+    }
+    return PC_OK;
 }
 
 PC_API int pcm_run(pc_multi *h, float *output, const float *input, int frameCount)

@@ -397,3 +397,97 @@ class PartConvMultiRelaxed(PartConvMulti):
     buffer, every level gets a full period."""
 
     _relaxed = 1
+
+
+class PartConvMax:
+    """PartConvMax (src/partconv.h, src/partconv.cpp:653-1313), the reference's
+    in-memory front end: channel modes chosen from (n, m, k), IRs from one
+    array, one greedy partition, per-channel buffers.  Runs on the batched GPU
+    engine (strict semantics, zero added latency)."""
+
+    def __init__(self, inVerbosity=0):
+        self._h = lib().pcm_create(int(inVerbosity), 0)
+        self.verbosity = int(inVerbosity)
+
+    def setup(self, inSampleRate, n, m, k, impulses, filterLength, stride, blockSizes, levels,
+              fftwOptLevel=1, wisdom=None, max_threads_per_level=1, max_level0_threads=0):
+        """PartConvMax::setup (src/partconv.cpp:653-847).  impulses: float32
+        array holding IR i, tap l at impulses[stride*(filterLength*i + l)];
+        blockSizes: the candidate block sizes (first = buffer size).  The FFTW
+        and thread arguments are accepted and have no effect.  Returns 0 / 1."""
+        imp = np.ascontiguousarray(impulses, dtype=np.float32).ravel()
+        need = stride * (filterLength * k - 1) + 1
+        if imp.size < need:
+            print(f"PartConvMax setup: impulses holds {imp.size} values, {need} needed", file=sys.stderr)
+            return 1
+        bs = (ctypes.c_int * int(levels))(*[int(v) for v in list(blockSizes)[:int(levels)]])
+        rc = lib().pcm_setup_max(self._h, int(inSampleRate), int(n), int(m), int(k),
+                                 ctypes.c_void_p(imp.ctypes.data), int(filterLength), int(stride), bs, int(levels))
+        return 1 if _report(rc, "PartConvMax setup") else 0
+
+    def run(self, output, input):
+        """PartConvMax::run (src/partconv.cpp:1219-1303): one tick.  output and
+        input are sequences of float32 arrays of buffer_size samples (one per
+        channel), or 2-D (channels, buffer_size) arrays."""
+        outs = [np.asarray(o) for o in output]
+        ins = [np.ascontiguousarray(x, dtype=np.float32) for x in input]
+        for o in outs:
+            if o.dtype != np.float32 or not o.flags.c_contiguous:
+                raise TypeError("output channels must be contiguous float32 arrays")
+        po = (ctypes.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+        pi = (ctypes.c_void_p * len(ins))(*[x.ctypes.data for x in ins])
+        return _report(lib().pcm_run_channels(self._h, po, pi), "PartConvMax run")
+
+    def process(self, x):
+        """Convenience: (n_samples, numInputChannels) host samples, tick by tick
+        through run(); returns (frames*buffer_size, numOutputChannels)."""
+        info = self.info()
+        N0, ni, no = info.buffer_size, info.numInputChannels, info.numOutputChannels
+        x = np.asarray(x, dtype=np.float32)
+        if x.ndim == 1:
+            x = x[:, None]
+        nfr = -(-x.shape[0] // N0)
+        xin = np.zeros((ni, nfr * N0), dtype=np.float32)
+        xin[:, :x.shape[0]] = x.T
+        y = np.zeros((no, nfr * N0), dtype=np.float32)
+        fo = np.zeros((no, N0), dtype=np.float32)
+        for t in range(nfr):
+            if self.run(fo, xin[:, t * N0:(t + 1) * N0]) != 0:
+                raise RuntimeError(last_error())
+            y[:, t * N0:(t + 1) * N0] = fo
+        return y.T.copy()
+
+    def reset(self):
+        lib().pcm_reset(self._h)
+
+    def cleanup(self):
+        return lib().pcm_cleanup(self._h)
+
+    def info(self):
+        i = PcMultiInfo()
+        if lib().pcm_get_info(self._h, ctypes.byref(i)) != 0:
+            raise RuntimeError(last_error())
+        return i
+
+    def instance_partition(self, i):
+        N = (ctypes.c_int * 16)()
+        P = (ctypes.c_int * 16)()
+        M = lib().pcm_instance_partition(self._h, int(i), N, P, 16)
+        return list(N)[:M], list(P)[:M]
+
+    # public fields (src/partconv.h:117-175)
+    FS = property(lambda self: self.info().FS)
+    buffer_size = property(lambda self: self.info().buffer_size)
+    max_block_size = property(lambda self: self.info().max_block_size)
+    numInputChannels = property(lambda self: self.info().numInputChannels)
+    numOutputChannels = property(lambda self: self.info().numOutputChannels)
+    num_levels = property(lambda self: self.info().num_levels)
+    frameNum = property(lambda self: self.info().frameNum)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pcm_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -427,6 +427,47 @@ def test_uneven_batched_instances_greedy_partitions():
     m.cleanup()
 
 
+@pytest.mark.parametrize("n,m,k", [(3, 3, 3), (3, 3, 1), (2, 6, 3)])
+def test_partconvmax_channel_modes(n, m, k):
+    """PartConvMax (src/partconv.cpp:653-847, 1219-1303): parallel distinct IRs,
+    one IR on every channel, filterbank; per-channel buffers; IRs read with a
+    stride from one array; every output channel against fp64."""
+    say(f"case PartConvMax n={n} m={m} k={k}")
+    from partconv_amd import PartConvMax
+    taps, stride, N0 = 9000, 2, 64
+    irs = [decaying_noise_ir(taps, 0.5, 48000, 400 + i) for i in range(k)]
+    imp = np.zeros(stride * taps * k, dtype=np.float32)
+    for i in range(k):
+        imp[stride * taps * i:stride * taps * (i + 1):stride] = irs[i]
+    pm = PartConvMax(0)
+    bs = [64, 256, 1024, 4096, 16384]
+    assert pm.setup(48000, n, m, k, imp, taps, stride, bs, len(bs), 1) == 0, last_error()
+    assert pm.buffer_size == N0 and pm.numInputChannels == n and pm.numOutputChannels == m
+    rng = np.random.default_rng(n * 10 + k)
+    nfr = 70
+    x = rng.standard_normal((nfr * N0, n)).astype(np.float32)
+    y = pm.process(x)
+    assert pm.frameNum == nfr
+    for o in range(m):
+        if n == m == k:
+            i_ir, i_in = o, o
+        elif k == 1:
+            i_ir, i_in = 0, o
+        else:
+            i_ir, i_in = o // n, o % n
+        r = O.direct_conv_f64(x[:, i_in], irs[i_ir])
+        assert rel(y[:, o], r) < TOL, o
+    pm.cleanup()
+
+
+def test_partconvmax_rejects_unsupported_channel_layout():
+    from partconv_amd import PartConvMax
+    pm = PartConvMax(0)
+    h = np.ones(1000, dtype=np.float32)
+    assert pm.setup(48000, 2, 3, 2, h, 500, 1, [64, 256, 1024], 3, 1) == 1
+    assert "channel configuration" in last_error()
+
+
 def test_mimo_as_pair_instances():
     """MIMO in the reference's own terms: one instance per (input i -> output o)
     pair, outputs summed (src/partconvMulti.cpp:911-921).  8x8 at reduced size;
