@@ -71,6 +71,12 @@ def timed_run(m, xd, yd, warmup, steps, torch, persistent=False):
     return time.perf_counter() - t0, m.stats()
 
 
+def _lib_process(m, y, x, nframes):
+    """pcm_process: nframes run() calls over interleaved host buffers."""
+    from partconv_amd._lib import lib
+    return lib().pcm_process(m._h, y.ctypes.data, x.ctypes.data, int(nframes))
+
+
 def cpu_baseline(irs, x_host, budget_s):
     """The fp32 oracle (CPU restatement of the reference algorithm) on the first
     ticks of the same workload, one instance per channel, the channels spread
@@ -447,6 +453,27 @@ def main():
         "tick_kernel": {"avg_us": tick_us, "alg_bytes_per_launch": tick_bytes,
                         "achieved_GBps": tick_bytes / (tick_us * 1e-6) / 1e9 if tick_us else None},
     }
+
+    # ---- host buffers (PCIe-inclusive: frames and output in pinned host memory),
+    # batched engine through run(): resident persistent session vs one launch per tick
+    hticks = 4 * TICKS_PER_STEP
+    xh = np.ascontiguousarray(x_host[:hticks * N0])
+    yh = np.zeros_like(xh)
+    host = {}
+    for name, sess in (("persistent_session", True), ("launch_per_tick", False)):
+        mh = build_engine("batched", irs)
+        mh.set_host_session(sess)
+        mh.process(xh[:TICKS_PER_STEP * N0])      # warmup
+        mh.clear_stats()
+        t0 = time.perf_counter()
+        if _lib_process(mh, yh, xh, hticks) != 0:
+            raise RuntimeError("host run failed")
+        hs = time.perf_counter() - t0
+        st_h = mh.stats()
+        host[name] = {"value": CHANNELS * hticks * N0 / hs, "p50_us": st_h.p50_us, "p99_us": st_h.p99_us,
+                      "misses": int(st_h.deadline_misses), "ticks": hticks}
+        mh.cleanup()
+    out["host_buffers_pcie_inclusive"] = host
 
     # ---- plain engine (same GPU, per-instance launches)
     if not args.no_plain:

@@ -144,8 +144,22 @@ int pcm_run_channels(pc_multi *h, float *const *output, const float *const *inpu
 /* PartConvMulti::run (src/partconvMulti.cpp:815-928) /
  * PartConvMultiRelaxed::run (src/partconvMulti.cpp:1017-1134): one tick.
  * input: frameCount x numInputChannels interleaved, output: frameCount x
- * numOutputChannels interleaved (host memory).  Blocks until output is ready. */
+ * numOutputChannels interleaved (host memory).  Blocks until output is ready.
+ * The batched engine serves these calls through a persistent tick kernel that
+ * stays resident between calls (frames and output through pinned host memory,
+ * the call only releases the tick); it is restarted after 1 s without a call,
+ * and stopped by run_device, reset, trace, cleanup. */
 int pcm_run(pc_multi *h, float *output, const float *input, int frameCount);
+
+/* nframes consecutive pcm_run calls over interleaved host buffers (frame t at
+ * input + t*buffer_size*nIn, output + t*buffer_size*nOut).  Convenience, no
+ * reference counterpart. */
+int pcm_process(pc_multi *h, float *output, const float *input, long long nframes);
+
+/* Host-buffer calls (pcm_run, pcm_run_channels) through a persistent session
+ * (1, default) or one launch per tick (0).  Identical output.  No reference
+ * counterpart. */
+int pcm_set_host_session(pc_multi *h, int on);
 
 /* Device-resident ticks: runs nframes consecutive ticks reading frame t from
  * d_in + t*buffer_size*nIn and writing d_out + t*buffer_size*nOut (device

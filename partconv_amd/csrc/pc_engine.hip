@@ -34,6 +34,8 @@
 #define PC_MIN_BLOCK 16
 #define PC_TW_MAX (2 * PC_MAX_BLOCK)
 #define PC_GO_WORD 1          // release counter: the word after the completion flag (same line)
+#define PC_SPIN_LIMIT 200000000ull  // persistent-kernel waits: 2 s at 100 MHz (s_memrealtime)
+#define PC_SESSION_IDLE_S 1.0       // host sessions restart after this long without a release
 #define PC_TICK_TIMEOUT_S 10.0
 
 static thread_local std::string g_err;
@@ -243,6 +245,12 @@ struct Engine {
     int2 *d_wrange = nullptr;
     unsigned long long *d_lvl_done = nullptr, *d_dev_tick = nullptr;
     bool workers_active = false;      // a persistent run with workers owns those levels
+    bool sess_active = false;         // a persistent kernel is serving ticks
+    uint64_t sess_end = 0;            // first tick the session's kernel does not serve
+    double sess_last = 0;             // host time of the session's last release
+    bool sess_served_late = false;    // persist_end accounted for a tick served while stopping
+    int host_session = 1;             // pcm_run through a persistent session (batched engine)
+    int sess_host = 0;                // the active session serves host buffers
     int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
     // stats
     std::vector<double> lat;
@@ -253,9 +261,11 @@ struct Engine {
 };
 
 static hipStream_t level_stream(Engine &E, int g) { return E.s_lvl[E.g_stream[g]]; }
+static int persist_end(Engine &E, bool stop);
 
 static void engine_free(Engine &E)
 {
+    if (E.sess_active) (void)persist_end(E, true);
     if (E.s_main) hipStreamSynchronize(E.s_main);
     for (hipStream_t s : E.s_lvl) hipStreamSynchronize(s);
     hipFree(E.d_wjobs);
@@ -791,6 +801,10 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
 static int engine_reset(Engine &E)
 {
     if (!E.ready) return fail(PC_ERR_STATE, "reset before setup");
+    if (E.sess_active) {
+        int rc = persist_end(E, true);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(E.s_main));
     for (hipStream_t s : E.s_lvl) HIPCHK(hipStreamSynchronize(s));
     for (Inst &I : E.inst) {
@@ -1167,12 +1181,21 @@ static int finish_worker_levels(Engine &E, uint64_t fend)
 // kernel, a kernel that exited) the host stops the kernel and finishes the
 // remaining ticks with per-tick launches, so output is never wrong or late
 // beyond the stall itself.
-static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, long long nframes,
-                                 long long *persistent_ticks)
+// ---- persistent sessions ----------------------------------------------------
+// persist_begin launches the persistent tick kernel for up to nticks ticks from
+// E.frame; persist_tick serves tick E.frame through it; persist_end finishes
+// the session (stopping the kernel if stop is set).  Per tick the host
+//   (1) checks, with hipEventQuery, that the host-launched level blocks due at
+//       this tick are complete,
+//   (2) releases the tick (go = f + 1 in pinned memory),
+//   (3) spins on the completion flag(s),
+//   (4) launches the host-launched level blocks that start at this tick.
+// If a level stream stalls beside the kernel or the kernel has ended, the tick
+// is reported unserved and the caller ends the session and falls back to
+// per-tick launches; output is never wrong or late beyond the stall itself.
+static int persist_begin(Engine &E, const float *d_in, long long in_stride, float *d_out, long long out_stride,
+                         long long nticks, int host_io)
 {
-    const uint64_t f0 = E.frame;
-    const long long in_stride = (long long)E.N0 * E.nIn, out_stride = (long long)E.N0 * E.nOut;
-    *persistent_ticks = 0;
     if (E.mix == 0) return 1;  // separate mix kernel per tick: per-tick launches
     {
         int rc = plan_workers(E);
@@ -1181,6 +1204,7 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     // the tick workgroups alone must leave room for the level kernels launched
     // beside the persistent kernel: else per-tick launches
     if ((long long)E.groups[0].members.size() > E.persist_budget) return 1;
+    const uint64_t f0 = E.frame;
     PersistArgs P{};
     TickArgs &a = P.A;
     Group &Gr = E.groups[0];
@@ -1207,7 +1231,9 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.nempty = E.nempty;
     a.done_flag = E.d_flag_map;
     a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
-    a.fence_sys = 0;
+    // frames in / output to pinned host memory: system-scope acquire after the
+    // release and system-scope release before the completion flag
+    a.fence_sys = host_io;
     // Level jobs launched from the host start only at ticks f = 0 mod k_min
     // (k_min over the host-launched levels); those ticks publish the ring with
     // a device-scope release, and the ticks that first read such a level's
@@ -1233,9 +1259,9 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     a.trace = E.d_trace;
     a.trace_cap = E.trace_cap;
     P.f0 = f0;
-    P.nticks = nframes;
+    P.nticks = nticks;
     P.go = E.d_flag_map + PC_GO_WORD;
-    P.spin_limit = 200000000ull;  // 2 s at 100 MHz
+    P.spin_limit = PC_SPIN_LIMIT;
     // the levels the first tick reads must be complete before the kernel owns the tick stream
     const int G = (int)E.gsize.size();
     for (int g = 1; g < G; g++)
@@ -1272,36 +1298,47 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         (void)hipGetLastError();
         return 1;  // caller falls back to per-tick launches
     }
-    long long t = 0;
-    bool stalled = false;
-    for (; t < nframes; t++) {
-        const uint64_t f = f0 + (uint64_t)t;
-        const double t0 = now_s();
-        for (int g = 1; g < G && !stalled; g++) {
-            if (!(level_due(E, g, f) && E.lvl_pending[g])) continue;
-            for (;;) {
-                hipError_t q = hipEventQuery(E.ev_lvl[g]);
-                if (q == hipSuccess) break;
-                if (q != hipErrorNotReady) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
-                if (now_s() - t0 > 0.2) {
-                    stalled = true;
-                    break;
-                }
-            }
-            if (!stalled) E.lvl_pending[g] = false;
+    E.sess_active = true;
+    E.sess_end = f0 + (uint64_t)nticks;
+    E.sess_last = now_s();
+    return PC_OK;
+}
+
+// Serve tick E.frame: 0 served, 1 not served (stalled: end the session and
+// fall back), other values are errors.
+static int persist_tick(Engine &E)
+{
+    const uint64_t f = E.frame;
+    if (!E.sess_active || f >= E.sess_end) return 1;
+    const int G = (int)E.gsize.size();
+    const double t0 = now_s();
+    for (int g = 1; g < G; g++) {
+        if (!(level_due(E, g, f) && E.lvl_pending[g])) continue;
+        for (;;) {
+            hipError_t q = hipEventQuery(E.ev_lvl[g]);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
+            if (now_s() - t0 > 0.2) return 1;
         }
-        if (stalled) break;
-        store_go(E, f + 1);
-        int wr = wait_flag(E, f + 1, true);
-        if (wr == 1) stalled = true;
-        else if (wr) return wr;
-        if (stalled) break;
-        record_latency(E, 1e6 * (now_s() - t0));
-        E.frame++;
-        int rc = start_levels(E, f, false);
-        if (rc) return rc;
+        E.lvl_pending[g] = false;
     }
-    if (stalled) {
+    store_go(E, f + 1);
+    E.sess_last = now_s();
+    int wr = wait_flag(E, f + 1, true);
+    if (wr) return wr;
+    record_latency(E, 1e6 * (now_s() - t0));
+    E.frame++;
+    return start_levels(E, f, false);
+}
+
+// End the session.  stop: the kernel may still be waiting for releases (a
+// stall, or a host session): stop it, then account for a tick it served after
+// all.  Completes the worker levels' state for the ticks served.
+static int persist_end(Engine &E, bool stop)
+{
+    if (!E.sess_active) return PC_OK;
+    E.sess_active = false;
+    if (stop) {
         store_go(E, PC_GO_STOP);
         HIPCHK(hipStreamSynchronize(E.s_main));
         // a stopped tick may have counted only some of its arrivals
@@ -1309,26 +1346,48 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         HIPCHK(hipMemsetAsync(E.d_ticket_ch, 0, std::max(E.nOut, 1) * sizeof(unsigned), E.s_main));
         HIPCHK(hipMemsetAsync(E.d_ticket_grp, 0, E.ticket_grp_bytes, E.s_main));
         HIPCHK(hipStreamSynchronize(E.s_main));
-    }
-    if (stalled && t < nframes && tick_done(E, f0 + (uint64_t)t + 1)) {
-        // the kernel served the tick after all (slow, not stalled): account for it
-        const uint64_t f = f0 + (uint64_t)t;
-        E.frame++;
-        int rc = start_levels(E, f, true);
-        if (rc) return rc;
-        t++;
-    }
-    *persistent_ticks = t;
-    if (E.workers_active) {
-        E.workers_active = false;
-        int rc = finish_worker_levels(E, f0 + (uint64_t)t);
-        if (rc) return rc;
-    }
-    if (stalled) {
-        for (; t < nframes; t++) {
-            int rc = engine_tick(E, d_in, in_stride, (long long)f0, d_out, out_stride, true);
+        if (E.frame < E.sess_end && tick_done(E, E.frame + 1)) {
+            // the kernel served the released tick after all (slow, not stalled)
+            const uint64_t f = E.frame;
+            E.frame++;
+            E.sess_served_late = true;
+            int rc = start_levels(E, f, true);
             if (rc) return rc;
         }
+    }
+    if (E.workers_active) {
+        E.workers_active = false;
+        int rc = finish_worker_levels(E, E.frame);
+        if (rc) return rc;
+    }
+    return PC_OK;
+}
+
+// Device-resident ticks served by one persistent tick kernel (batched engine);
+// after a stall the remaining ticks use per-tick launches.
+static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, long long nframes,
+                                 long long *persistent_ticks)
+{
+    const uint64_t f0 = E.frame;
+    const long long in_stride = (long long)E.N0 * E.nIn, out_stride = (long long)E.N0 * E.nOut;
+    *persistent_ticks = 0;
+    int rc = persist_begin(E, d_in, in_stride, d_out, out_stride, nframes, 0);
+    if (rc) return rc;  // 1: per-tick launches
+    bool stalled = false;
+    while (E.frame < f0 + (uint64_t)nframes) {
+        rc = persist_tick(E);
+        if (rc == 1) {
+            stalled = true;
+            break;
+        }
+        if (rc) return rc;
+    }
+    rc = persist_end(E, stalled);
+    if (rc) return rc;
+    *persistent_ticks = (long long)(E.frame - f0);
+    while (E.frame < f0 + (uint64_t)nframes) {
+        rc = engine_tick(E, d_in, in_stride, (long long)f0, d_out, out_stride, true);
+        if (rc) return rc;
     }
     return PC_OK;
 }
@@ -1474,6 +1533,47 @@ PC_API int pcm_setup_mem(pc_multi *h, int FS, int buffer_size, int nIn, int nOut
     return multi_setup(h, FS, buffer_size, nIn, nOut, d);
 }
 
+// One tick of the host-buffer API (frame already in the pinned input staging):
+// through a persistent session when the batched engine can run one -- the
+// kernel stays resident between calls and each call only releases a tick --
+// else one launch per tick.  A session that has been idle for
+// PC_SESSION_IDLE_S is restarted before the release, so its kernel never
+// reaches its own wait limit (PC_SPIN_LIMIT) while a tick is in flight.
+static int session_stop(Engine &E)
+{
+    return E.sess_active ? persist_end(E, true) : PC_OK;
+}
+
+static int host_tick(pc_multi *h)
+{
+    Engine &E = h->E;
+    if (E.mode == PC_BATCHED && E.host_session) {
+        if (E.sess_active && (E.sess_host == 0 || now_s() - E.sess_last > PC_SESSION_IDLE_S)) {
+            int rc = session_stop(E);
+            if (rc) return rc;
+        }
+        if (!E.sess_active) {
+            int rc = persist_begin(E, E.d_in_map, 0, E.d_out_map, 0, 1ll << 40, 1);
+            if (rc > 1) return rc;
+            E.sess_host = (rc == 0);
+            if (rc == 1) E.host_session = 0;  // this engine cannot hold a session: launches from now on
+        }
+        if (E.sess_active) {
+            int rc = persist_tick(E);
+            if (rc == 0) {
+                h->persistent_ticks++;
+                return PC_OK;
+            }
+            if (rc != 1) return rc;
+            E.sess_served_late = false;
+            rc = session_stop(E);
+            if (rc) return rc;
+            if (E.sess_served_late) return PC_OK;  // the stopped kernel served this tick
+        }
+    }
+    return engine_tick(E, E.d_in_map, 0, (long long)E.frame, E.d_out_map, 0, true);
+}
+
 // PartConvMax::setup (src/partconv.cpp:653-847): channel mode from (n, m, k),
 // the IRs from one array (IR i, tap l at impulses[stride*(filterLength*i + l)]),
 // one greedy partition over blockSizes for every instance.
@@ -1526,7 +1626,7 @@ PC_API int pcm_run_channels(pc_multi *h, float *const *output, const float *cons
         if (!input[c]) return fail(PC_ERR_ARG, "null input channel");
         for (int j = 0; j < N0; j++) E.h_in[(size_t)j * E.nIn + c] = input[c][j];
     }
-    int rc = engine_tick(E, E.d_in_map, 0, (long long)E.frame, E.d_out_map, 0, true);
+    int rc = host_tick(h);
     if (rc) return rc;
     for (int c = 0; c < E.nOut; c++) {
         if (!output[c]) return fail(PC_ERR_ARG, "null output channel");
@@ -1546,9 +1646,22 @@ PC_API int pcm_run(pc_multi *h, float *output, const float *input, int frameCoun
         return fail(PC_ERR_FRAMECOUNT, m);
     }
     memcpy(E.h_in, input, (size_t)E.N0 * E.nIn * sizeof(float));
-    int rc = engine_tick(E, E.d_in_map, 0, (long long)E.frame, E.d_out_map, 0, true);
+    int rc = host_tick(h);
     if (rc) return rc;
     memcpy(output, E.h_out, (size_t)E.N0 * E.nOut * sizeof(float));
+    return PC_OK;
+}
+
+PC_API int pcm_process(pc_multi *h, float *output, const float *input, long long nframes)
+{
+    if (!h || !output || !input || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    const size_t fi = (size_t)E.N0 * E.nIn, fo = (size_t)E.N0 * E.nOut;
+    for (long long t = 0; t < nframes; t++) {
+        int rc = pcm_run(h, output + t * fo, input + t * fi, E.N0);
+        if (rc) return rc;
+    }
     return PC_OK;
 }
 
@@ -1558,6 +1671,10 @@ PC_API int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long lon
     if (!h || !d_out || !d_in || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
     Engine &E = h->E;
     if (!E.ready) return fail(PC_ERR_STATE, "run before setup");
+    {
+        int rc = session_stop(E);   // a host-buffer session ends here
+        if (rc) return rc;
+    }
     if (sync_each_tick == 2 && E.mode == PC_BATCHED && nframes > 0) {
         long long served = 0;
         int rc = engine_run_persistent(E, d_in, d_out, nframes, &served);
@@ -1728,6 +1845,10 @@ PC_API int pcm_trace(pc_multi *h, long long nticks)
 {
     if (!h) return fail(PC_ERR_ARG, "null handle");
     Engine &E = h->E;
+    {
+        int rc = session_stop(E);
+        if (rc) return rc;
+    }
     HIPCHK(hipDeviceSynchronize());
     hipFree(E.d_trace);
     E.d_trace = nullptr;
@@ -1744,10 +1865,21 @@ PC_API int pcm_set_persistent_workers(pc_multi *h, int mode)
 {
     if (!h) return fail(PC_ERR_ARG, "null handle");
     if (mode < 0 || mode > 2) return fail(PC_ERR_ARG, "workers mode must be 0, 1 or 2");
+    int rc = session_stop(h->E);
+    if (rc) return rc;
     h->E.persist_workers = mode;
     h->E.nworkers = -1;   // plan again
     hipFree(h->E.d_wrange);
     h->E.d_wrange = nullptr;
+    return PC_OK;
+}
+
+PC_API int pcm_set_host_session(pc_multi *h, int on)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    int rc = session_stop(h->E);
+    if (rc) return rc;
+    h->E.host_session = on ? 1 : 0;
     return PC_OK;
 }
 

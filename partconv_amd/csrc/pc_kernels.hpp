@@ -561,7 +561,8 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         // ring at level-start ticks, or host-memory output
         wg_stores_done();
         if (tid == 0) {
-            if (A.fence_sys || (f & A.fence_mask) == 0) __threadfence();
+            if (A.fence_sys) __threadfence_system();
+            else if ((f & A.fence_mask) == 0) __threadfence();
             const int grp = i / PC_TICKET_GROUP;
             const int ngrp = (A.njobs + PC_TICKET_GROUP - 1) / PC_TICKET_GROUP;
             const int gsz = min(PC_TICKET_GROUP, A.njobs - grp * PC_TICKET_GROUP);
@@ -1244,6 +1245,8 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             // level blocks written by other workgroups or kernels (other XCDs) are
             // first read at deadline ticks: acquire at device scope there
             if (go && (waited || (fr & P.acq_mask) == 0)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // frames in pinned host memory, written before the release
+            if (go && P.A.fence_sys) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
             if (!go && P.dev_tick)
                 __hip_atomic_store(P.dev_tick + blockIdx.x, PC_GO_STOP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_go = go;

@@ -293,12 +293,8 @@ class PartConvMulti:
         xin = np.zeros((nfr * N0, info.numInputChannels), dtype=np.float32)
         xin[:x.shape[0]] = x
         y = np.zeros((nfr * N0, info.numOutputChannels), dtype=np.float32)
-        for t in range(nfr):
-            fi = np.ascontiguousarray(xin[t * N0:(t + 1) * N0])
-            fo = np.zeros((N0, info.numOutputChannels), dtype=np.float32)
-            if self.run(fo, fi, N0) != 0:
-                raise RuntimeError(last_error())
-            y[t * N0:(t + 1) * N0] = fo
+        if lib().pcm_process(self._h, y.ctypes.data, xin.ctypes.data, nfr) != 0:
+            raise RuntimeError(last_error())
         return y
 
     def reset(self):
@@ -343,6 +339,11 @@ class PartConvMulti:
     def trace(self, nticks):
         """Record GPU timestamps of the next persistent run (see pcm_trace)."""
         return _report(lib().pcm_trace(self._h, int(nticks)), "trace")
+
+    def set_host_session(self, on):
+        """run() through a persistent tick kernel kept resident between calls
+        (True, default) or one launch per tick (False); identical output."""
+        return _report(lib().pcm_set_host_session(self._h, 1 if on else 0), "set_host_session")
 
     def set_persistent_workers(self, mode):
         """Short-period levels in persistent runs: True/1 inside the kernel (worker

@@ -393,6 +393,48 @@ def test_many_instances_run_short_levels_in_tick_workgroups():
         assert rel(outs[1][:, i], O.direct_conv_f64(x[:, i], irs[i])) < TOL, i
 
 
+@pytest.mark.parametrize("relaxed", [False, True])
+def test_host_run_through_persistent_session(relaxed):
+    """run() with host buffers through the resident persistent kernel (frames and
+    output through pinned memory) equals one launch per tick, bitwise; the
+    session survives a pause longer than its idle limit (restart) and a
+    device-resident call in between; plus fp64."""
+    say(f"case host session relaxed={relaxed}")
+    import time
+    import torch
+    insts = [dict(N=[64, 256, 2048, 16384], P=[6, 14, 14, 0], in_ch=i, out_ch=(i + 1) % 3) for i in range(3)]
+    irs = [decaying_noise_ir(40000, 1.0, 48000, 500 + i) for i in range(3)]
+    nfr = 300
+    rng = np.random.default_rng(77)
+    x = rng.standard_normal((nfr * 64, 3)).astype(np.float32)
+    ref = make_multi(insts, irs, 64, 3, 3, relaxed=relaxed)
+    ref.set_host_session(False)
+    y_ref = ref.process(x)
+    assert ref.stats().persistent_ticks == 0
+    ref.cleanup()
+    m = make_multi(insts, irs, 64, 3, 3, relaxed=relaxed)
+    a = 137 * 64
+    y1 = m.process(x[:a])
+    time.sleep(1.3)                      # idle past the session limit: restarted on the next call
+    b = 201 * 64
+    y2 = m.process(x[a:b])
+    # device-resident ticks in between (the session stops, then resumes)
+    xd = torch.from_numpy(np.ascontiguousarray(x[b:b + 40 * 64])).cuda()
+    yd = torch.zeros_like(xd)
+    torch.cuda.synchronize()
+    assert m.run_device(yd.data_ptr(), xd.data_ptr(), 40, True, persistent=True) == 0, last_error()
+    y3 = yd.cpu().numpy()
+    y4 = m.process(x[b + 40 * 64:])
+    y = np.concatenate([y1, y2, y3, y4])
+    assert m.stats().persistent_ticks == nfr
+    m.cleanup()
+    assert np.array_equal(y, y_ref)
+    r = O.direct_conv_f64(x[:, 0], irs[0])
+    if relaxed:
+        r = np.concatenate([np.zeros(64), r[:-64]])
+    assert rel(y[:, 1], r) < TOL
+
+
 def test_uneven_batched_instances_greedy_partitions():
     """Configuration-3 shape at reduced size: uneven IR lengths (log-uniform),
     each instance with its own greedy partition over 128..32768, one batched
