@@ -208,19 +208,31 @@ def bench_mimo(args):
     yd = torch.zeros_like(xd)
     torch.cuda.synchronize()
     stride = tps * n0 * n
-    for w in range(args.warmup):
-        if m.run_device(yd.data_ptr(), xd.data_ptr() + 4 * w * stride, tps, True, persistent=True) != 0:
-            raise RuntimeError(last_error())
+    # warmup and timed steps: one persistent launch each, every tick released
+    # and waited for individually
+    if args.warmup and m.run_device(yd.data_ptr(), xd.data_ptr(), args.warmup * tps, True, persistent=True) != 0:
+        raise RuntimeError(last_error())
     m.clear_stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s_ in range(args.steps):
-        off = (args.warmup + s_) * stride
-        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, tps, True, persistent=True) != 0:
-            raise RuntimeError(last_error())
+    off = args.warmup * stride
+    if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, args.steps * tps, True,
+                    persistent=True) != 0:
+        raise RuntimeError(last_error())
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     st = m.stats()
+    # the same ticks with one tick-kernel launch per tick, for comparison
+    m.reset()
+    m.clear_stats()
+    yl = torch.zeros_like(xd)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if m.run_device(yl.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, args.steps * tps, True) != 0:
+        raise RuntimeError(last_error())
+    torch.cuda.synchronize()
+    lsecs = time.perf_counter() - t1
+    lst = m.stats()
     m.cleanup()
     # throughput form on the matrix cores: same filters, same input
     h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n)]).astype(np.float32)).cuda()
@@ -263,7 +275,10 @@ def bench_mimo(args):
         "config": {"workload": "MIMO 32x32, 65536-tap IRs, [256x30, 4096x15], buffer 256 at 48 kHz; real time as "
                                "1024 pair instances (batched, persistent tick kernel)"},
         "latency_us": {"p50": st.p50_us, "p99": st.p99_us, "max": st.max_us, "period": st.period_us,
-                       "misses": int(st.deadline_misses), "ticks": int(st.ticks)},
+                       "misses": int(st.deadline_misses), "ticks": int(st.ticks),
+                       "persistent_ticks": int(st.persistent_ticks)},
+        "launch_per_tick": {"value": n * args.steps * tps * n0 / lsecs, "p50_us": lst.p50_us,
+                            "p99_us": lst.p99_us, "misses": int(lst.deadline_misses)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err_rt,
         "throughput_form": {"output_channel_samples_per_s": n * T / tput_s, "seconds_per_call": tput_s,

@@ -220,8 +220,8 @@ int pcm_trace(pc_multi *h, long long nticks);
 long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
 
 /* Level workers of persistent runs (tuning switch, no reference counterpart):
- * mode 1 (default): the blocks of levels with period <= 8 ticks and N <= 2048
- * run inside the persistent kernel -- in worker workgroups of their own (one
+ * mode 1 (default): the blocks of levels with period <= 8 ticks, N <= 2048 and
+ * at most 8 bins per thread of the persistent kernel run inside it -- in worker workgroups of their own (one
  * per instance and level) when those fit in half of the device, else in each
  * instance's tick workgroup after its tick ("self mode"); mode 2: self mode
  * always; mode 0: every level block is a host launch on the level's stream.

@@ -621,9 +621,14 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         return J;
     };
     // levels served by in-kernel workers during persistent runs: period <= 8
-    // ticks and a single-workgroup block (N <= 2048)
+    // ticks and a single-workgroup block (N <= 2048) of at most 8 bins per
+    // thread of the persistent kernel (its thread count is TickCfg's; worker_fits)
     E.g_worker.assign(G, 0);
-    for (int g = 1; g < G; g++) E.g_worker[g] = (E.gsize[g] <= 8 && (N0 << E.gshift[g]) < PC_BIG_MIN) ? 1 : 0;
+    const int tick_nt = (N0 <= 64) ? 64 : (N0 >= 512 ? 256 : 128);
+    for (int g = 1; g < G; g++) {
+        const int Ng = N0 << E.gshift[g];
+        E.g_worker[g] = (E.gsize[g] <= 8 && Ng < PC_BIG_MIN && Ng / tick_nt <= 8) ? 1 : 0;
+    }
     for (auto &mem : members) {
         Group Gr;
         Gr.members = mem;

@@ -314,7 +314,9 @@ k_level_small(const LevelJob *__restrict__ jobs, int N0, uint64_t m, uint64_t bc
 template <int LOGN>
 struct TickCfg {
     static constexpr int N = 1 << LOGN;
-    static constexpr int NT = (N <= 64) ? 64 : ((N >= 256) ? 256 : N);
+    // 64 / 128 threads up to N = 256 (one or two wavefronts: more instances'
+    // tick workgroups fit beside the level kernels), 256 from N = 512
+    static constexpr int NT = (N <= 64) ? 64 : ((N >= 512) ? 256 : 128);
     static constexpr int IPT = (N + NT - 1) / NT;
 };
 
@@ -322,6 +324,12 @@ struct TickCfg {
 // frame: H_0 (constant), S (from the previous tick's phase B) and the previous
 // frame (left half of the window).  k_tick loads it each launch; k_persist
 // keeps it in registers across ticks.
+// A level of block size 2^LOGNL runs in the persistent kernel's workgroups
+// (NT threads) only if each thread holds at most 8 of its bins: the larger
+// blocks would raise the kernel's register count for every workgroup.
+template <int LOGNL, int NT>
+constexpr bool worker_fits() { return LOGNL <= 11 && (1 << LOGNL) / NT <= 8; }
+
 template <int LOGN>
 struct TickState {
     cf h0[TickCfg<LOGN>::IPT];
@@ -688,7 +696,8 @@ __device__ __forceinline__ cf xor_cf(cf v, int lane)
 }
 
 struct W64 {
-    cf tw[6];       // stage d = 32,16,..,1: W_{2d}^{lane & (d-1)}
+    cf tw[6];       // stage d = 32,16,..,1: W_{2d}^{lane & (d-1)} on lanes with bit d set, 1 elsewhere
+    float sg[6];    // stage d: -1 on lanes with bit d set, +1 elsewhere
     cf ws;          // split twiddle exp(-i pi k / 64) of this lane's bin k
     int k;          // bin held by this lane after the forward transform
     int partner;    // lane holding bin (64 - k) mod 64
@@ -697,26 +706,34 @@ struct W64 {
 __device__ __forceinline__ void w64_setup(W64 &w, const cf *__restrict__ tw, int lane)
 {
     int s = 0;
-    for (int d = 32; d >= 1; d >>= 1, s++) w.tw[s] = tw[2 * d + (lane & (d - 1))];
+    for (int d = 32; d >= 1; d >>= 1, s++) {
+        const bool hi = (lane & d) != 0;
+        w.tw[s] = hi ? tw[2 * d + (lane & (d - 1))] : make_float2(1.f, 0.f);
+        w.sg[s] = hi ? -1.f : 1.f;
+    }
     w.k = brev6(lane);
     w.partner = brev6((64 - w.k) & 63);
     w.ws = tw[128 + w.k];
 }
 
-// One radix-2 stage at distance D (stage index S into the lane's twiddles).
+// One radix-2 stage at distance D (stage index S), without branches: the
+// lower lane of a pair computes v + o (twiddle 1), the upper (o - v) W.
 template <int D, int S>
 __device__ __forceinline__ cf w64_fwd_stage(cf v, const W64 &w, int lane)
 {
     const cf o = xor_cf<D>(v, lane);
-    return (lane & D) ? cmul(csub(o, v), w.tw[S]) : cadd(v, o);
+    const cf u = make_float2(o.x + w.sg[S] * v.x, o.y + w.sg[S] * v.y);
+    return cmul(u, w.tw[S]);
 }
 
+// Inverse stage: t = v conj(W) on the upper lane (v on the lower), then
+// t + o / o - t.
 template <int D, int S>
 __device__ __forceinline__ cf w64_inv_stage(cf v, const W64 &w, int lane)
 {
-    const cf t = (lane & D) ? cmulc(v, w.tw[S]) : v;
+    const cf t = cmulc(v, w.tw[S]);
     const cf o = xor_cf<D>(t, lane);
-    return (lane & D) ? csub(o, t) : cadd(t, o);
+    return make_float2(o.x + w.sg[S] * t.x, o.y + w.sg[S] * t.y);
 }
 
 __device__ __forceinline__ cf w64_fwd(cf v, const W64 &w, int lane)
@@ -1061,17 +1078,17 @@ __device__ __forceinline__ void self_level(const PersistArgs &P, int q, uint64_t
     constexpr int NT = TickCfg<LOGN>::NT;
     const WorkJob &W = P.wjobs[q];
     const int d = W.logN - LOGN;
-    if constexpr (LOGN + 1 <= 11) {
+    if constexpr (worker_fits<LOGN + 1, TickCfg<LOGN>::NT>()) {
         if (d == 1) {
             level_block<LOGN + 1, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 1)), tw, tid);
         }
     }
-    if constexpr (LOGN + 2 <= 11) {
+    if constexpr (worker_fits<LOGN + 2, TickCfg<LOGN>::NT>()) {
         if (d == 2) {
             level_block<LOGN + 2, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 2)), tw, tid);
         }
     }
-    if constexpr (LOGN + 3 <= 11) {
+    if constexpr (worker_fits<LOGN + 3, TickCfg<LOGN>::NT>()) {
         if (d == 3) {
             level_block<LOGN + 3, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 3)), tw, tid);
         }
@@ -1089,13 +1106,13 @@ __device__ __forceinline__ void level_worker_dispatch(const PersistArgs &P, int2
 {
     constexpr int NT = TickCfg<LOGN>::NT;
     const int d = P.wjobs[range.x].logN - LOGN;
-    if constexpr (LOGN + 1 <= 11) {
+    if constexpr (worker_fits<LOGN + 1, TickCfg<LOGN>::NT>()) {
         if (d == 1) return level_worker<LOGN + 1, NT>(P, range, bufA, bufB, tw, tid, s_ok);
     }
-    if constexpr (LOGN + 2 <= 11) {
+    if constexpr (worker_fits<LOGN + 2, TickCfg<LOGN>::NT>()) {
         if (d == 2) return level_worker<LOGN + 2, NT>(P, range, bufA, bufB, tw, tid, s_ok);
     }
-    if constexpr (LOGN + 3 <= 11) {
+    if constexpr (worker_fits<LOGN + 3, TickCfg<LOGN>::NT>()) {
         if (d == 3) return level_worker<LOGN + 3, NT>(P, range, bufA, bufB, tw, tid, s_ok);
     }
 }
@@ -1151,13 +1168,13 @@ k_worker_fix(const WorkJob *__restrict__ wjobs, unsigned long long *lvl_done, ui
     const WorkJob &W = wjobs[blockIdx.x];
     unsigned long long *done = lvl_done + blockIdx.x;
     const int d = W.logN - LOGN;
-    if constexpr (LOGN + 1 <= 11) {
+    if constexpr (worker_fits<LOGN + 1, TickCfg<LOGN>::NT>()) {
         if (d == 1) worker_fix<LOGN + 1, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
     }
-    if constexpr (LOGN + 2 <= 11) {
+    if constexpr (worker_fits<LOGN + 2, TickCfg<LOGN>::NT>()) {
         if (d == 2) worker_fix<LOGN + 2, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
     }
-    if constexpr (LOGN + 3 <= 11) {
+    if constexpr (worker_fits<LOGN + 3, TickCfg<LOGN>::NT>()) {
         if (d == 3) worker_fix<LOGN + 3, NT>(W, done, fend, N0, pc_lds, tw, threadIdx.x);
     }
 }
