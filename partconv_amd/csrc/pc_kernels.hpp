@@ -478,24 +478,32 @@ __device__ __forceinline__ void tick_phaseB(const TickJob &TJ, uint64_t f, int t
             s[q] = (P > 1) ? pmul(xr[q], J.H[(size_t)N + k], k == 0) : make_float2(0.f, 0.f);
         }
     }
-    int r = slot;
-    for (int p = 2; p < P; p++) {
-        r = (r == 0) ? P - 1 : r - 1;   // (f + 1 - p) mod P
-        const cf *a = J.fdl + (size_t)r * N;
-        const cf *h = J.H + (size_t)p * N;
-        cf av[IPT], hv[IPT];
+    // partitions 2..P-1 in batches of B: all loads of a batch in flight
+    // together, accumulated in partition order
+    constexpr int B = (IPT >= 4) ? 2 : 8 / IPT;
+    for (int p0 = 2; p0 < P; p0 += B) {
+        cf av[B][IPT], hv[B][IPT];
 #pragma unroll
-        for (int q = 0; q < IPT; q++) {
-            const int k = tid + q * NT;
-            if (k < N) {
-                av[q] = a[k];
-                hv[q] = h[k];
+        for (int b = 0; b < B; b++) {
+            const int p = p0 + b;
+            int r = slot + 1 - p;               // (f + 1 - p) mod P
+            if (r < 0) r += P;
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const int k = tid + q * NT;
+                if (p < P && k < N) {
+                    av[b][q] = gload2(J.fdl + (size_t)r * N + k);
+                    hv[b][q] = gload2(J.H + (size_t)p * N + k);
+                }
             }
         }
 #pragma unroll
-        for (int q = 0; q < IPT; q++) {
-            const int k = tid + q * NT;
-            if (k < N) s[q] = pmac(s[q], av[q], hv[q], k == 0);
+        for (int b = 0; b < B; b++) {
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const int k = tid + q * NT;
+                if (p0 + b < P && k < N) s[q] = pmac(s[q], av[b][q], hv[b][q], k == 0);
+            }
         }
     }
 #pragma unroll
@@ -873,10 +881,22 @@ __device__ __forceinline__ void tick64_phaseB(const TickJob &TJ, uint64_t f, cf 
     const int slot = (int)(f % (uint64_t)P);
     J.fdl[(size_t)slot * 64 + k] = xr;
     cf s = (P > 1) ? pmul(xr, J.H[64 + k], k == 0) : make_float2(0.f, 0.f);
-    int r = slot;
-    for (int p = 2; p < P; p++) {
-        r = (r == 0) ? P - 1 : r - 1;
-        s = pmac(s, J.fdl[(size_t)r * 64 + k], J.H[(size_t)p * 64 + k], k == 0);
+    // partitions 2..P-1 in batches of 8: loads in flight together, summed in order
+    for (int p0 = 2; p0 < P; p0 += 8) {
+        cf av[8], hv[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int p = p0 + b;
+            int r = slot + 1 - p;               // (f + 1 - p) mod P
+            if (r < 0) r += P;
+            if (p < P) {
+                av[b] = gload2(J.fdl + (size_t)r * 64 + k);
+                hv[b] = gload2(J.H + (size_t)p * 64 + k);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            if (p0 + b < P) s = pmac(s, av[b], hv[b], k == 0);
     }
     J.S[k] = s;
     st.sp = s;

@@ -510,6 +510,32 @@ def test_partconvmax_rejects_unsupported_channel_layout():
     assert "channel configuration" in last_error()
 
 
+def test_mono_reference_configuration_full_length():
+    """The reference's own CPU-runnable case: 10 s of white noise at 44.1 kHz,
+    a 65536-tap decaying-noise IR (unit L2), block sizes [64, 1024, 8192] with
+    the nearest feasible counts [30, 15, 6] (the stated [16, 15, 7] fails the
+    delay rule), sample by sample against the fp32 oracle and fp64, strict and
+    relaxed; the stated partition is rejected like the reference rejects it."""
+    say("case configuration 1 (mono, 10 s)")
+    from partconv_amd.workloads import CFG1, CFG1_STATED, cfg1
+    x, h = cfg1(seed=1)
+    for relaxed in (False, True):
+        m = make_multi([dict(N=CFG1["N"], P=CFG1["P"], in_ch=0, out_ch=0)], [h], 64, 1, 1,
+                       relaxed=relaxed, FS=CFG1["FS"])
+        y = m.process(x)[:x.size, 0]
+        m.cleanup()
+        ref = O.OracleMulti(CFG1["FS"], 64, 1, 1, [dict(N=CFG1["N"], P=CFG1["P"])], [h],
+                            relaxed=relaxed).process(x)[:x.size, 0]
+        assert rel(y, ref) < TOL
+        r = O.direct_conv_f64(x, h)
+        if relaxed:
+            r = np.concatenate([np.zeros(64), r[:-64]])
+        assert rel(y, r) < TOL
+    bad = PartConvMulti(0)
+    assert bad.setup_mem(CFG1["FS"], 64, 1, 1, [dict(ir=h, **CFG1_STATED)]) != 0
+    assert "partition" in last_error().lower() or "level" in last_error().lower()
+
+
 def test_mimo_as_pair_instances():
     """MIMO in the reference's own terms: one instance per (input i -> output o)
     pair, outputs summed (src/partconvMulti.cpp:911-921).  8x8 at reduced size;
