@@ -404,6 +404,7 @@ def main():
     mb = build_engine("batched", irs)
     mb.probe(3)  # delay-line partial-sum kernel of the 16384 level
     secs, st = timed_run(mb, xd, yd, args.warmup, args.steps, torch, persistent=True)
+    release = "device memory via the PCIe BAR" if mb.release_in_device_memory else "pinned host memory"
     avg_us, launches, bytes_per = mb.probe_read()
     mb.probe(-1)
     ticks = args.steps * TICKS_PER_STEP
@@ -452,7 +453,7 @@ def main():
         "config": {"workload": "16 channels x 262144-tap IRs, 48 kHz, partition [64x6,256x14,2048x14,16384x15], "
                                "buffer 64, inputs resident in HBM, batched engine, persistent tick kernel",
                    "channels": CHANNELS, "taps": TAPS, "buffer_size": N0,
-                   "ticks_per_step": TICKS_PER_STEP},
+                   "ticks_per_step": TICKS_PER_STEP, "release_word": release},
         "latency_us": {"p50": st.p50_us, "p99": st.p99_us, "max": st.max_us, "period": st.period_us,
                        "misses": int(st.deadline_misses), "ticks": int(st.ticks),
                        "persistent_ticks": int(st.persistent_ticks)},

@@ -230,6 +230,12 @@ long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
 int pcm_set_persistent_workers(pc_multi *h, int mode);
 int pcm_workers_info(pc_multi *h, int *self_mode);
 
+/* Where the persistent kernel's release counter lives (no reference
+ * counterpart): 1 = device memory written by the host through the PCIe BAR
+ * (large-BAR systems; the kernel polls locally), 0 = pinned host memory (the
+ * kernel polls across the bus), -1 = not set up. */
+int pcm_release_info(const pc_multi *h);
+
 /* ----------------------------------------------------------------------------
  * PartConvFilter (single instance, caller-driven levels)
  * ------------------------------------------------------------------------- */

@@ -72,6 +72,7 @@ SIGNATURES = {
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "pcm_run_channels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "pcm_workers_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "pcm_release_info": (ctypes.c_int, [_vp]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),

@@ -30,7 +30,7 @@ def build(verbose=False):
            "-fvisibility=hidden", "-Wall", "-Wno-unused-result", "-Wno-unused-value",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-I", str(ROOT / "include"),
-           "-o", str(LIB) + ".tmp"] + [str(s) for s in SOURCES]
+           "-o", str(LIB) + ".tmp"] + [str(s) for s in SOURCES] + ["-lhsa-runtime64"]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     subprocess.run(cmd, check=True)

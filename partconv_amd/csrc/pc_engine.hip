@@ -13,6 +13,9 @@
 //     a tick costs one launch plus a spin on pinned memory.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
+#include <immintrin.h>
 
 #include <algorithm>
 #include <array>
@@ -20,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -89,6 +93,137 @@ static int get_twiddles(int dev, cf **out)
     }
     *out = g_tw[dev];
     return PC_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Release words in device memory, written by the host through the PCIe BAR.
+//
+// The host releases each tick of the persistent kernel by writing a counter
+// that the kernel polls.  In pinned host memory every poll is a read across
+// the bus (about 1.2 us on MI355X); in device memory the kernel polls locally
+// and only the host's write crosses the bus (posted), which measured 0.77 us
+// less per host round trip at 16 workgroups (3.25 -> 2.48 us).  On large-BAR
+// systems the HSA runtime maps device memory into the host's address space:
+// one fine-grained allocation of PC_REL_SLOTS 64-byte slots per device, with
+// CPU access granted, hands out one slot per engine.  The allocation is kept
+// until the process ends (device memory that kernels have polled is never
+// returned); slots are recycled.  Without a large BAR, or if any step fails,
+// engines keep the release word in pinned host memory.
+// ----------------------------------------------------------------------------
+#define PC_REL_SLOTS 256
+#define PC_REL_STRIDE 8  // 64-byte slots, in 8-byte words
+
+struct ReleasePage {
+    bool tried = false;
+    unsigned long long *base = nullptr;
+    std::vector<int> free_slots;
+};
+static std::mutex g_rel_mu;
+static ReleasePage g_rel[64];
+
+struct HsaFind {
+    uint32_t bdf = 0, domain = 0;
+    hsa_agent_t gpu{}, cpu{};
+    bool has_gpu = false, has_cpu = false;
+    hsa_amd_memory_pool_t pool{};
+    bool has_pool = false;
+};
+
+static hsa_status_t rel_agent_cb(hsa_agent_t a, void *data)
+{
+    HsaFind *f = static_cast<HsaFind *>(data);
+    hsa_device_type_t t;
+    if (hsa_agent_get_info(a, HSA_AGENT_INFO_DEVICE, &t) != HSA_STATUS_SUCCESS) return HSA_STATUS_SUCCESS;
+    if (t == HSA_DEVICE_TYPE_CPU && !f->has_cpu) {
+        f->cpu = a;
+        f->has_cpu = true;
+    } else if (t == HSA_DEVICE_TYPE_GPU && !f->has_gpu) {
+        uint32_t bdf = 0, dom = 0;
+        hsa_agent_get_info(a, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_BDFID, &bdf);
+        hsa_agent_get_info(a, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_DOMAIN, &dom);
+        if (bdf == f->bdf && dom == f->domain) {
+            f->gpu = a;
+            f->has_gpu = true;
+        }
+    }
+    return HSA_STATUS_SUCCESS;
+}
+
+static hsa_status_t rel_pool_cb(hsa_amd_memory_pool_t pool, void *data)
+{
+    HsaFind *f = static_cast<HsaFind *>(data);
+    hsa_amd_segment_t seg;
+    uint32_t flags = 0;
+    if (hsa_amd_memory_pool_get_info(pool, HSA_AMD_MEMORY_POOL_INFO_SEGMENT, &seg) != HSA_STATUS_SUCCESS)
+        return HSA_STATUS_SUCCESS;
+    if (seg != HSA_AMD_SEGMENT_GLOBAL) return HSA_STATUS_SUCCESS;
+    hsa_amd_memory_pool_get_info(pool, HSA_AMD_MEMORY_POOL_INFO_GLOBAL_FLAGS, &flags);
+    if ((flags & HSA_AMD_MEMORY_POOL_GLOBAL_FLAG_FINE_GRAINED) && !f->has_pool) {
+        f->pool = pool;
+        f->has_pool = true;
+    }
+    return HSA_STATUS_SUCCESS;
+}
+
+// The page of device `dev`, created on first use; nullptr when unavailable.
+static unsigned long long *release_page(int dev)
+{
+    ReleasePage &R = g_rel[dev];
+    if (R.tried) return R.base;
+    R.tried = true;
+    int large_bar = 0, bus = 0, devno = 0, domain = 0;
+    if (hipDeviceGetAttribute(&large_bar, hipDeviceAttributeIsLargeBar, dev) != hipSuccess || !large_bar) return nullptr;
+    if (hipDeviceGetAttribute(&bus, hipDeviceAttributePciBusId, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&devno, hipDeviceAttributePciDeviceId, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&domain, hipDeviceAttributePciDomainID, dev) != hipSuccess)
+        return nullptr;
+    if (hsa_init() != HSA_STATUS_SUCCESS) return nullptr;
+    HsaFind f;
+    f.bdf = ((uint32_t)bus << 8) | ((uint32_t)devno << 3);
+    f.domain = (uint32_t)domain;
+    hsa_iterate_agents(rel_agent_cb, &f);
+    if (!f.has_gpu || !f.has_cpu) return nullptr;
+    hsa_amd_agent_iterate_memory_pools(f.gpu, rel_pool_cb, &f);
+    if (!f.has_pool) return nullptr;
+    void *p = nullptr;
+    const size_t bytes = (size_t)PC_REL_SLOTS * PC_REL_STRIDE * sizeof(unsigned long long);
+    if (hsa_amd_memory_pool_allocate(f.pool, bytes, 0, &p) != HSA_STATUS_SUCCESS || !p) return nullptr;
+    if (hsa_amd_agents_allow_access(1, &f.cpu, nullptr, p) != HSA_STATUS_SUCCESS) return nullptr;  // kept
+    volatile unsigned long long *h = static_cast<volatile unsigned long long *>(p);
+    for (size_t i = 0; i < bytes / sizeof(unsigned long long); i++) h[i] = 0;
+    h[0] = 0x5ca1ab1eull;  // the host must read back what it wrote
+    _mm_mfence();
+    const bool ok = (h[0] == 0x5ca1ab1eull);
+    h[0] = 0;
+    _mm_mfence();
+    if (!ok) return nullptr;
+    R.base = static_cast<unsigned long long *>(p);
+    for (int s = PC_REL_SLOTS - 1; s >= 0; s--) R.free_slots.push_back(s);
+    return R.base;
+}
+
+// A release word for an engine on `dev`: a device-memory slot (returns the
+// slot index) or -1 when none is available.
+static int release_slot_get(int dev, unsigned long long **word)
+{
+    if (dev < 0 || dev >= 64) return -1;
+    std::lock_guard<std::mutex> lk(g_rel_mu);
+    unsigned long long *base = release_page(dev);
+    ReleasePage &R = g_rel[dev];
+    if (!base || R.free_slots.empty()) return -1;
+    const int s = R.free_slots.back();
+    R.free_slots.pop_back();
+    *word = base + (size_t)s * PC_REL_STRIDE;
+    *(volatile unsigned long long *)*word = 0;
+    _mm_mfence();
+    return s;
+}
+
+static void release_slot_put(int dev, int slot)
+{
+    if (slot < 0 || dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lk(g_rel_mu);
+    g_rel[dev].free_slots.push_back(slot);
 }
 
 // ----------------------------------------------------------------------------
@@ -221,7 +356,15 @@ struct Engine {
     unsigned long long *h_flag = nullptr;
     float *d_in_map = nullptr, *d_out_map = nullptr;   // device views of the pinned buffers
     unsigned long long *d_flag_map = nullptr;
-    unsigned long long *h_wg = nullptr, *d_wg_map = nullptr;  // per-workgroup flags (direct mix)
+    // release counter of the persistent kernel: a device-memory slot written
+    // through the BAR (go_slot >= 0), else word PC_GO_WORD of h_flag
+    unsigned long long *go_h = nullptr, *go_d = nullptr;
+    int go_slot = -1;
+    // per-workgroup flags (direct mix): persistent runs use nwg flags
+    // PC_WG_FLAG_STRIDE words apart from h_wg, per-tick launches nwg packed
+    // flags after them; wg_persist says which set the current tick uses
+    unsigned long long *h_wg = nullptr, *d_wg_map = nullptr;
+    bool wg_persist = false;
     int nwg = 0;
     bool use_wgflags = false;   // direct mix with few instances: one host flag per workgroup
     uint64_t frame = 0;
@@ -263,6 +406,12 @@ struct Engine {
 static hipStream_t level_stream(Engine &E, int g) { return E.s_lvl[E.g_stream[g]]; }
 static int persist_end(Engine &E, bool stop);
 
+// Words of the per-workgroup flag block: the persistent set, then the packed set.
+static size_t wg_flag_words(const Engine &E)
+{
+    return (size_t)std::max(E.nwg, 1) * (PC_WG_FLAG_STRIDE + 1);
+}
+
 static void engine_free(Engine &E)
 {
     if (E.sess_active) (void)persist_end(E, true);
@@ -303,6 +452,9 @@ static void engine_free(Engine &E)
     if (E.h_in) hipHostFree(E.h_in);
     if (E.h_out) hipHostFree(E.h_out);
     if (E.h_flag) hipHostFree(E.h_flag);
+    release_slot_put(E.dev, E.go_slot);
+    E.go_slot = -1;
+    E.go_h = E.go_d = nullptr;
     if (E.h_wg) hipHostFree(E.h_wg);
     E.h_wg = E.d_wg_map = nullptr;
     E.h_in = E.h_out = nullptr;
@@ -772,12 +924,19 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_in_map, E.h_in, 0));
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_out_map, E.h_out, 0));
     HIPCHK(hipHostGetDevicePointer((void **)&E.d_flag_map, E.h_flag, 0));
+    E.go_slot = release_slot_get(E.dev, &E.go_h);
+    if (E.go_slot >= 0) {
+        E.go_d = E.go_h;  // one address for host and device
+    } else {
+        E.go_h = E.h_flag + PC_GO_WORD;
+        E.go_d = E.d_flag_map + PC_GO_WORD;
+    }
     E.nwg = (E.mode == PC_BATCHED) ? (int)E.inst.size() : 0;
     E.use_wgflags = (E.mix == 1 && E.nwg <= 32);
     if (E.use_wgflags) {
-        HIPCHK(hipHostMalloc(&E.h_wg, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long),
+        HIPCHK(hipHostMalloc(&E.h_wg, wg_flag_words(E) * sizeof(unsigned long long),
                              hipHostMallocCoherent | hipHostMallocMapped));
-        memset(E.h_wg, 0, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long));
+        memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
         HIPCHK(hipHostGetDevicePointer((void **)&E.d_wg_map, E.h_wg, 0));
     }
     memset(E.h_in, 0, (size_t)N0 * nIn * sizeof(float));
@@ -827,7 +986,7 @@ static int engine_reset(Engine &E)
     E.frame = 0;
     std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
     *E.h_flag = 0;
-    if (E.h_wg) memset(E.h_wg, 0, std::max<size_t>(E.nwg, 1) * sizeof(unsigned long long));
+    if (E.h_wg) memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
     return PC_OK;
 }
 
@@ -850,9 +1009,10 @@ static void record_latency(Engine &E, double us)
 static bool tick_done(const Engine &E, unsigned long long target)
 {
     if (E.use_wgflags) {
-        const volatile unsigned long long *w = E.h_wg;
+        const size_t stride = E.wg_persist ? PC_WG_FLAG_STRIDE : 1;
+        const volatile unsigned long long *w = E.h_wg + (E.wg_persist ? 0 : (size_t)E.nwg * PC_WG_FLAG_STRIDE);
         for (int i = 0; i < E.nwg; i++)
-            if (w[i] < target) return false;
+            if (w[(size_t)i * stride] < target) return false;
         return true;
     }
     return *(const volatile unsigned long long *)E.h_flag >= target;
@@ -1016,7 +1176,8 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         a.mix = E.mix;
         a.y_inst = E.y_inst;
         a.done_flag = E.d_flag_map;
-        a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
+        a.wg_flags = E.use_wgflags ? E.d_wg_map + (size_t)E.nwg * PC_WG_FLAG_STRIDE : nullptr;
+        a.wg_stride = 1;
         LaunchExtra x;
         if (E.probe_g == 0) {
             int rc = probe_take(E, x);
@@ -1149,7 +1310,13 @@ static int plan_workers(Engine &E)
 
 static void store_go(Engine &E, unsigned long long v)
 {
-    __atomic_store_n(E.h_flag + PC_GO_WORD, v, __ATOMIC_RELEASE);
+    if (E.go_slot >= 0) {
+        _mm_sfence();  // frames and state written before the release
+        *(volatile unsigned long long *)E.go_h = v;
+        _mm_sfence();  // send the (write-combined) store now
+    } else {
+        __atomic_store_n(E.go_h, v, __ATOMIC_RELEASE);
+    }
 }
 
 template <int LOGN>
@@ -1236,6 +1403,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     a.nempty = E.nempty;
     a.done_flag = E.d_flag_map;
     a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
+    a.wg_stride = PC_WG_FLAG_STRIDE;
     // frames in / output to pinned host memory: system-scope acquire after the
     // release and system-scope release before the completion flag
     a.fence_sys = host_io;
@@ -1256,16 +1424,18 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         P.pub_mask = (separate && kw) ? kw - 1 : ~0ull;
     };
     set_masks(false, false);
-    // with many workgroups only workgroup 0 polls the bus for the release and
-    // relays it through device memory (d_ticket[2..3]); up to 32 each poll the
-    // bus themselves (measured faster than the relay at 16 instances)
-    P.dev_go = (a.njobs > 32) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
+    // A release word in device memory is polled by every workgroup.  One in
+    // pinned host memory is polled across the bus: with many workgroups only
+    // workgroup 0 polls it and relays it through device memory
+    // (d_ticket[2..3]); up to 32 each poll the bus themselves (measured faster
+    // than the relay at 16 instances)
+    P.dev_go = (a.njobs > 32 && E.go_slot < 0) ? reinterpret_cast<unsigned long long *>(E.d_ticket + 2) : nullptr;
     if (P.dev_go) HIPCHK(hipMemsetAsync(P.dev_go, 0, sizeof(unsigned long long), E.s_main));
     a.trace = E.d_trace;
     a.trace_cap = E.trace_cap;
     P.f0 = f0;
     P.nticks = nticks;
-    P.go = E.d_flag_map + PC_GO_WORD;
+    P.go = E.go_d;
     P.spin_limit = PC_SPIN_LIMIT;
     // the levels the first tick reads must be complete before the kernel owns the tick stream
     const int G = (int)E.gsize.size();
@@ -1304,6 +1474,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         return 1;  // caller falls back to per-tick launches
     }
     E.sess_active = true;
+    E.wg_persist = true;
     E.sess_end = f0 + (uint64_t)nticks;
     E.sess_last = now_s();
     return PC_OK;
@@ -1360,6 +1531,7 @@ static int persist_end(Engine &E, bool stop)
             if (rc) return rc;
         }
     }
+    E.wg_persist = false;  // per-tick launches use the packed flags
     if (E.workers_active) {
         E.workers_active = false;
         int rc = finish_worker_levels(E, E.frame);
@@ -1896,6 +2068,12 @@ PC_API int pcm_workers_info(pc_multi *h, int *self_mode)
     if (plan_workers(h->E)) return -1;
     if (self_mode) *self_mode = h->E.self_levels ? 1 : 0;
     return h->E.nworkers;
+}
+
+PC_API int pcm_release_info(const pc_multi *h)
+{
+    if (!h || !h->E.ready) return -1;
+    return h->E.go_slot >= 0 ? 1 : 0;
 }
 
 PC_API long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap)

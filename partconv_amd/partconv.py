@@ -363,6 +363,13 @@ class PartConvMulti:
         lib().pcm_workers_info(self._h, ctypes.byref(v))
         return bool(v.value)
 
+    @property
+    def release_in_device_memory(self):
+        """True when the persistent kernel's release counter is in device memory
+        written through the PCIe BAR (polled locally), False when it is in
+        pinned host memory (polled across the bus)."""
+        return lib().pcm_release_info(self._h) == 1
+
     def trace_read(self, cap=1 << 20):
         """Per tick 8 GPU clock values (100 MHz): release seen / published by
         workgroups 0, njobs/2, njobs-1; completion raised; workgroup 0 ready."""
