@@ -406,6 +406,18 @@ struct Engine {
 static hipStream_t level_stream(Engine &E, int g) { return E.s_lvl[E.g_stream[g]]; }
 static int persist_end(Engine &E, bool stop);
 
+// Frame layouts of a tick: the caller's interleaved device buffers, or the
+// engine's pinned host staging, which is planar (channel c's N0 samples
+// contiguous) so each workgroup reads and writes whole 64-byte lines across the
+// bus; run() converts to and from the caller's interleaved buffers on the host.
+static void set_io_layout(const Engine &E, TickArgs &a, bool in_staging, bool out_staging)
+{
+    a.in_cs = in_staging ? E.N0 : 1;
+    a.in_ss = in_staging ? 1 : E.nIn;
+    a.out_cs = out_staging ? E.N0 : 1;
+    a.out_ss = out_staging ? 1 : E.nOut;
+}
+
 // Words of the per-workgroup flag block: the persistent set, then the packed set.
 static size_t wg_flag_words(const Engine &E)
 {
@@ -1156,6 +1168,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
     a.out = out;
     a.out_stride = out_stride;
     a.nOut = E.nOut;
+    set_io_layout(E, a, in == E.d_in_map, out == E.d_out_map);
     a.relaxed = E.relaxed;
     a.src_off = E.d_src_off;
     a.src_list = E.d_src_list;
@@ -1188,7 +1201,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         if (E.mix == 0) {
             const int n = E.nOut * E.N0;
             hipLaunchKernelGGL(k_mix, dim3((n + 255) / 256), dim3(256), 0, E.s_main, E.y_inst, E.d_src_off,
-                               E.d_src_list, E.N0, E.nOut, out, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
+                               E.d_src_list, E.N0, E.nOut, out, a.out_cs, a.out_ss, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
                                E.d_flag_map, f);
         }
     } else {
@@ -1203,7 +1216,7 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
         }
         const int n = E.nOut * E.N0;
         hipLaunchKernelGGL(k_mix, dim3((n + 255) / 256), dim3(256), 0, E.s_main, E.y_inst, E.d_src_off,
-                           E.d_src_list, E.N0, E.nOut, out, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
+                           E.d_src_list, E.N0, E.nOut, out, a.out_cs, a.out_ss, (long long)(f - fbase) * out_stride, E.d_ticket + 1,
                            E.d_flag_map, f);
     }
     HIPCHK(hipGetLastError());
@@ -1390,6 +1403,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     a.out = d_out;
     a.out_stride = out_stride;
     a.nOut = E.nOut;
+    set_io_layout(E, a, host_io, host_io);
     a.relaxed = E.relaxed;
     a.mix = E.mix;
     a.y_inst = E.y_inst;
@@ -1801,13 +1815,13 @@ PC_API int pcm_run_channels(pc_multi *h, float *const *output, const float *cons
     const int N0 = E.N0;
     for (int c = 0; c < E.nIn; c++) {
         if (!input[c]) return fail(PC_ERR_ARG, "null input channel");
-        for (int j = 0; j < N0; j++) E.h_in[(size_t)j * E.nIn + c] = input[c][j];
+        memcpy(E.h_in + (size_t)c * N0, input[c], (size_t)N0 * sizeof(float));
     }
     int rc = host_tick(h);
     if (rc) return rc;
     for (int c = 0; c < E.nOut; c++) {
         if (!output[c]) return fail(PC_ERR_ARG, "null output channel");
-        for (int j = 0; j < N0; j++) output[c][j] = E.h_out[(size_t)j * E.nOut + c];
+        memcpy(output[c], E.h_out + (size_t)c * N0, (size_t)N0 * sizeof(float));
     }
     return PC_OK;
 }
@@ -1822,10 +1836,22 @@ PC_API int pcm_run(pc_multi *h, float *output, const float *input, int frameCoun
         snprintf(m, sizeof m, "frameCount (%d) != buffer_size (%d)", frameCount, E.N0);
         return fail(PC_ERR_FRAMECOUNT, m);
     }
-    memcpy(E.h_in, input, (size_t)E.N0 * E.nIn * sizeof(float));
+    {
+        const int N0 = E.N0, ni = E.nIn;
+        for (int c = 0; c < ni; c++) {
+            float *d = E.h_in + (size_t)c * N0;
+            for (int j = 0; j < N0; j++) d[j] = input[(size_t)j * ni + c];
+        }
+    }
     int rc = host_tick(h);
     if (rc) return rc;
-    memcpy(output, E.h_out, (size_t)E.N0 * E.nOut * sizeof(float));
+    {
+        const int N0 = E.N0, no = E.nOut;
+        for (int c = 0; c < no; c++) {
+            const float *src = E.h_out + (size_t)c * N0;
+            for (int j = 0; j < N0; j++) output[(size_t)j * no + c] = src[j];
+        }
+    }
     return PC_OK;
 }
 

@@ -88,9 +88,11 @@ struct TickArgs {
     long long in_stride;   // floats between consecutive frames (0 = same buffer)
     long long fbase;       // frame index of in[0]
     int nIn;
-    float *out;            // interleaved output
+    int in_cs, in_ss;      // sample j of channel c at in[c * in_cs + j * in_ss] (interleaved: 1, nIn)
+    float *out;            // output frame(s)
     long long out_stride;
     int nOut;
+    int out_cs, out_ss;    // as in_cs, in_ss (interleaved: 1, nOut; host staging is planar: N0, 1)
     int relaxed;
     int mix;               // 0: write y_inst only (plain mode), 1: direct, 2: reduce
     float *y_inst;         // njobs x N0 per-instance outputs (mix == 0 or 2)
@@ -385,7 +387,7 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
     for (int q = 0; q < IPT; q++) {
         const int j = tid + q * NT;
         if (j < N) {
-            xv[q] = inf[inch + (long long)j * A.nIn];
+            xv[q] = inf[(long long)inch * A.in_cs + (long long)j * A.in_ss];
             y0p[q] = (A.relaxed && f > 0) ? gmem(J.ybuf)[(size_t)((f - 1) & 1) * N + j] : 0.f;
             lev[q] = 0.f;
         }
@@ -546,11 +548,11 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     const int i = blockIdx.x;
     if (A.mix == 1) {
         if (!TJ.mute) {
-            float *o = A.out + (long long)(f - A.fbase) * A.out_stride + TJ.out_ch;
+            float *o = A.out + (long long)(f - A.fbase) * A.out_stride + (long long)TJ.out_ch * A.out_cs;
 #pragma unroll
             for (int q = 0; q < IPT; q++) {
                 const int j = tid + q * NT;
-                if (j < N) o[(long long)j * A.nOut] = acc[q];
+                if (j < N) o[(long long)j * A.out_ss] = acc[q];
             }
         }
         if (A.wg_flags) {
@@ -627,11 +629,11 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     __syncthreads();
     if (!*s_last) return;
     __threadfence();
-    float *oc = A.out + (long long)(f - A.fbase) * A.out_stride + c;
+    float *oc = A.out + (long long)(f - A.fbase) * A.out_stride + (long long)c * A.out_cs;
     for (int j = tid; j < N; j += NT) {
         float sum = 0.f;
         for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++) sum += A.y_inst[(size_t)A.src_list[q] * N + j];
-        oc[(long long)j * A.nOut] = sum;
+        oc[(long long)j * A.out_ss] = sum;
     }
     __syncthreads();
     if (tid == 0) {
@@ -644,7 +646,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     if (!*s_last) return;
     for (int e = tid; e < A.nempty * N; e += NT) {
         const int ce = A.empty_ch[e / N], j = e % N;
-        A.out[(long long)(f - A.fbase) * A.out_stride + ce + (long long)j * A.nOut] = 0.f;
+        A.out[(long long)(f - A.fbase) * A.out_stride + (long long)ce * A.out_cs + (long long)j * A.out_ss] = 0.f;
     }
     __syncthreads();
     if (tid == 0) {
@@ -801,8 +803,9 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
     // loads: lanes 32..63 fetch the new frame's pairs, lanes 0..31 the level blocks due
     float2 xv = make_float2(0.f, 0.f), lev = make_float2(0.f, 0.f), y0p = make_float2(0.f, 0.f);
     if (lane >= 32) {
-        xv.x = inf[TJ.in_ch + (long long)(2 * lo) * A.nIn];
-        xv.y = inf[TJ.in_ch + (long long)(2 * lo + 1) * A.nIn];
+        const float *ic = inf + (long long)TJ.in_ch * A.in_cs;
+        xv.x = ic[(long long)(2 * lo) * A.in_ss];
+        xv.y = ic[(long long)(2 * lo + 1) * A.in_ss];
     } else {
         if (A.relaxed && f > 0) y0p = gload2(reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64) + lo);
         // level blocks four at a time: loads in flight together, summed in level order
@@ -1383,11 +1386,11 @@ k_partial(const LevelJob *__restrict__ jobs, int N, int wg_per_job, uint64_t b)
     reinterpret_cast<float4 *>(J.S)[k2] = acc;
 }
 
-// Plain-mode mix: out[c + j nOut] = sum over unmuted instances feeding c, in
+// Plain-mode mix: out[c cs + j ss] = sum over unmuted instances feeding c, in
 // instance order (src/partconvMulti.cpp:906-921).  One launch per tick; the
 // last block to finish (device counter `cnt`) raises the host flag.
 __global__ void k_mix(const float *__restrict__ y_inst, const int *__restrict__ src_off,
-                      const int *__restrict__ src_list, int N0, int nOut, float *out,
+                      const int *__restrict__ src_list, int N0, int nOut, float *out, int cs, int ss,
                       long long out_off, unsigned *cnt, unsigned long long *done_flag, uint64_t f)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1396,7 +1399,7 @@ __global__ void k_mix(const float *__restrict__ y_inst, const int *__restrict__ 
         float sum = 0.f;
         for (int q = src_off[c]; q < src_off[c + 1]; q++)
             sum += y_inst[(size_t)src_list[q] * N0 + j];
-        out[out_off + c + (long long)j * nOut] = sum;
+        out[out_off + (long long)c * cs + (long long)j * ss] = sum;
     }
     if (done_flag) {
         __syncthreads();
