@@ -157,8 +157,12 @@ int pcm_run(pc_multi *h, float *output, const float *input, int frameCount);
 int pcm_process(pc_multi *h, float *output, const float *input, long long nframes);
 
 /* Host-buffer calls (pcm_run, pcm_run_channels) through a persistent session
- * (1, default) or one launch per tick (0).  Identical output.  No reference
- * counterpart. */
+ * (1, default) or one launch per tick (0).  In a session on a large-BAR device
+ * with nIn * buffer_size <= 16384, each input frame is written into device
+ * memory as tagged words (each 8-byte word a sample and its tick), which the
+ * kernel polls directly: the frame is its own release.  2 selects a session
+ * without them (release word, frame read from pinned host memory).  Identical
+ * output in every mode.  No reference counterpart. */
 int pcm_set_host_session(pc_multi *h, int on);
 
 /* Device-resident ticks: runs nframes consecutive ticks reading frame t from

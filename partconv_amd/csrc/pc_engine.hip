@@ -41,6 +41,7 @@
 #define PC_SPIN_LIMIT 200000000ull  // persistent-kernel waits: 2 s at 100 MHz (s_memrealtime)
 #define PC_SESSION_IDLE_S 1.0       // host sessions restart after this long without a release
 #define PC_TICK_TIMEOUT_S 10.0
+#define PC_LL_MAX_WORDS 16384       // host sessions use tagged input words up to nIn * N0 of these
 
 static thread_local std::string g_err;
 
@@ -96,30 +97,37 @@ static int get_twiddles(int dev, cf **out)
 }
 
 // ----------------------------------------------------------------------------
-// Release words in device memory, written by the host through the PCIe BAR.
+// Device memory that the host writes through the PCIe BAR.
 //
-// The host releases each tick of the persistent kernel by writing a counter
-// that the kernel polls.  In pinned host memory every poll is a read across
-// the bus (about 1.2 us on MI355X); in device memory the kernel polls locally
-// and only the host's write crosses the bus (posted), which measured 0.77 us
-// less per host round trip at 16 workgroups (3.25 -> 2.48 us).  On large-BAR
-// systems the HSA runtime maps device memory into the host's address space:
-// one fine-grained allocation of PC_REL_SLOTS 64-byte slots per device, with
-// CPU access granted, hands out one slot per engine.  The allocation is kept
-// until the process ends (device memory that kernels have polled is never
-// returned); slots are recycled.  Without a large BAR, or if any step fails,
-// engines keep the release word in pinned host memory.
+// On large-BAR systems the HSA runtime maps device memory into the host's
+// address space.  Blocks come from the device's fine-grained global pool with
+// CPU access granted (each checked by a host write and read-back).  They are
+// never returned to the runtime -- device memory that kernels have polled is
+// kept until the process ends -- but freed blocks are reused by later engines.
+// Uses:
+//   * release words: the host releases each tick of the persistent kernel by
+//     writing a counter that the kernel polls.  In pinned host memory every
+//     poll is a read across the bus (about 1.2 us on MI355X); in device memory
+//     the kernel polls locally and only the host's write crosses the bus
+//     (posted), measured 0.6-0.8 us less per host round trip at 16 workgroups.
+//     Engines take 64-byte slots of one page per device.
+//   * the input frames of host sessions (tagged words, see ll_tag).
+// Without a large BAR, or if any step fails, callers fall back to pinned host
+// memory.
 // ----------------------------------------------------------------------------
 #define PC_REL_SLOTS 256
 #define PC_REL_STRIDE 8  // 64-byte slots, in 8-byte words
 
-struct ReleasePage {
-    bool tried = false;
-    unsigned long long *base = nullptr;
+struct BarDevice {
+    bool tried = false, ok = false;
+    hsa_amd_memory_pool_t pool{};
+    hsa_agent_t cpu{};
+    std::vector<std::pair<size_t, void *>> free_blocks;
+    unsigned long long *rel_base = nullptr;
     std::vector<int> free_slots;
 };
-static std::mutex g_rel_mu;
-static ReleasePage g_rel[64];
+static std::mutex g_bar_mu;
+static BarDevice g_bar[64];
 
 struct HsaFind {
     uint32_t bdf = 0, domain = 0;
@@ -165,41 +173,72 @@ static hsa_status_t rel_pool_cb(hsa_amd_memory_pool_t pool, void *data)
     return HSA_STATUS_SUCCESS;
 }
 
-// The page of device `dev`, created on first use; nullptr when unavailable.
-static unsigned long long *release_page(int dev)
+// Find the device's fine-grained pool and the CPU agent (once); g_bar_mu held.
+static bool bar_init(int dev)
 {
-    ReleasePage &R = g_rel[dev];
-    if (R.tried) return R.base;
-    R.tried = true;
+    BarDevice &B = g_bar[dev];
+    if (B.tried) return B.ok;
+    B.tried = true;
     int large_bar = 0, bus = 0, devno = 0, domain = 0;
-    if (hipDeviceGetAttribute(&large_bar, hipDeviceAttributeIsLargeBar, dev) != hipSuccess || !large_bar) return nullptr;
+    if (hipDeviceGetAttribute(&large_bar, hipDeviceAttributeIsLargeBar, dev) != hipSuccess || !large_bar) return false;
     if (hipDeviceGetAttribute(&bus, hipDeviceAttributePciBusId, dev) != hipSuccess ||
         hipDeviceGetAttribute(&devno, hipDeviceAttributePciDeviceId, dev) != hipSuccess ||
         hipDeviceGetAttribute(&domain, hipDeviceAttributePciDomainID, dev) != hipSuccess)
-        return nullptr;
-    if (hsa_init() != HSA_STATUS_SUCCESS) return nullptr;
+        return false;
+    if (hsa_init() != HSA_STATUS_SUCCESS) return false;
     HsaFind f;
     f.bdf = ((uint32_t)bus << 8) | ((uint32_t)devno << 3);
     f.domain = (uint32_t)domain;
     hsa_iterate_agents(rel_agent_cb, &f);
-    if (!f.has_gpu || !f.has_cpu) return nullptr;
+    if (!f.has_gpu || !f.has_cpu) return false;
     hsa_amd_agent_iterate_memory_pools(f.gpu, rel_pool_cb, &f);
-    if (!f.has_pool) return nullptr;
+    if (!f.has_pool) return false;
+    B.pool = f.pool;
+    B.cpu = f.cpu;
+    B.ok = true;
+    return true;
+}
+
+// A zeroed block of `bytes` (rounded up to 4 KiB) of host-writable device
+// memory on `dev`, or nullptr; g_bar_mu held.
+static void *bar_alloc_locked(int dev, size_t bytes)
+{
+    if (!bar_init(dev)) return nullptr;
+    BarDevice &B = g_bar[dev];
+    bytes = (bytes + 4095) & ~(size_t)4095;
     void *p = nullptr;
-    const size_t bytes = (size_t)PC_REL_SLOTS * PC_REL_STRIDE * sizeof(unsigned long long);
-    if (hsa_amd_memory_pool_allocate(f.pool, bytes, 0, &p) != HSA_STATUS_SUCCESS || !p) return nullptr;
-    if (hsa_amd_agents_allow_access(1, &f.cpu, nullptr, p) != HSA_STATUS_SUCCESS) return nullptr;  // kept
+    for (size_t i = 0; i < B.free_blocks.size(); i++)
+        if (B.free_blocks[i].first == bytes) {
+            p = B.free_blocks[i].second;
+            B.free_blocks.erase(B.free_blocks.begin() + i);
+            break;
+        }
+    if (!p) {
+        if (hsa_amd_memory_pool_allocate(B.pool, bytes, 0, &p) != HSA_STATUS_SUCCESS || !p) return nullptr;
+        if (hsa_amd_agents_allow_access(1, &B.cpu, nullptr, p) != HSA_STATUS_SUCCESS) return nullptr;  // kept
+        volatile unsigned long long *h = static_cast<volatile unsigned long long *>(p);
+        h[0] = 0x5ca1ab1eull;  // the host must read back what it wrote
+        _mm_mfence();
+        if (h[0] != 0x5ca1ab1eull) return nullptr;
+    }
     volatile unsigned long long *h = static_cast<volatile unsigned long long *>(p);
     for (size_t i = 0; i < bytes / sizeof(unsigned long long); i++) h[i] = 0;
-    h[0] = 0x5ca1ab1eull;  // the host must read back what it wrote
     _mm_mfence();
-    const bool ok = (h[0] == 0x5ca1ab1eull);
-    h[0] = 0;
-    _mm_mfence();
-    if (!ok) return nullptr;
-    R.base = static_cast<unsigned long long *>(p);
-    for (int s = PC_REL_SLOTS - 1; s >= 0; s--) R.free_slots.push_back(s);
-    return R.base;
+    return p;
+}
+
+static void *bar_alloc(int dev, size_t bytes)
+{
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_bar_mu);
+    return bar_alloc_locked(dev, bytes);
+}
+
+static void bar_free(int dev, void *p, size_t bytes)
+{
+    if (!p || dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lk(g_bar_mu);
+    g_bar[dev].free_blocks.push_back({(bytes + 4095) & ~(size_t)4095, p});
 }
 
 // A release word for an engine on `dev`: a device-memory slot (returns the
@@ -207,13 +246,19 @@ static unsigned long long *release_page(int dev)
 static int release_slot_get(int dev, unsigned long long **word)
 {
     if (dev < 0 || dev >= 64) return -1;
-    std::lock_guard<std::mutex> lk(g_rel_mu);
-    unsigned long long *base = release_page(dev);
-    ReleasePage &R = g_rel[dev];
-    if (!base || R.free_slots.empty()) return -1;
-    const int s = R.free_slots.back();
-    R.free_slots.pop_back();
-    *word = base + (size_t)s * PC_REL_STRIDE;
+    std::lock_guard<std::mutex> lk(g_bar_mu);
+    BarDevice &B = g_bar[dev];
+    if (!B.rel_base) {
+        if (B.tried && !B.ok) return -1;
+        B.rel_base = static_cast<unsigned long long *>(
+            bar_alloc_locked(dev, (size_t)PC_REL_SLOTS * PC_REL_STRIDE * sizeof(unsigned long long)));
+        if (!B.rel_base) return -1;
+        for (int s = PC_REL_SLOTS - 1; s >= 0; s--) B.free_slots.push_back(s);
+    }
+    if (B.free_slots.empty()) return -1;
+    const int s = B.free_slots.back();
+    B.free_slots.pop_back();
+    *word = B.rel_base + (size_t)s * PC_REL_STRIDE;
     *(volatile unsigned long long *)*word = 0;
     _mm_mfence();
     return s;
@@ -222,8 +267,8 @@ static int release_slot_get(int dev, unsigned long long **word)
 static void release_slot_put(int dev, int slot)
 {
     if (slot < 0 || dev < 0 || dev >= 64) return;
-    std::lock_guard<std::mutex> lk(g_rel_mu);
-    g_rel[dev].free_slots.push_back(slot);
+    std::lock_guard<std::mutex> lk(g_bar_mu);
+    g_bar[dev].free_slots.push_back(slot);
 }
 
 // ----------------------------------------------------------------------------
@@ -365,6 +410,10 @@ struct Engine {
     // flags after them; wg_persist says which set the current tick uses
     unsigned long long *h_wg = nullptr, *d_wg_map = nullptr;
     bool wg_persist = false;
+    // tagged input words of host sessions (see ll_tag): written by the host
+    // into device memory through the BAR; ll_active while a session uses them
+    unsigned long long *ll_in = nullptr;
+    bool ll_active = false;
     int nwg = 0;
     bool use_wgflags = false;   // direct mix with few instances: one host flag per workgroup
     uint64_t frame = 0;
@@ -392,7 +441,8 @@ struct Engine {
     uint64_t sess_end = 0;            // first tick the session's kernel does not serve
     double sess_last = 0;             // host time of the session's last release
     bool sess_served_late = false;    // persist_end accounted for a tick served while stopping
-    int host_session = 1;             // pcm_run through a persistent session (batched engine)
+    int host_session = 1;             // pcm_run through a persistent session (batched engine);
+                                      // 2: without tagged words
     int sess_host = 0;                // the active session serves host buffers
     int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
     // stats
@@ -468,6 +518,9 @@ static void engine_free(Engine &E)
     E.go_slot = -1;
     E.go_h = E.go_d = nullptr;
     if (E.h_wg) hipHostFree(E.h_wg);
+    bar_free(E.dev, E.ll_in, (size_t)E.nIn * E.N0 * sizeof(unsigned long long));
+    E.ll_in = nullptr;
+    E.ll_active = false;
     E.h_wg = E.d_wg_map = nullptr;
     E.h_in = E.h_out = nullptr;
     E.h_flag = nullptr;
@@ -951,6 +1004,8 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
         HIPCHK(hipHostGetDevicePointer((void **)&E.d_wg_map, E.h_wg, 0));
     }
+    if (E.mode == PC_BATCHED && E.mix != 0 && (size_t)nIn * N0 <= PC_LL_MAX_WORDS)
+        E.ll_in = static_cast<unsigned long long *>(bar_alloc(E.dev, (size_t)nIn * N0 * sizeof(unsigned long long)));
     memset(E.h_in, 0, (size_t)N0 * nIn * sizeof(float));
     E.frame = 0;
     E.lat.assign(1 << 20, 0.0);
@@ -1332,6 +1387,28 @@ static void store_go(Engine &E, unsigned long long v)
     }
 }
 
+// Host sessions with tagged input words: tick f's frame (planar h_in) as
+// tagged words in device memory; the words are the release.
+static void ll_release(Engine &E, uint64_t f)
+{
+    const unsigned tag = ll_tag(f);
+    const size_t n = (size_t)E.nIn * E.N0;
+    unsigned long long *w = E.ll_in;
+    _mm_sfence();
+    for (size_t i = 0; i < n; i++) w[i] = ll_word(E.h_in[i], tag);
+    _mm_sfence();  // send the (write-combined) stores now
+}
+
+// Clear the words before a session starts: none may carry the tag of a tick
+// the session will serve (frames rewind on reset).
+static void ll_clear(Engine &E)
+{
+    const size_t n = (size_t)E.nIn * E.N0;
+    _mm_sfence();
+    for (size_t i = 0; i < n; i++) E.ll_in[i] = 0;
+    _mm_mfence();
+}
+
 template <int LOGN>
 static void launch_worker_fix_t(Engine &E, uint64_t fend)
 {
@@ -1419,8 +1496,13 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     a.wg_flags = E.use_wgflags ? E.d_wg_map : nullptr;
     a.wg_stride = PC_WG_FLAG_STRIDE;
     // frames in / output to pinned host memory: system-scope acquire after the
-    // release and system-scope release before the completion flag
+    // release (not with tagged input words: each carries its tick) and
+    // system-scope release before the completion flag
+    const bool ll = host_io && E.ll_in && E.host_session == 1;
     a.fence_sys = host_io;
+    a.ll = ll ? 1 : 0;
+    a.ll_in = E.ll_in;  // one address for host and device
+    if (ll) ll_clear(E);
     // Level jobs launched from the host start only at ticks f = 0 mod k_min
     // (k_min over the host-launched levels); those ticks publish the ring with
     // a device-scope release, and the ticks that first read such a level's
@@ -1489,6 +1571,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     }
     E.sess_active = true;
     E.wg_persist = true;
+    E.ll_active = ll;
     E.sess_end = f0 + (uint64_t)nticks;
     E.sess_last = now_s();
     return PC_OK;
@@ -1512,7 +1595,8 @@ static int persist_tick(Engine &E)
         }
         E.lvl_pending[g] = false;
     }
-    store_go(E, f + 1);
+    if (E.ll_active) ll_release(E, f);
+    else store_go(E, f + 1);
     E.sess_last = now_s();
     int wr = wait_flag(E, f + 1, true);
     if (wr) return wr;
@@ -1546,6 +1630,7 @@ static int persist_end(Engine &E, bool stop)
         }
     }
     E.wg_persist = false;  // per-tick launches use the packed flags
+    E.ll_active = false;
     if (E.workers_active) {
         E.workers_active = false;
         int rc = finish_worker_levels(E, E.frame);
@@ -2082,7 +2167,8 @@ PC_API int pcm_set_host_session(pc_multi *h, int on)
     if (!h) return fail(PC_ERR_ARG, "null handle");
     int rc = session_stop(h->E);
     if (rc) return rc;
-    h->E.host_session = on ? 1 : 0;
+    if (on < 0 || on > 2) return fail(PC_ERR_ARG, "host session mode must be 0, 1 or 2");
+    h->E.host_session = on;
     return PC_OK;
 }
 

@@ -111,7 +111,24 @@ struct TickArgs {
     int nempty;
     unsigned long long *trace;      // optional timestamps (persistent runs), PC_TRACE_W per tick
     long long trace_cap;
+    // host sessions with tagged input words (ll != 0): the input frame as
+    // planar [channel][N0] words (sample bits | tag << 32), see ll_tag; they
+    // are the release (no release word, no system-scope acquire)
+    int ll;
+    const unsigned long long *ll_in;
 };
+
+// Tag of tick f in the tagged input words of host sessions.  A word carries
+// one sample in its low half and the tick's tag in its high half, written by
+// one 8-byte store, so the kernel, which polls the words themselves, sees the
+// sample whenever it sees the tag: no separate release word to wait for before
+// the frame can be fetched.  Tags run 1 .. 2^32 - 2: consecutive ticks never
+// share one, and 0 marks a cleared word.
+__host__ __device__ inline unsigned ll_tag(uint64_t f) { return (unsigned)(f % 4294967294ull) + 1u; }
+__host__ __device__ inline unsigned long long ll_word(float v, unsigned tag)
+{
+    return ((unsigned long long)tag << 32) | (unsigned long long)__builtin_bit_cast(unsigned, v);
+}
 
 // Completion of many workgroups: arrivals are counted per group of
 // PC_TICKET_GROUP workgroups, and the last of each group arrives at a global
@@ -370,7 +387,8 @@ __device__ __forceinline__ void tick_load_state(const LevelJob &J, uint64_t f, i
 template <int LOGN>
 __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, cf *bufA,
                                             cf *bufB, const cf *__restrict__ tw, int tid, TickState<LOGN> &st,
-                                            float (&acc)[TickCfg<LOGN>::IPT], cf (&xr)[TickCfg<LOGN>::IPT])
+                                            float (&acc)[TickCfg<LOGN>::IPT], cf (&xr)[TickCfg<LOGN>::IPT],
+                                            const float *llx = nullptr)
 {
     constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
     const LevelJob &J = TJ.l0;
@@ -387,7 +405,7 @@ __device__ __forceinline__ void tick_phaseA(const TickJob &TJ, const TickArgs &A
     for (int q = 0; q < IPT; q++) {
         const int j = tid + q * NT;
         if (j < N) {
-            xv[q] = inf[(long long)inch * A.in_cs + (long long)j * A.in_ss];
+            xv[q] = llx ? llx[q] : inf[(long long)inch * A.in_cs + (long long)j * A.in_ss];
             y0p[q] = (A.relaxed && f > 0) ? gmem(J.ybuf)[(size_t)((f - 1) & 1) * N + j] : 0.f;
             lev[q] = 0.f;
         }
@@ -620,7 +638,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     if (TJ.mute) return;
     const int c = TJ.out_ch;
     const int cnt = A.src_off[c + 1] - A.src_off[c];
-    __syncthreads();
+    wg_stores_done();  // every wave's y_inst stores, not only thread 0's
     if (tid == 0) {
         __threadfence();
         const unsigned t = atomicAdd(A.ticket_ch + c, 1u);
@@ -635,7 +653,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         for (int q = A.src_off[c]; q < A.src_off[c + 1]; q++) sum += A.y_inst[(size_t)A.src_list[q] * N + j];
         oc[(long long)j * A.out_ss] = sum;
     }
-    __syncthreads();
+    wg_stores_done();  // every wave's output stores before the last arrival publishes them
     if (tid == 0) {
         A.ticket_ch[c] = 0u;
         __threadfence();
@@ -648,7 +666,7 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         const int ce = A.empty_ch[e / N], j = e % N;
         A.out[(long long)(f - A.fbase) * A.out_stride + (long long)ce * A.out_cs + (long long)j * A.out_ss] = 0.f;
     }
-    __syncthreads();
+    wg_stores_done();
     if (tid == 0) {
         *A.ticket = 0u;
         trace_stamp(A, f, 6);
@@ -791,7 +809,7 @@ __device__ __forceinline__ void tick64_load_state(const LevelJob &J, uint64_t f,
 // Phase A for buffer_size 64; leaves the instance's output sample `lane` in
 // *acc (through LDS) and X_f of bin st.w.k in *xr.
 __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, float *obuf, int lane,
-                                              TickState64 &st, float *acc, cf *xr)
+                                              TickState64 &st, float *acc, cf *xr, const float *llx = nullptr)
 {
     const LevelJob &J = TJ.l0;
     const long long mask = J.ring_mask;
@@ -803,9 +821,13 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
     // loads: lanes 32..63 fetch the new frame's pairs, lanes 0..31 the level blocks due
     float2 xv = make_float2(0.f, 0.f), lev = make_float2(0.f, 0.f), y0p = make_float2(0.f, 0.f);
     if (lane >= 32) {
-        const float *ic = inf + (long long)TJ.in_ch * A.in_cs;
-        xv.x = ic[(long long)(2 * lo) * A.in_ss];
-        xv.y = ic[(long long)(2 * lo + 1) * A.in_ss];
+        if (llx) {
+            xv = make_float2(llx[0], llx[1]);
+        } else {
+            const float *ic = inf + (long long)TJ.in_ch * A.in_cs;
+            xv.x = ic[(long long)(2 * lo) * A.in_ss];
+            xv.y = ic[(long long)(2 * lo + 1) * A.in_ss];
+        }
     } else {
         if (A.relaxed && f > 0) y0p = gload2(reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 64) + lo);
         // level blocks four at a time: loads in flight together, summed in level order
@@ -1225,6 +1247,64 @@ __device__ __forceinline__ unsigned long long poll_host(const unsigned long long
     }
 }
 
+// Samples per thread that phase A takes from the tagged input words: two for
+// lanes 32..63 of the 64-point form, else IPT.
+template <int LOGN>
+struct LLCnt {
+    static constexpr int v = (LOGN == 6) ? 2 : TickCfg<LOGN>::IPT;
+};
+
+// Host sessions with tagged words: wait until this workgroup's input words of
+// tick f carry the tick's tag and return the thread's samples in x.  Every
+// wave polls its own words (system-scope loads: device memory written through
+// the BAR, or pinned host memory); lane 0 also reads the release word for the
+// stop bit.  Returns 1 when the frame is in, 0 on stop or timeout (uniform over
+// the workgroup).
+template <int LOGN>
+__device__ __forceinline__ int ll_wait_frame(const TickJob &TJ, const TickArgs &A, const unsigned long long *go,
+                                             uint64_t f, int tid, unsigned long long limit,
+                                             float (&x)[LLCnt<LOGN>::v])
+{
+    constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, C = LLCnt<LOGN>::v;
+    const unsigned want = ll_tag(f);
+    const unsigned long long *src = A.ll_in + (size_t)TJ.in_ch * N;
+    const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + limit;
+    int stop = 0;
+    for (;;) {
+        int mine = 1;
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+            int j;
+            bool valid;
+            if constexpr (LOGN == 6) {
+                valid = tid >= 32;
+                j = 2 * (tid & 31) + q;
+            } else {
+                j = tid + q * NT;
+                valid = j < N;
+            }
+            if (valid) {
+                const unsigned long long w = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                x[q] = __uint_as_float((unsigned)w);
+                mine &= ((unsigned)(w >> 32) == want) ? 1 : 0;
+            }
+        }
+        if (__all(mine)) break;
+        int st = 0;
+        if ((tid & 63) == 0) {
+            const unsigned long long g = __hip_atomic_load(go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            st = ((g & PC_GO_STOP) != 0) || __builtin_amdgcn_s_memrealtime() > t_end;
+        }
+        if (__any(st)) {
+            stop = 1;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if constexpr (NT > 64) stop = __syncthreads_or(stop);
+    return !stop;
+}
+
 template <int LOGN>
 __global__ void __launch_bounds__(TickCfg<LOGN>::NT)
 k_persist(PersistArgs P, const cf *__restrict__ tw)
@@ -1256,13 +1336,18 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     else tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
+        float llx[LLCnt<LOGN>::v];
+        int ll_in = 0;
+        if (P.A.ll) ll_in = ll_wait_frame<LOGN>(sTJ, P.A, P.go, f, tid, P.spin_limit, llx);
         if (tid == 0) {
             const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + P.spin_limit;
             int go = 0;
             // with many workgroups only workgroup 0 polls host memory over the bus;
             // it relays the counter through device memory to the others
             const bool host_poll = (P.dev_go == nullptr) || blockIdx.x == 0;
-            if (host_poll) {
+            if (P.A.ll) {
+                go = ll_in;  // the tagged input words were the release
+            } else if (host_poll) {
                 const unsigned long long g = poll_host(P.go, f + 1, t_end);
                 if (P.dev_go && g) __hip_atomic_store(P.dev_go, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 go = (g != 0 && !(g & PC_GO_STOP));
@@ -1293,8 +1378,9 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             // level blocks written by other workgroups or kernels (other XCDs) are
             // first read at deadline ticks: acquire at device scope there
             if (go && (waited || (fr & P.acq_mask) == 0)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            // frames in pinned host memory, written before the release
-            if (go && P.A.fence_sys) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+            // frames in pinned host memory, written before the release (tagged
+            // input words need none: each word carries its tick)
+            if (go && P.A.fence_sys && !P.A.ll) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
             if (!go && P.dev_tick)
                 __hip_atomic_store(P.dev_tick + blockIdx.x, PC_GO_STOP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_go = go;
@@ -1307,13 +1393,14 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         float acc[IPT];
         if constexpr (LOGN == 6) {
             cf xr;
-            tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr);
+            tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr,
+                          P.A.ll ? llx : nullptr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
             tick64_phaseB(sTJ, f, xr, st64);
         } else {
             cf xr[IPT];
-            tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr);
+            tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr, P.A.ll ? llx : nullptr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
             tick_phaseB<LOGN>(sTJ, f, tid, xr, st);

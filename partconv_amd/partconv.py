@@ -342,8 +342,12 @@ class PartConvMulti:
 
     def set_host_session(self, on):
         """run() through a persistent tick kernel kept resident between calls
-        (True, default) or one launch per tick (False); identical output."""
-        return _report(lib().pcm_set_host_session(self._h, 1 if on else 0), "set_host_session")
+        (True / 1, default; input frames as tagged words in device memory where
+        they fit), 2 (a session with a release word and the frame read from
+        pinned host memory),
+        or one launch per tick (False / 0); identical output."""
+        mode = 1 if on is True else (0 if on is False else int(on))
+        return _report(lib().pcm_set_host_session(self._h, mode), "set_host_session")
 
     def set_persistent_workers(self, mode):
         """Short-period levels in persistent runs: True/1 inside the kernel (worker

@@ -393,13 +393,15 @@ def test_many_instances_run_short_levels_in_tick_workgroups():
         assert rel(outs[1][:, i], O.direct_conv_f64(x[:, i], irs[i])) < TOL, i
 
 
+@pytest.mark.parametrize("session", [1, 2])
 @pytest.mark.parametrize("relaxed", [False, True])
-def test_host_run_through_persistent_session(relaxed):
-    """run() with host buffers through the resident persistent kernel (frames and
-    output through pinned memory) equals one launch per tick, bitwise; the
-    session survives a pause longer than its idle limit (restart) and a
-    device-resident call in between; plus fp64."""
-    say(f"case host session relaxed={relaxed}")
+def test_host_run_through_persistent_session(relaxed, session):
+    """run() with host buffers through the resident persistent kernel (session
+    1: input frames as tagged words in device memory; 2: release word and
+    pinned frames) equals one launch per tick, bitwise; the session survives
+    a pause longer than its idle limit (restart) and a device-resident call in
+    between; plus fp64."""
+    say(f"case host session relaxed={relaxed} session={session}")
     import time
     import torch
     insts = [dict(N=[64, 256, 2048, 16384], P=[6, 14, 14, 0], in_ch=i, out_ch=(i + 1) % 3) for i in range(3)]
@@ -413,6 +415,7 @@ def test_host_run_through_persistent_session(relaxed):
     assert ref.stats().persistent_ticks == 0
     ref.cleanup()
     m = make_multi(insts, irs, 64, 3, 3, relaxed=relaxed)
+    m.set_host_session(session)
     a = 137 * 64
     y1 = m.process(x[:a])
     time.sleep(1.3)                      # idle past the session limit: restarted on the next call
@@ -433,6 +436,40 @@ def test_host_run_through_persistent_session(relaxed):
     if relaxed:
         r = np.concatenate([np.zeros(64), r[:-64]])
     assert rel(y[:, 1], r) < TOL
+
+
+@pytest.mark.parametrize("session", [1, 2])
+@pytest.mark.parametrize("N0", [64, 128])
+def test_host_session_summed_outputs(N0, session):
+    """Host session with several sources per output channel (the reduce mix),
+    a muted instance and an output channel with no source: bitwise equal to
+    one launch per tick, and a reset mid-stream replays the same output."""
+    say(f"case host session reduce N0={N0} session={session}")
+    N = [N0, 4 * N0, 32 * N0]
+    P = [6, 14, 0]
+    insts = [dict(N=N, P=P, in_ch=0, out_ch=0), dict(N=N, P=P, in_ch=1, out_ch=0, scale=0.5),
+             dict(N=N, P=P, in_ch=1, out_ch=2), dict(N=N, P=P, in_ch=2, out_ch=2, mute=1),
+             dict(N=N, P=P, in_ch=2, out_ch=3)]
+    irs = [decaying_noise_ir(30000, 0.8, 48000, 900 + i) for i in range(len(insts))]
+    nfr = 3 * 32 + 7
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((nfr * N0, 3)).astype(np.float32)
+    ref = make_multi(insts, irs, N0, 3, 4)
+    ref.set_host_session(False)
+    y_ref = ref.process(x)
+    ref.cleanup()
+    m = make_multi(insts, irs, N0, 3, 4)
+    m.set_host_session(session)
+    y = m.process(x)
+    assert m.stats().persistent_ticks == nfr
+    assert np.array_equal(y, y_ref)
+    assert not np.any(y[:, 1])
+    m.reset()
+    y2 = m.process(x[:40 * N0])
+    m.cleanup()
+    assert np.array_equal(y2, y_ref[:40 * N0])
+    r = O.direct_conv_f64(x[:, 0], irs[0]) + O.direct_conv_f64(0.5 * x[:, 1].astype(np.float64), irs[1])
+    assert rel(y[:, 0], r) < TOL
 
 
 def test_uneven_batched_instances_greedy_partitions():
