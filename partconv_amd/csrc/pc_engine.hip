@@ -19,12 +19,16 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/partconv_amd.h"
@@ -372,6 +376,27 @@ static BigShape big_shape(int N)
     return s;
 }
 
+// Launches of host-launched level blocks during persistent runs, issued by a
+// helper thread so that the tick thread is not held by hipLaunchKernel calls
+// (a few microseconds each) between two releases.  The tick thread queues
+// (level, start tick); the helper launches the level's batches on its stream,
+// records the level's event and then publishes done[g] = start tick + 1.  The
+// tick that first reads the block waits for done[g] before it queries the
+// event.  The reference starts its level workers from the audio thread with
+// condition variables (src/partconvMulti.cpp:860-903); here the helper only
+// issues launches, the GPU does the work.
+struct LevelLauncher {
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::pair<int, uint64_t>> q;
+    bool stop = false;
+    int busy = 0;  // queued or in progress
+    std::unique_ptr<std::atomic<uint64_t>[]> done;
+    int err = 0;
+    std::string msg;
+};
+
 struct Engine {
     int dev = 0, verbosity = 0, relaxed = 0, mode = PC_BATCHED;
     int FS = 0, N0 = 0, nIn = 0, nOut = 0;
@@ -386,6 +411,9 @@ struct Engine {
     std::vector<int> g_stream;        // global level -> stream slot
     std::vector<hipEvent_t> ev_lvl;   // per global level
     std::vector<bool> lvl_pending;
+    std::unique_ptr<LevelLauncher> lau;   // persistent runs: level launches off the tick thread
+    std::vector<bool> lvl_async;          // level g's pending block was queued to it
+    std::vector<uint64_t> lvl_start;      // start tick of level g's pending block
     hipEvent_t ev_tick = nullptr;
     std::vector<Group> groups;
     int mix = 1;
@@ -455,6 +483,7 @@ struct Engine {
 
 static hipStream_t level_stream(Engine &E, int g) { return E.s_lvl[E.g_stream[g]]; }
 static int persist_end(Engine &E, bool stop);
+static void launcher_stop(Engine &E);
 
 // Frame layouts of a tick: the caller's interleaved device buffers, or the
 // engine's pinned host staging, which is planar (channel c's N0 samples
@@ -477,6 +506,7 @@ static size_t wg_flag_words(const Engine &E)
 static void engine_free(Engine &E)
 {
     if (E.sess_active) (void)persist_end(E, true);
+    launcher_stop(E);
     if (E.s_main) hipStreamSynchronize(E.s_main);
     for (hipStream_t s : E.s_lvl) hipStreamSynchronize(s);
     hipFree(E.d_wjobs);
@@ -977,6 +1007,8 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     for (int g = 1; g < G; g++) E.g_stream[g] = (g - 1) % std::max(1, nls);
     E.ev_lvl.assign(G, nullptr);
     E.lvl_pending.assign(G, false);
+    E.lvl_async.assign(G, false);
+    E.lvl_start.assign(G, 0);
     for (int g = 0; g < G; g++) HIPCHK(hipEventCreateWithFlags(&E.ev_lvl[g], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&E.ev_tick, hipEventDisableTiming));
     HIPCHK(hipHostMalloc(&E.h_in, (size_t)N0 * nIn * sizeof(float), hipHostMallocCoherent | hipHostMallocMapped));
@@ -1052,6 +1084,9 @@ static int engine_reset(Engine &E)
     HIPCHK(hipStreamSynchronize(E.s_main));
     E.frame = 0;
     std::fill(E.lvl_pending.begin(), E.lvl_pending.end(), false);
+    std::fill(E.lvl_async.begin(), E.lvl_async.end(), false);
+    if (E.lau)  // frames rewind: so do the helper's progress marks
+        for (size_t g = 0; g < E.gsize.size(); g++) E.lau->done[g].store(0);
     *E.h_flag = 0;
     if (E.h_wg) memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
     return PC_OK;
@@ -1158,9 +1193,88 @@ static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
     return PC_OK;
 }
 
+// Launch level g's blocks that start at tick f on its stream and record its event.
+static int launch_level(Engine &E, int g, uint64_t f)
+{
+    hipStream_t s = level_stream(E, g);
+    const uint64_t m = f >> E.gshift[g];
+    for (Group &Gr : E.groups)
+        for (Batch &b : Gr.batches) {
+            if (b.g != g) continue;
+            int rc = launch_batch(E, b, m, s);
+            if (rc) return rc;
+        }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(E.ev_lvl[g], s));
+    return PC_OK;
+}
+
+static void launcher_main(Engine *E)
+{
+    LevelLauncher &L = *E->lau;
+    (void)hipSetDevice(E->dev);
+    for (;;) {
+        std::pair<int, uint64_t> job;
+        {
+            std::unique_lock<std::mutex> lk(L.mu);
+            L.cv.wait(lk, [&] { return L.stop || !L.q.empty(); });
+            if (L.q.empty()) return;  // stopped with nothing left
+            job = L.q.front();
+            L.q.erase(L.q.begin());
+        }
+        const int rc = launch_level(*E, job.first, job.second);
+        std::lock_guard<std::mutex> lk(L.mu);
+        if (rc && !L.err) {
+            L.err = rc;
+            L.msg = g_err;
+        }
+        L.done[job.first].store(job.second + 1, std::memory_order_release);
+        L.busy--;
+        L.cv.notify_all();
+    }
+}
+
+static void launcher_start(Engine &E)
+{
+    if (E.lau) return;
+    E.lau.reset(new LevelLauncher());
+    const size_t G = E.gsize.size();
+    E.lau->done.reset(new std::atomic<uint64_t>[G]);
+    for (size_t g = 0; g < G; g++) E.lau->done[g].store(0);
+    E.lau->th = std::thread(launcher_main, &E);
+}
+
+// Wait until every queued launch is issued; report the first error.
+static int launcher_drain(Engine &E)
+{
+    if (!E.lau) return PC_OK;
+    LevelLauncher &L = *E.lau;
+    std::unique_lock<std::mutex> lk(L.mu);
+    L.cv.wait(lk, [&] { return L.busy == 0; });
+    if (L.err) {
+        const int rc = L.err;
+        L.err = 0;
+        return fail(rc, L.msg);
+    }
+    return PC_OK;
+}
+
+static void launcher_stop(Engine &E)
+{
+    if (!E.lau) return;
+    {
+        std::lock_guard<std::mutex> lk(E.lau->mu);
+        E.lau->stop = true;
+    }
+    E.lau->cv.notify_all();
+    if (E.lau->th.joinable()) E.lau->th.join();
+    E.lau.reset();
+}
+
 // Launch the level blocks whose period starts at tick f.  With use_event the
 // level streams wait for the tick kernel of f on the device; without it the
-// caller has already observed tick f's completion on the host.
+// caller has already observed tick f's completion on the host, and the
+// launches are queued to the helper thread when it runs (persistent runs).
 static int start_levels(Engine &E, uint64_t f, bool use_event)
 {
     const int G = (int)E.gsize.size();
@@ -1169,24 +1283,28 @@ static int start_levels(Engine &E, uint64_t f, bool use_event)
         const uint64_t k = (uint64_t)E.gsize[g];
         if (f % k != 0) continue;
         if (E.workers_active && E.g_worker[g]) continue;  // the persistent kernel's workers run it
-        hipStream_t s = level_stream(E, g);
+        E.lvl_start[g] = f;
+        E.lvl_pending[g] = true;
+        if (!use_event && E.lau) {
+            {
+                std::lock_guard<std::mutex> lk(E.lau->mu);
+                E.lau->q.push_back({g, f});
+                E.lau->busy++;
+            }
+            E.lau->cv.notify_all();
+            E.lvl_async[g] = true;
+            continue;
+        }
+        E.lvl_async[g] = false;
         if (use_event) {
             if (!recorded) {
                 HIPCHK(hipEventRecord(E.ev_tick, E.s_main));
                 recorded = true;
             }
-            HIPCHK(hipStreamWaitEvent(s, E.ev_tick, 0));
+            HIPCHK(hipStreamWaitEvent(level_stream(E, g), E.ev_tick, 0));
         }
-        const uint64_t m = f >> E.gshift[g];
-        for (Group &Gr : E.groups)
-            for (Batch &b : Gr.batches) {
-                if (b.g != g) continue;
-                int rc = launch_batch(E, b, m, s);
-                if (rc) return rc;
-            }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(E.ev_lvl[g], s));
-        E.lvl_pending[g] = true;
+        int rc = launch_level(E, g, f);
+        if (rc) return rc;
     }
     return PC_OK;
 }
@@ -1572,6 +1690,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     E.sess_active = true;
     E.wg_persist = true;
     E.ll_active = ll;
+    launcher_start(E);
     E.sess_end = f0 + (uint64_t)nticks;
     E.sess_last = now_s();
     return PC_OK;
@@ -1587,6 +1706,12 @@ static int persist_tick(Engine &E)
     const double t0 = now_s();
     for (int g = 1; g < G; g++) {
         if (!(level_due(E, g, f) && E.lvl_pending[g])) continue;
+        if (E.lvl_async[g]) {
+            // the helper thread must have issued the block (and recorded the event)
+            while (E.lau->done[g].load(std::memory_order_acquire) < E.lvl_start[g] + 1)
+                if (now_s() - t0 > 0.2) return 1;
+            E.lvl_async[g] = false;
+        }
         for (;;) {
             hipError_t q = hipEventQuery(E.ev_lvl[g]);
             if (q == hipSuccess) break;
@@ -1612,6 +1737,10 @@ static int persist_end(Engine &E, bool stop)
 {
     if (!E.sess_active) return PC_OK;
     E.sess_active = false;
+    // every queued level launch issued (and its event recorded) before
+    // anything below waits on or relaunches levels
+    const int lrc = launcher_drain(E);
+    std::fill(E.lvl_async.begin(), E.lvl_async.end(), false);
     if (stop) {
         store_go(E, PC_GO_STOP);
         HIPCHK(hipStreamSynchronize(E.s_main));
@@ -1636,7 +1765,7 @@ static int persist_end(Engine &E, bool stop)
         int rc = finish_worker_levels(E, E.frame);
         if (rc) return rc;
     }
-    return PC_OK;
+    return lrc;
 }
 
 // Device-resident ticks served by one persistent tick kernel (batched engine);
