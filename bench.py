@@ -77,6 +77,72 @@ def _lib_process(m, y, x, nframes):
     return lib().pcm_process(m._h, y.ctypes.data, x.ctypes.data, int(nframes))
 
 
+HBM_PEAK_GBPS = 8000.0
+
+
+def block_bytes(N, P):
+    """Canonical algorithmic bytes of one level block: H (P rows) and FDL (P rows
+    read, one written) of N+1 complex bins, the 2N-sample window read and the N
+    outputs written: 8 (N+1)(2P+1) + 12 N."""
+    return 8.0 * (N + 1) * (2 * P + 1) + 12.0 * N
+
+
+def tick_bytes(parts, n0):
+    """Algorithmic bytes per tick of a set of instances: each level of block
+    size N runs one block every N / n0 ticks."""
+    return sum(block_bytes(n, p) * n0 / n for N, P in parts for n, p in zip(N, P))
+
+
+def hbm_roofline(bytes_per_tick, ticks, secs, ch_per_tick):
+    """Whole-workload HBM roofline: algorithmic bytes of every block served,
+    over the wall time, against MI355X's 8 TB/s."""
+    gbps = bytes_per_tick * ticks / secs / 1e9
+    return {"alg_bytes_per_ch_sample": bytes_per_tick / ch_per_tick, "achieved_GBps": gbps,
+            "peak_GBps": HBM_PEAK_GBPS, "frac": gbps / HBM_PEAK_GBPS,
+            "note": "whole workload: canonical block bytes 8(N+1)(2P+1)+12N of every block over wall time"}
+
+
+def kernel_roofline(avg_us, launches, bytes_per, kernel):
+    """Roofline of one probed kernel (HIP events on its own stream)."""
+    if not avg_us:
+        return None
+    gbps = bytes_per / (avg_us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": None, "kernel": kernel, "avg_us": avg_us, "launches": launches,
+            "alg_bytes_per_launch": bytes_per}
+
+
+def oracle_threads(units, budget_s, calib_ticks):
+    """Time OracleMulti units (each (engine, input columns)) on threads over the
+    host's cores, every unit processing the same number of ticks; the tick count
+    is calibrated for about budget_s.  Returns (ticks, seconds, cores)."""
+    import threading
+    cores = max(1, min(len(units), len(os.sched_getaffinity(0))))
+
+    def run(nticks):
+        def work(w):
+            for u in range(w, len(units), cores):
+                eng, cols = units[u]
+                eng.process_frames(cols, nticks)
+        th = [threading.Thread(target=work, args=(w,)) for w in range(cores)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0
+
+    t_cal = run(calib_ticks)
+    for eng, _ in units:
+        eng.reset()
+    max_ticks = min(cols.shape[0] for _, cols in units) // units[0][0].N0
+    ticks = max(calib_ticks, min(max_ticks, int(calib_ticks * budget_s / max(t_cal, 1e-6))))
+    t = run(ticks)
+    for eng, _ in units:
+        eng.close()
+    return ticks, t, cores
+
+
 def cpu_baseline(irs, x_host, budget_s):
     """The fp32 oracle (CPU restatement of the reference algorithm) on the first
     ticks of the same workload, one instance per channel, the channels spread
@@ -137,6 +203,9 @@ def bench_uneven(args):
     if m.setup_mem(FS, n0, n, n, insts) != 0:
         raise RuntimeError(last_error())
     t_setup = time.perf_counter() - t_setup
+    parts = [m.instance_partition(i) for i in range(n)]
+    gtop = m.info().num_levels - 1
+    m.probe(gtop)  # delay-line partial-sum kernel of the largest level
     tps = 256  # one period of the 32768 level
     total = (args.warmup + args.steps) * tps
     xd = torch.randn((total * n0, n), dtype=torch.float32, device="cuda",
@@ -158,6 +227,8 @@ def bench_uneven(args):
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     st = m.stats()
+    avg_us, launches, bytes_per = m.probe_read()
+    m.probe(-1)
     # fp64 check of two channels (shortest and longest IR) over the whole run
     x = xd.cpu().numpy()
     y = yd.cpu().numpy()
@@ -170,6 +241,18 @@ def bench_uneven(args):
         err = max(err, float(np.max(np.abs(y[lo:, c] - r[lo:])) / np.sqrt(np.mean(r[lo:] ** 2))))
     m.cleanup()
     value = n * args.steps * tps * n0 / secs
+    cpu = None
+    if not args.no_cpu:
+        # the fp32 oracle on a seeded random sample of 64 of the instances (mono
+        # engines, the same IRs, partitions and input columns), over the host's cores
+        from oracle import oracle as O
+        pick = np.random.default_rng(33).choice(n, size=min(64, n), replace=False)
+        units = [(O.OracleMulti(FS, n0, 1, 1, [dict(N=insts[i]["N"], P=insts[i]["P"])], [irs[i]], prec="f32"),
+                  np.ascontiguousarray(x[:, i:i + 1])) for i in pick]
+        ticks, t, cores = oracle_threads(units, args.cpu_budget, 256)
+        cpu = {"value": len(units) * ticks * n0 / t, "unit": "channel-samples/s", "cores": cores, "kind": "port",
+               "sample": f"{ticks} ticks x {len(units)} of the {n} instances (seeded random pick) through the "
+                         f"fp32 CPU restatement, one mono engine per instance on {cores} threads"}
     print(json.dumps({
         "metric": "channel-samples/s at p99 block latency < block period (uneven batched instances)",
         "value": value, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -182,7 +265,12 @@ def bench_uneven(args):
                        "misses": int(st.deadline_misses), "ticks": int(st.ticks),
                        "persistent_ticks": int(st.persistent_ticks)},
         "p99_below_period": bool(st.p99_us < st.period_us),
-        "max_rel_err_vs_fp64": err}), flush=True)
+        "max_rel_err_vs_fp64": err,
+        "roofline": kernel_roofline(avg_us, launches, bytes_per,
+                                    f"k_partial (largest level, {int(parts and max(max(N) for N, _ in parts))}, "
+                                    "delay-line sum of every instance that has it)"),
+        "hbm_roofline": hbm_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0),
+        "cpu_baseline": cpu}), flush=True)
 
 
 def bench_mimo(args):
@@ -201,6 +289,8 @@ def bench_mimo(args):
     m = PartConvMulti(0)
     if m.setup_mem(FS, n0, n, n, insts) != 0:
         raise RuntimeError(last_error())
+    parts = [m.instance_partition(i) for i in range(len(insts))]
+    m.probe(1)  # delay-line partial-sum kernel of the 4096 level
     tps = 16
     total = (args.warmup + args.steps) * tps
     g = torch.Generator(device="cuda").manual_seed(4)
@@ -222,6 +312,8 @@ def bench_mimo(args):
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     st = m.stats()
+    avg_us, launches, bytes_per = m.probe_read()
+    m.probe(-1)
     # the same ticks with one tick-kernel launch per tick, for comparison
     m.reset()
     m.clear_stats()
@@ -266,6 +358,18 @@ def bench_mimo(args):
     lo = args.warmup * tps * n0
     err_rt = float(np.max(np.abs(y[lo:, o] - r[lo:])) / np.sqrt(np.mean(r[lo:] ** 2)))
     err_mx = float(np.max(np.abs(ymx[o].cpu().numpy() - r)) / np.sqrt(np.mean(r ** 2)))
+    cpu = None
+    if not args.no_cpu:
+        # the fp32 oracle as the reference formulates MIMO (pair instances summed
+        # per output): 16 of the 32 outputs, each an engine of its 32 pairs
+        from oracle import oracle as O
+        xh = np.ascontiguousarray(x.astype(np.float32))
+        units = [(O.OracleMulti(FS, n0, n, 1, [dict(N=[256, 4096], P=[30, 0], in_ch=i, out_ch=0) for i in range(n)],
+                                [irs[oo][i] for i in range(n)], prec="f32"), xh) for oo in range(16)]
+        ticks, t, cores = oracle_threads(units, args.cpu_budget, 16)
+        cpu = {"value": len(units) * ticks * n0 / t, "unit": "channel-samples/s", "cores": cores, "kind": "port",
+               "sample": f"{ticks} ticks x {len(units)} of the {n} outputs (32 pair instances each) through the "
+                         f"fp32 CPU restatement, one engine per output on {cores} threads"}
     print(json.dumps({
         "metric": "channel-samples/s at p99 block latency < block period (MIMO 32x32)",
         "value": n * args.steps * tps * n0 / secs, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps,
@@ -281,6 +385,10 @@ def bench_mimo(args):
                             "p99_us": lst.p99_us, "misses": int(lst.deadline_misses)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err_rt,
+        "roofline": kernel_roofline(avg_us, launches, bytes_per,
+                                    "k_partial (4096 level, delay-line sum of the 1024 pair instances)"),
+        "hbm_roofline": hbm_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0),
+        "cpu_baseline": cpu,
         "throughput_form": {"output_channel_samples_per_s": n * T / tput_s, "seconds_per_call": tput_s,
                             "samples": T, "matrix_core_tflops_effective": flops / tput_s / 1e12,
                             "mfma_peak_tflops_fp32": 157.3, "max_rel_err_vs_fp64": err_mx}}), flush=True)
@@ -340,6 +448,23 @@ def bench_diff(args):
     with torch.no_grad():
         yc = conv(x, h0)[3].cpu().numpy()
     err = float(np.max(np.abs(yc - r)) / np.sqrt(np.mean(r ** 2)))
+    # whole-step HBM roofline: the forward and the two adjoint products (dL/dx,
+    # dL/dh) each stream the partition spectra of every block once
+    Ns, Ps = conv.N, conv.P
+    fwd_per_sample = sum(block_bytes(nl, pl) / nl for nl, pl in zip(Ns, Ps))
+    step_bytes = 3.0 * fwd_per_sample * C * T
+    gbps = step_bytes * args.steps / secs / 1e9
+    roof = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": None, "kernel": "whole step (forward + dL/dx + dL/dh kernels)",
+            "alg_bytes_per_step": step_bytes, "alg_bytes_per_clip_sample": 3.0 * fwd_per_sample,
+            "partition": [list(Ns), list(Ps)],
+            "note": "canonical block bytes 8(N+1)(2P+1)+12N for every block of every clip, three passes; the "
+                    "partition spectra are shared by the 64 clips and served from cache, so this can exceed "
+                    "the HBM peak"}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_diff(x[:8].cpu().numpy(), h0.detach().cpu().numpy(), tgt[:8].cpu().numpy(),
+                                args.cpu_budget)
     print(json.dumps({
         "metric": "clip-samples/s, forward + adjoint (differentiable partitioned convolution)",
         "value": C * T * args.steps / secs, "unit": "clip-samples/s", "n_gpus": 1, "steps": args.steps,
@@ -350,7 +475,42 @@ def bench_diff(args):
         "max_rel_err_vs_fp64": err,
         "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
                               "ms_per_step": 1e3 * secs2 / args.steps,
-                              "recomputed_levels": int(conv2.recomputed_levels)}}), flush=True)
+                              "recomputed_levels": int(conv2.recomputed_levels)},
+        "roofline": roof, "cpu_baseline": cpu}), flush=True)
+
+
+def cpu_baseline_diff(x, h, tgt, budget_s):
+    """The same step on the host: forward y = x * h (causal, truncated), MSE
+    loss gradient g, dL/dx = correlation of g with h, dL/dh = sum over clips of
+    the correlation of g with x; whole-signal fp32 FFTs (scipy.fft, all host
+    cores).  No reference counterpart exists (the reference has no adjoint);
+    this restates the mathematics on the CPU as a baseline."""
+    import scipy.fft as sf
+    cores = len(os.sched_getaffinity(0))
+    C, T = x.shape
+    taps = h.size
+    L = sf.next_fast_len(T + taps)
+
+    def step():
+        X = sf.rfft(x, L, axis=1, workers=cores)
+        H = sf.rfft(h, L, workers=cores)
+        y = sf.irfft(X * H, L, axis=1, workers=cores)[:, :T]
+        g = (2.0 / (C * T)) * (y - tgt)
+        G = sf.rfft(g.astype(np.float32), L, axis=1, workers=cores)
+        dx = sf.irfft(G * np.conj(H), L, axis=1, workers=cores)[:, :T]
+        dh = sf.irfft(np.sum(G * np.conj(X), axis=0), L, workers=cores)[:taps]
+        return dx, dh
+
+    step()
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < budget_s or reps < 2:
+        step()
+        reps += 1
+    t = time.perf_counter() - t0
+    return {"value": C * T * reps / t, "unit": "clip-samples/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} steps over {C} of the 64 clips: forward + dL/dx + dL/dh as whole-signal fp32 "
+                      f"FFTs (scipy.fft, {cores} workers)"}
 
 
 def main():
@@ -402,6 +562,7 @@ def main():
 
     # ---- batched engine, persistent tick kernel (headline)
     mb = build_engine("batched", irs)
+    parts = [mb.instance_partition(i) for i in range(CHANNELS)]
     mb.probe(3)  # delay-line partial-sum kernel of the 16384 level
     secs, st = timed_run(mb, xd, yd, args.warmup, args.steps, torch, persistent=True)
     release = "device memory via the PCIe BAR" if mb.release_in_device_memory else "pinned host memory"
@@ -427,7 +588,7 @@ def main():
     mb.reset()
     mb.probe(0)
     timed_run(mb, xd, yd, 0, 2, torch)
-    tick_us, _, tick_bytes = mb.probe_read()
+    tick_us, _, tick_b = mb.probe_read()
     mb.probe(-1)
     mb.cleanup()
 
@@ -466,8 +627,9 @@ def main():
                      "frac": (bytes_per / (avg_us * 1e-6) / 1e9) / 8000.0 if avg_us else None,
                      "traffic": traffic, "kernel": "k_partial (16384-level delay-line sum over 14 partitions, 16 jobs)",
                      "avg_us": avg_us, "launches": launches, "alg_bytes_per_launch": bytes_per},
-        "tick_kernel": {"avg_us": tick_us, "alg_bytes_per_launch": tick_bytes,
-                        "achieved_GBps": tick_bytes / (tick_us * 1e-6) / 1e9 if tick_us else None},
+        "tick_kernel": {"avg_us": tick_us, "alg_bytes_per_launch": tick_b,
+                        "achieved_GBps": tick_b / (tick_us * 1e-6) / 1e9 if tick_us else None},
+        "hbm_roofline": hbm_roofline(tick_bytes(parts, N0), ticks, secs, CHANNELS * N0),
     }
 
     # ---- host buffers (PCIe-inclusive: frames and output in pinned host memory),
