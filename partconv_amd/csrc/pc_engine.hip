@@ -871,7 +871,7 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     // ticks and a single-workgroup block (N <= 2048) of at most 8 bins per
     // thread of the persistent kernel (its thread count is TickCfg's; worker_fits)
     E.g_worker.assign(G, 0);
-    const int tick_nt = (N0 <= 64) ? 64 : (N0 >= 512 ? 256 : 128);
+    const int tick_nt = (N0 <= 64) ? 64 : ((N0 >= 512 || N0 == 128) ? 256 : 128);  // TickCfg<log2 N0>::NT
     for (int g = 1; g < G; g++) {
         const int Ng = N0 << E.gshift[g];
         E.g_worker[g] = (E.gsize[g] <= 8 && Ng < PC_BIG_MIN && Ng / tick_nt <= 8) ? 1 : 0;

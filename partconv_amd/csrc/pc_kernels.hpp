@@ -339,9 +339,12 @@ k_level_small(const LevelJob *__restrict__ jobs, int N0, uint64_t m, uint64_t bc
 template <int LOGN>
 struct TickCfg {
     static constexpr int N = 1 << LOGN;
-    // 64 / 128 threads up to N = 256 (one or two wavefronts: more instances'
-    // tick workgroups fit beside the level kernels), 256 from N = 512
-    static constexpr int NT = (N <= 64) ? 64 : ((N >= 512) ? 256 : 128);
+    // 64 threads for N = 64 (the register-resident 64-point form); 256 for
+    // N = 128 (the tick itself uses half of them, but the short-period level
+    // blocks that tick workgroups run in self mode go twice as wide: measured
+    // 6% more throughput at 512 instances) and from N = 512; 128 otherwise (two
+    // wavefronts: more instances' tick workgroups fit beside the level kernels)
+    static constexpr int NT = (N <= 64) ? 64 : ((N >= 512 || N == 128) ? 256 : 128);
     static constexpr int IPT = (N + NT - 1) / NT;
 };
 
