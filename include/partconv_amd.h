@@ -339,6 +339,33 @@ int pcx_set_filters(pc_mimo *x, const float *d_h, void *stream);
 int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *stream);
 void pcx_destroy(pc_mimo *x);
 
+/* ----------------------------------------------------------------------------
+ * Real-time MIMO engine (no reference counterpart: the reference runs an
+ * n_in x n_out network as one PartConvMulti instance per pair, summed per
+ * output, src/partconvMulti.cpp:815-928).  Same output as those pair
+ * instances (strict semantics, zero added latency), with the input side
+ * shared: one transform and one delay line per input and level, and per
+ * frequency bin a complex (n_out x n_in P) matrix-vector product streaming the
+ * filter spectra.  Level 0 runs in one persistent kernel per call (each tick
+ * released by the host and waited for); levels >= 1 are launched per block.
+ * n_in, n_out <= 32; N[0] == buffer_size in 64..1024; block sizes <= 4096;
+ * partition rules as PartConvFilter::setup (the last count is recounted).
+ * ------------------------------------------------------------------------- */
+typedef struct pc_mimo_rt pc_mimo_rt;
+pc_mimo_rt *pcxr_create(int FS, int buffer_size, int M, const int *N, const int *P, int taps, int n_in,
+                        int n_out);
+/* d_h: [n_out][n_in][taps] float32 on the device */
+int pcxr_set_filters(pc_mimo_rt *r, const float *d_h);
+/* nframes ticks; d_in / d_out: frames of buffer_size x n_in / n_out interleaved
+ * samples on the device (PartConvMulti::run's layout, frame after frame) */
+int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long long nframes);
+int pcxr_reset(pc_mimo_rt *r);
+int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st);
+int pcxr_clear_stats(pc_mimo_rt *r);
+/* partition after the last count is recounted; returns the level count */
+int pcxr_partition(const pc_mimo_rt *r, int *N, int *P, int cap);
+void pcxr_destroy(pc_mimo_rt *r);
+
 #ifdef __cplusplus
 }
 #endif

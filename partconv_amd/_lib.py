@@ -105,6 +105,15 @@ SIGNATURES = {
     "pcx_set_filters": (ctypes.c_int, [_vp, _vp, _vp]),
     "pcx_forward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp]),
     "pcx_destroy": (None, [_vp]),
+    "pcxr_create": (_vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int]),
+    "pcxr_set_filters": (ctypes.c_int, [_vp, _vp]),
+    "pcxr_run_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong]),
+    "pcxr_reset": (ctypes.c_int, [_vp]),
+    "pcxr_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
+    "pcxr_clear_stats": (ctypes.c_int, [_vp]),
+    "pcxr_partition": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int]),
+    "pcxr_destroy": (None, [_vp]),
 }
 
 

@@ -34,6 +34,7 @@
 #include "../../include/partconv_amd.h"
 #include "pc_io.hpp"
 #include "pc_kernels.hpp"
+#include "pc_mimo_rt.hpp"
 
 #define PC_API extern "C" __attribute__((visibility("default")))
 #define PC_LEVEL_STREAMS 3
@@ -3110,4 +3111,406 @@ PC_API void pcx_destroy(pc_mimo *x)
     hipFree(x->tmp);
     pcc_destroy(x->c);
     delete x;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: real-time MIMO engine (no reference counterpart; the reference runs a
+// MIMO network as n_in x n_out PartConvMulti instances).  Kernels and the
+// algorithm: pc_mimo_rt.hpp.  Host loop as the instance engine's persistent
+// runs: one cooperative launch serves the ticks of a call, each tick released
+// by the host (release word through the BAR where available) and waited for
+// on per-output completion flags; level blocks >= 1 are launched after the
+// tick that starts them, on one stream per level, and each deadline tick waits
+// for its level's event before it is released.
+// ---------------------------------------------------------------------------
+#define MXR_MAX_N 4096  // level blocks transform in one workgroup's LDS
+
+struct pc_mimo_rt {
+    int dev = 0, FS = 0, N0 = 0, n_in = 0, n_out = 0, taps = 0, M = 0;
+    std::vector<int> N, P, T, delay;
+    MxrArgs A{};
+    std::vector<void *> allocs;
+    cf *tw = nullptr;
+    hipStream_t s_main = nullptr;
+    std::vector<hipStream_t> s_lvl;
+    std::vector<hipEvent_t> ev_lvl;
+    std::vector<bool> pending;
+    unsigned long long *h_flags = nullptr, *d_flags = nullptr;
+    unsigned long long *go_h = nullptr, *go_d = nullptr, *h_go_pinned = nullptr;
+    int go_slot = -1;
+    unsigned long long *d_bar = nullptr;
+    uint64_t frame = 0;
+    std::vector<double> lat;
+    unsigned long long ticks = 0, misses = 0;
+    double lat_sum = 0, lat_max = 0;
+    bool filters = false;
+};
+
+static void mxr_free(pc_mimo_rt *r)
+{
+    if (r->s_main) hipStreamSynchronize(r->s_main);
+    for (hipStream_t s : r->s_lvl)
+        if (s) hipStreamSynchronize(s);
+    for (void *p : r->allocs) hipFree(p);
+    r->allocs.clear();
+    for (hipStream_t s : r->s_lvl)
+        if (s) hipStreamDestroy(s);  // level 0 has none (it runs in the persistent kernel)
+    r->s_lvl.clear();
+    for (hipEvent_t e : r->ev_lvl) hipEventDestroy(e);
+    r->ev_lvl.clear();
+    if (r->s_main) hipStreamDestroy(r->s_main);
+    r->s_main = nullptr;
+    if (r->h_flags) hipHostFree(r->h_flags);
+    if (r->h_go_pinned) hipHostFree(r->h_go_pinned);
+    r->h_flags = r->h_go_pinned = nullptr;
+    release_slot_put(r->dev, r->go_slot);
+    r->go_slot = -1;
+}
+
+static int mxr_alloc(pc_mimo_rt *r, void **p, size_t bytes)
+{
+    HIPCHK(hipMalloc(p, bytes));
+    r->allocs.push_back(*p);
+    HIPCHK(hipMemset(*p, 0, bytes));
+    return PC_OK;
+}
+
+static int mxr_setup(pc_mimo_rt *r)
+{
+    HIPCHK(hipGetDevice(&r->dev));
+    // level blocks of 4096 transform in 64 KiB of LDS (two float2 buffers)
+    const int lds12 = 2 * 4096 * (int)sizeof(cf);
+    HIPCHK(hipFuncSetAttribute((const void *)k_mxr_spec<12>, hipFuncAttributeMaxDynamicSharedMemorySize, lds12));
+    HIPCHK(hipFuncSetAttribute((const void *)k_mxr_fwd<12>, hipFuncAttributeMaxDynamicSharedMemorySize, lds12));
+    HIPCHK(hipFuncSetAttribute((const void *)k_mxr_inv<12>, hipFuncAttributeMaxDynamicSharedMemorySize, lds12));
+    int rc = get_twiddles(r->dev, &r->tw);
+    if (rc) return rc;
+    const int N0 = r->N0, M = r->M;
+    long long need = 0;
+    for (int l = 0; l < M; l++)
+        need = std::max(need, std::max(2ll * r->N[l], (long long)r->N[l] + N0 + r->T[l]) + N0);
+    long long R = 1;
+    while (R < need) R <<= 1;
+    MxrArgs &A = r->A;
+    A.n_in = r->n_in;
+    A.n_out = r->n_out;
+    A.N0 = N0;
+    A.M = M;
+    A.R = R;
+    A.ring_mask = R - 1;
+    if ((rc = mxr_alloc(r, (void **)&A.ring, (size_t)r->n_in * 2 * R * sizeof(float)))) return rc;
+    for (int l = 0; l < M; l++) {
+        MxrLevel &L = A.lv[l];
+        L.N = r->N[l];
+        L.P = r->P[l];
+        L.kshift = ilog2(r->N[l] / N0);
+        L.T = r->T[l];
+        const size_t nb = (size_t)L.N * sizeof(cf);
+        cf *H = nullptr;
+        if ((rc = mxr_alloc(r, (void **)&L.fdl, (size_t)r->n_in * L.P * nb))) return rc;
+        if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * r->n_in * nb))) return rc;
+        L.H = H;
+        if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)r->n_out * nb))) return rc;
+        if ((rc = mxr_alloc(r, (void **)&L.Y, (size_t)r->n_out * nb))) return rc;
+        if (l > 0 && (rc = mxr_alloc(r, (void **)&L.ybuf, (size_t)r->n_out * 2 * L.N * sizeof(float)))) return rc;
+    }
+    if ((rc = mxr_alloc(r, (void **)&r->d_bar, 64))) return rc;
+    HIPCHK(hipStreamCreateWithFlags(&r->s_main, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, r->dev));
+    std::vector<uint32_t> cu_mask((prop.multiProcessorCount + 31) / 32, 0u);
+    for (int c = 0; c < prop.multiProcessorCount; c++) cu_mask[c / 32] |= 1u << (c % 32);
+    r->ev_lvl.assign(M, nullptr);
+    r->pending.assign(M, false);
+    for (int l = 0; l < M; l++) {
+        hipStream_t s = nullptr;
+        // a queue of its own (full CU mask): level blocks never wait behind the persistent kernel
+        if (l > 0) HIPCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)cu_mask.size(), cu_mask.data()));
+        r->s_lvl.push_back(s);
+        HIPCHK(hipEventCreateWithFlags(&r->ev_lvl[l], hipEventDisableTiming));
+    }
+    HIPCHK(hipHostMalloc(&r->h_flags, (size_t)r->n_out * 8 * sizeof(unsigned long long),
+                         hipHostMallocCoherent | hipHostMallocMapped));
+    memset(r->h_flags, 0, (size_t)r->n_out * 8 * sizeof(unsigned long long));
+    HIPCHK(hipHostGetDevicePointer((void **)&r->d_flags, r->h_flags, 0));
+    r->go_slot = release_slot_get(r->dev, &r->go_h);
+    if (r->go_slot >= 0) {
+        r->go_d = r->go_h;
+    } else {
+        HIPCHK(hipHostMalloc(&r->h_go_pinned, 64, hipHostMallocCoherent | hipHostMallocMapped));
+        memset(r->h_go_pinned, 0, 64);
+        r->go_h = r->h_go_pinned;
+        HIPCHK(hipHostGetDevicePointer((void **)&r->go_d, r->h_go_pinned, 0));
+    }
+    A.go = r->go_d;
+    A.flags = r->d_flags;
+    A.bar = r->d_bar;
+    A.spin_limit = PC_SPIN_LIMIT;
+    r->lat.reserve(1 << 16);
+    return PC_OK;
+}
+
+PC_API pc_mimo_rt *pcxr_create(int FS, int buffer_size, int M, const int *N, const int *P, int taps, int n_in,
+                               int n_out)
+{
+    if (!N || !P || M < 1 || M > PC_MAX_LEVELS || taps < 1 || FS < 1) {
+        fail(PC_ERR_ARG, "bad argument");
+        return nullptr;
+    }
+    if (n_in < 1 || n_in > MXR_MAX_IO || n_out < 1 || n_out > MXR_MAX_IO) {
+        fail(PC_ERR_ARG, "real-time MIMO sizes must be 1..32");
+        return nullptr;
+    }
+    if (N[0] != buffer_size || buffer_size < 64 || buffer_size > 1024) {
+        fail(PC_ERR_ARG, "real-time MIMO: N[0] must equal buffer_size, 64..1024");
+        return nullptr;
+    }
+    pc_mimo_rt *r = new pc_mimo_rt();
+    r->FS = FS;
+    r->N0 = buffer_size;
+    r->n_in = n_in;
+    r->n_out = n_out;
+    r->taps = taps;
+    r->M = M;
+    r->N.assign(N, N + M);
+    r->P.assign(P, P + M);
+    int rc = check_partition(r->N, r->P, taps, r->delay, r->T);
+    for (int l = 0; !rc && l < M; l++)
+        if (r->N[l] > MXR_MAX_N) rc = fail(PC_ERR_PARTITION, "real-time MIMO: block sizes up to 4096");
+    if (!rc) rc = mxr_setup(r);
+    if (rc) {
+        mxr_free(r);
+        delete r;
+        return nullptr;
+    }
+    return r;
+}
+
+template <int LOGN>
+static void mxr_spec_t(pc_mimo_rt *r, const MxrLevel &L, const float *d_h)
+{
+    hipLaunchKernelGGL(k_mxr_spec<LOGN>, dim3(L.P, r->n_in * r->n_out), dim3(MXR_NT), 2 * (1 << LOGN) * sizeof(cf),
+                       r->s_main, L, d_h, r->taps, r->n_in, r->n_out, r->tw);
+}
+
+// d_h: [n_out][n_in][taps] on the device
+PC_API int pcxr_set_filters(pc_mimo_rt *r, const float *d_h)
+{
+    if (!r || !d_h) return fail(PC_ERR_ARG, "null argument");
+    for (int l = 0; l < r->M; l++) {
+        const MxrLevel &L = r->A.lv[l];
+        switch (ilog2(L.N)) {
+        case 6: mxr_spec_t<6>(r, L, d_h); break;
+        case 7: mxr_spec_t<7>(r, L, d_h); break;
+        case 8: mxr_spec_t<8>(r, L, d_h); break;
+        case 9: mxr_spec_t<9>(r, L, d_h); break;
+        case 10: mxr_spec_t<10>(r, L, d_h); break;
+        case 11: mxr_spec_t<11>(r, L, d_h); break;
+        case 12: mxr_spec_t<12>(r, L, d_h); break;
+        default: return fail(PC_ERR_ARG, "block size out of range");
+        }
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(r->s_main));
+    r->filters = true;
+    return PC_OK;
+}
+
+template <int LOGN>
+static void mxr_level_t(pc_mimo_rt *r, int l, uint64_t m)
+{
+    const MxrLevel &L = r->A.lv[l];
+    hipStream_t s = r->s_lvl[l];
+    const size_t lds = 2 * (1 << LOGN) * sizeof(cf);
+    hipLaunchKernelGGL(k_mxr_fwd<LOGN>, dim3(r->n_in), dim3(MXR_NT), lds, s, L, r->A.ring, r->A.R, r->A.ring_mask,
+                       r->N0, m, r->tw);
+    hipLaunchKernelGGL(k_mxr_comb, dim3(L.N), dim3(MXR_NT), 0, s, L, r->n_in, r->n_out, m);
+    hipLaunchKernelGGL(k_mxr_inv<LOGN>, dim3(r->n_out), dim3(MXR_NT), lds, s, L, m, r->tw);
+    hipLaunchKernelGGL(k_mxr_part, dim3(L.N), dim3(MXR_NT), 0, s, L, r->n_in, r->n_out, m + 1);
+}
+
+// Level l's block that starts at tick f (after that tick's frame is in the ring).
+static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f)
+{
+    const uint64_t m = f >> r->A.lv[l].kshift;
+    switch (ilog2(r->A.lv[l].N)) {
+    case 7: mxr_level_t<7>(r, l, m); break;
+    case 8: mxr_level_t<8>(r, l, m); break;
+    case 9: mxr_level_t<9>(r, l, m); break;
+    case 10: mxr_level_t<10>(r, l, m); break;
+    case 11: mxr_level_t<11>(r, l, m); break;
+    case 12: mxr_level_t<12>(r, l, m); break;
+    default: return fail(PC_ERR_ARG, "block size out of range");
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(r->ev_lvl[l], r->s_lvl[l]));
+    r->pending[l] = true;
+    return PC_OK;
+}
+
+template <int LOGN>
+static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
+{
+    MxrArgs a = r->A;
+    const cf *t = r->tw;
+    void *args[] = {(void *)&a, (void *)&t};
+    return hipLaunchCooperativeKernel((const void *)k_mxr_persist<LOGN>, dim3(G), dim3(MXR_NT), args,
+                                      (unsigned)(2 * (1 << LOGN) * sizeof(cf)), r->s_main);
+}
+
+static void mxr_store_go(pc_mimo_rt *r, unsigned long long v)
+{
+    if (r->go_slot >= 0) {
+        _mm_sfence();
+        *(volatile unsigned long long *)r->go_h = v;
+        _mm_sfence();
+    } else {
+        __atomic_store_n(r->go_h, v, __ATOMIC_RELEASE);
+    }
+}
+
+// d_in / d_out: nframes frames of buffer_size x n_in / n_out interleaved
+// samples on the device.  Every tick is released and waited for (real time).
+PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long long nframes)
+{
+    if (!r || !d_out || !d_in || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
+    if (!r->filters) return fail(PC_ERR_STATE, "run before set_filters");
+    if (nframes == 0) return PC_OK;
+    const int N0 = r->N0;
+    MxrArgs &A = r->A;
+    A.in = d_in;
+    A.in_stride = (long long)N0 * r->n_in;
+    A.out = d_out;
+    A.out_stride = (long long)N0 * r->n_out;
+    A.fbase = (long long)r->frame;
+    A.f0 = r->frame;
+    A.nticks = nframes;
+    // the grid: at least every input and output, at most one workgroup per bin
+    const int G = std::max(std::max(r->n_in, r->n_out), std::min(N0, 256));
+    HIPCHK(hipMemsetAsync(r->d_bar, 0, 64, r->s_main));
+    mxr_store_go(r, r->frame);
+    hipError_t le = hipErrorInvalidValue;
+    switch (ilog2(N0)) {
+    case 6: le = mxr_launch_t<6>(r, G); break;
+    case 7: le = mxr_launch_t<7>(r, G); break;
+    case 8: le = mxr_launch_t<8>(r, G); break;
+    case 9: le = mxr_launch_t<9>(r, G); break;
+    case 10: le = mxr_launch_t<10>(r, G); break;
+    }
+    if (le != hipSuccess) return fail(PC_ERR_HIP, std::string("real-time MIMO launch: ") + hipGetErrorString(le));
+    const double period = 1e6 * N0 / r->FS;
+    int err = PC_OK;
+    for (long long t = 0; t < nframes && !err; t++) {
+        const uint64_t f = r->frame;
+        const double t0 = now_s();
+        // blocks first read at this tick (strict: (f + 1) % k == 0) must be complete
+        for (int l = 1; l < r->M && !err; l++) {
+            const uint64_t k = 1ull << A.lv[l].kshift;
+            if ((f + 1) % k != 0 || !r->pending[l]) continue;
+            for (;;) {
+                const hipError_t q = hipEventQuery(r->ev_lvl[l]);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) {
+                    err = fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
+                    break;
+                }
+                if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
+                    err = fail(PC_ERR_TIMEOUT, "level block did not complete");
+                    break;
+                }
+            }
+            r->pending[l] = false;
+        }
+        if (err) break;
+        mxr_store_go(r, f + 1);
+        for (int o = 0; o < r->n_out && !err; o++) {
+            const volatile unsigned long long *fl = r->h_flags + (size_t)o * 8;
+            while (*fl < f + 1) {
+                if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
+                    err = fail(PC_ERR_TIMEOUT, "tick did not complete");
+                    break;
+                }
+            }
+        }
+        if (err) break;
+        const double us = 1e6 * (now_s() - t0);
+        r->lat.push_back(us);
+        r->ticks++;
+        r->lat_sum += us;
+        r->lat_max = std::max(r->lat_max, us);
+        if (us > period) r->misses++;
+        r->frame++;
+        for (int l = 1; l < r->M && !err; l++)
+            if (f % (1ull << A.lv[l].kshift) == 0) err = mxr_start_level(r, l, f);
+    }
+    if (err) mxr_store_go(r, PC_GO_STOP);  // the kernel's waits end at once
+    HIPCHK(hipStreamSynchronize(r->s_main));
+    return err;
+}
+
+PC_API int pcxr_reset(pc_mimo_rt *r)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(r->s_main));
+    for (int l = 1; l < r->M; l++) HIPCHK(hipStreamSynchronize(r->s_lvl[l]));
+    const MxrArgs &A = r->A;
+    HIPCHK(hipMemsetAsync(A.ring, 0, (size_t)r->n_in * 2 * A.R * sizeof(float), r->s_main));
+    for (int l = 0; l < r->M; l++) {
+        const MxrLevel &L = A.lv[l];
+        const size_t nb = (size_t)L.N * sizeof(cf);
+        HIPCHK(hipMemsetAsync(L.fdl, 0, (size_t)r->n_in * L.P * nb, r->s_main));
+        HIPCHK(hipMemsetAsync(L.S, 0, (size_t)r->n_out * nb, r->s_main));
+        if (L.ybuf) HIPCHK(hipMemsetAsync(L.ybuf, 0, (size_t)r->n_out * 2 * L.N * sizeof(float), r->s_main));
+    }
+    HIPCHK(hipStreamSynchronize(r->s_main));
+    memset(r->h_flags, 0, (size_t)r->n_out * 8 * sizeof(unsigned long long));
+    std::fill(r->pending.begin(), r->pending.end(), false);
+    r->frame = 0;
+    return PC_OK;
+}
+
+PC_API int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st)
+{
+    if (!r || !st) return fail(PC_ERR_ARG, "null argument");
+    memset(st, 0, sizeof *st);
+    st->ticks = r->ticks;
+    st->deadline_misses = r->misses;
+    st->period_us = 1e6 * r->N0 / r->FS;
+    st->persistent_ticks = r->ticks;
+    if (!r->lat.empty()) {
+        std::vector<double> v = r->lat;
+        std::sort(v.begin(), v.end());
+        st->p50_us = v[v.size() / 2];
+        st->p99_us = v[std::min(v.size() - 1, (size_t)(0.99 * v.size()))];
+        st->max_us = r->lat_max;
+        st->mean_us = r->lat_sum / (double)r->ticks;
+    }
+    return PC_OK;
+}
+
+PC_API int pcxr_clear_stats(pc_mimo_rt *r)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    r->lat.clear();
+    r->ticks = r->misses = 0;
+    r->lat_sum = r->lat_max = 0;
+    return PC_OK;
+}
+
+PC_API int pcxr_partition(const pc_mimo_rt *r, int *N, int *P, int cap)
+{
+    if (!r || !N || !P) return -1;
+    const int M = std::min(cap, r->M);
+    for (int l = 0; l < M; l++) {
+        N[l] = r->N[l];
+        P[l] = r->P[l];
+    }
+    return r->M;
+}
+
+PC_API void pcxr_destroy(pc_mimo_rt *r)
+{
+    if (!r) return;
+    mxr_free(r);
+    delete r;
 }

@@ -1,0 +1,289 @@
+// pc_mimo_rt.hpp -- real-time MIMO engine: n_in inputs, n_out outputs, one
+// filter per (output, input) pair, all with the same partition.
+//
+// The reference formulates a MIMO network as n_in x n_out independent
+// PartConvFilter instances whose outputs are summed per output channel
+// (src/partconvMulti.cpp:815-928): every input spectrum is computed, stored
+// in a delay line and streamed once per pair.  Here the input side is shared:
+// per level and block, each input's window is transformed once (one delay
+// line per input), and per frequency bin the delay-line product is a complex
+// (n_out x n_in P) matrix times (n_in P) vector,
+//     Y_o[k] = sum_p sum_i H_p[o][i][k] X_i^(m-p)[k],
+// followed by one inverse transform per output.  The product is split as in
+// the instance engine: Y = (p = 0 term) + S, S = sum_{p>=1} precomputed after
+// block m-1, off the path from a block's input to its output.
+//
+// Level 0 runs in one persistent cooperative kernel (k_mxr_persist), per tick:
+//   1. release (host word), input workgroups: frame -> ring, window -> r2c ->
+//      level-0 delay line of that input;
+//   2. grid barrier; every workgroup: p = 0 product of its bins -> Y;
+//   3. grid barrier; output workgroups: c2r of Y_o + the level blocks due ->
+//      output frame, completion flag;
+//   4. every workgroup: S for the next tick (its bins), streaming the
+//      filter spectra -- overlapped with the host's round trip.
+// Levels >= 1 are host-launched per block on their own streams: k_mxr_fwd
+// (one workgroup per input), k_mxr_comb (per bin), k_mxr_inv (per output),
+// k_mxr_part (per bin, the next block's S).
+//
+// Layouts (device, bins in natural order, packed bin 0 = (X_0, X_N)):
+//   ring[i]   : 2R floats (doubled ring, as the instance engine's)
+//   fdl       : [n_in][P][N]     H : [N][P][n_out][n_in] (scaled by 1/2N)
+//   S, Y      : [n_out][N]       ybuf : [n_out][2][N] (levels >= 1)
+// Frames in and out are interleaved as PartConvMulti::run's buffers.
+#pragma once
+#include "pc_kernels.hpp"
+
+#define MXR_NT 256          // threads of every real-time MIMO kernel
+#define MXR_MAX_IO 32       // inputs and outputs: lane t -> output t/8, inputs 4(t%8) .. +3
+
+struct MxrLevel {
+    int N, P, kshift;
+    long long T;            // taps before this level
+    cf *fdl;
+    const cf *H;
+    cf *S;
+    cf *Y;
+    float *ybuf;
+};
+
+struct MxrArgs {
+    int n_in, n_out, N0, M;
+    float *ring;            // [n_in][2R]
+    long long R, ring_mask;
+    MxrLevel lv[PC_MAX_LEVELS];
+    const float *in;
+    long long in_stride;
+    float *out;
+    long long out_stride;
+    long long fbase;
+    uint64_t f0;
+    long long nticks;
+    const unsigned long long *go;
+    unsigned long long *flags;  // host-visible, one per output workgroup, 8 words apart
+    unsigned long long *bar;    // grid barrier counter (zeroed before each launch)
+    unsigned long long spin_limit;
+};
+
+// Bin k of level L, for every output: the sum over partitions p in [p_lo, p_hi)
+// and inputs of H_p[o][i][k] X_i^(b-p)[k], plus base[o][k] if given, into
+// dst[o][k].  Lane t handles output t/8 and inputs 4(t%8) .. 4(t%8)+3; the
+// eight partial sums of an output are added in a fixed tree (lanes 8o .. 8o+7
+// are consecutive), so the result does not depend on timing.
+__device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, int k, uint64_t b, int p_lo,
+                                        int p_hi, const cf *base, cf *dst, int tid)
+{
+    const int o = tid >> 3, ig = tid & 7;
+    const bool bin0 = (k == 0);
+    cf acc = make_float2(0.f, 0.f);
+    if (o < n_out) {
+        const int bm = (int)(b % (uint64_t)L.P);
+        for (int p = p_lo; p < p_hi; p++) {
+            int r = bm - p;
+            if (r < 0) r += L.P;
+            const cf *h = L.H + (((size_t)k * L.P + p) * n_out + o) * n_in;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = 4 * ig + q;
+                if (i < n_in) acc = pmac(acc, gload2(L.fdl + ((size_t)i * L.P + r) * L.N + k), gload2(h + i), bin0);
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        acc.x += __shfl_xor(acc.x, d);
+        acc.y += __shfl_xor(acc.y, d);
+    }
+    if (o < n_out && ig == 0) {
+        if (base) {
+            const cf s = base[(size_t)o * L.N + k];
+            acc.x += s.x;
+            acc.y += s.y;
+        }
+        dst[(size_t)o * L.N + k] = acc;
+    }
+}
+
+// Grid barrier number nb (1, 2, ... within one launch) of G workgroups:
+// every wave's stores complete, device-scope release, arrive, wait (bounded),
+// device-scope acquire.  Returns 0 on timeout.
+__device__ __forceinline__ int mxr_grid_sync(unsigned long long *ctr, unsigned long long nb, int G,
+                                             unsigned long long limit, int tid, int *s_ok)
+{
+    wg_stores_done();
+    if (tid == 0) {
+        __threadfence();
+        atomicAdd(ctr, 1ull);
+        const unsigned long long target = nb * (unsigned long long)G;
+        const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + limit;
+        int ok = 1;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__builtin_amdgcn_s_memrealtime() > t_end) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok;
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(MXR_NT) k_mxr_persist(MxrArgs A, const cf *__restrict__ tw)
+{
+    constexpr int N0 = 1 << LOGN;
+    extern __shared__ cf mxr_lds[];
+    __shared__ int s_ok;
+    cf *bufA = mxr_lds, *bufB = mxr_lds + N0;
+    const int tid = threadIdx.x, w = blockIdx.x, G = gridDim.x;
+    const MxrLevel &L0 = A.lv[0];
+    unsigned long long nb = 0;
+    for (long long t = 0; t < A.nticks; t++) {
+        const uint64_t f = A.f0 + (uint64_t)t;
+        // ---- release
+        if (tid == 0) {
+            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + A.spin_limit;
+            int go = 0;
+            for (;;) {
+                const unsigned long long g = __hip_atomic_load(A.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (g & PC_GO_STOP) break;
+                if (g >= f + 1) {
+                    go = 1;
+                    break;
+                }
+                if (__builtin_amdgcn_s_memrealtime() > t_end) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_ok = go;
+        }
+        __syncthreads();
+        if (!s_ok) return;
+        // ---- 1. input workgroups: frame -> ring, window [frame f-1 | frame f] -> r2c -> delay line
+        if (w < A.n_in) {
+            float *ring = A.ring + (size_t)w * 2 * A.R;
+            const float *fr = A.in + (long long)(f - A.fbase) * A.in_stride + w;
+            float *win = reinterpret_cast<float *>(bufA);
+            const long long t0 = (long long)f * N0;
+            for (int j = tid; j < N0; j += MXR_NT) {
+                const float x = fr[(long long)j * A.n_in];
+                const long long p = (t0 + j) & A.ring_mask;
+                ring[p] = x;
+                ring[p + A.R] = x;
+                win[j] = ring[(t0 - N0 + j) & A.ring_mask];
+                win[N0 + j] = x;
+            }
+            __syncthreads();
+            const cf *X = small_forward<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+            cf *row = L0.fdl + ((size_t)w * L0.P + (size_t)(f % (uint64_t)L0.P)) * N0;
+            for (int k = tid; k < N0; k += MXR_NT) row[k] = X[k];
+        }
+        if (!mxr_grid_sync(A.bar, ++nb, G, A.spin_limit, tid, &s_ok)) return;
+        // ---- 2. p = 0 term + S for every bin -> Y
+        for (int k = w; k < N0; k += G) mxr_bin(L0, A.n_in, A.n_out, k, f, 0, 1, L0.S, L0.Y, tid);
+        if (!mxr_grid_sync(A.bar, ++nb, G, A.spin_limit, tid, &s_ok)) return;
+        // ---- 3. output workgroups: c2r + the level blocks due -> output frame, flag
+        if (w < A.n_out) {
+            for (int k = tid; k < N0; k += MXR_NT) bufA[k] = L0.Y[(size_t)w * N0 + k];
+            __syncthreads();
+            const cf *z = small_inverse<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+            const float *zf = reinterpret_cast<const float *>(z);
+            const uint64_t fr = f + 1;  // strict: block fr/k - 1, sub-block fr % k
+            float *out = A.out + (long long)(f - A.fbase) * A.out_stride + w;
+            for (int j = tid; j < N0; j += MXR_NT) {
+                float y = zf[j];
+                for (int l = 1; l < A.M; l++) {
+                    const MxrLevel &L = A.lv[l];
+                    const uint64_t qb = fr >> L.kshift;
+                    if (qb == 0) continue;
+                    const uint64_t sub = fr & ((1ull << L.kshift) - 1);
+                    y += gmem(L.ybuf)[((size_t)w * 2 + (size_t)((qb - 1) & 1)) * L.N + sub * N0 + j];
+                }
+                out[(long long)j * A.n_out] = y;
+            }
+            wg_stores_done();
+            if (tid == 0)
+                __hip_atomic_store(A.flags + (size_t)w * 8, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        // ---- 4. S for tick f + 1: partitions 1 .. P-1 of every bin
+        for (int k = w; k < N0; k += G) mxr_bin(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, L0.S, tid);
+        __syncthreads();
+    }
+}
+
+// Level >= 1, block m: input workgroup i transforms its window into delay-line row m % P.
+template <int LOGN>
+__global__ void __launch_bounds__(MXR_NT) k_mxr_fwd(MxrLevel L, const float *__restrict__ ring_all, long long R,
+                                                     long long ring_mask, int N0, uint64_t m,
+                                                     const cf *__restrict__ tw)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ cf mxr_lds[];
+    cf *bufA = mxr_lds, *bufB = mxr_lds + N;
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const float *ring = ring_all + (size_t)i * 2 * R;
+    const long long s = (long long)m * N - N0 - L.T;
+    const cf *wsrc = reinterpret_cast<const cf *>(ring + (s & ring_mask));
+    for (int n = tid; n < N; n += MXR_NT) bufA[n] = wsrc[n];
+    __syncthreads();
+    const cf *X = small_forward<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+    cf *row = L.fdl + ((size_t)i * L.P + (size_t)(m % (uint64_t)L.P)) * N;
+    for (int k = tid; k < N; k += MXR_NT) row[k] = X[k];
+}
+
+// Level >= 1, block b: p = 0 term + S -> Y (one workgroup per bin).
+__global__ void __launch_bounds__(MXR_NT) k_mxr_comb(MxrLevel L, int n_in, int n_out, uint64_t b)
+{
+    mxr_bin(L, n_in, n_out, blockIdx.x, b, 0, 1, L.S, L.Y, threadIdx.x);
+}
+
+// Level >= 1, S for block b (partitions 1 .. P-1; one workgroup per bin).
+__global__ void __launch_bounds__(MXR_NT) k_mxr_part(MxrLevel L, int n_in, int n_out, uint64_t b)
+{
+    mxr_bin(L, n_in, n_out, blockIdx.x, b, 1, L.P, nullptr, L.S, threadIdx.x);
+}
+
+// Level >= 1, block m: output workgroup o: c2r of Y_o -> double-buffer half m & 1.
+template <int LOGN>
+__global__ void __launch_bounds__(MXR_NT) k_mxr_inv(MxrLevel L, uint64_t m, const cf *__restrict__ tw)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ cf mxr_lds[];
+    cf *bufA = mxr_lds, *bufB = mxr_lds + N;
+    const int o = blockIdx.x, tid = threadIdx.x;
+    for (int k = tid; k < N; k += MXR_NT) bufA[k] = L.Y[(size_t)o * N + k];
+    __syncthreads();
+    const cf *z = small_inverse<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+    const float *zf = reinterpret_cast<const float *>(z);
+    float *dst = L.ybuf + ((size_t)o * 2 + (size_t)(m & 1)) * N;
+    for (int j = tid; j < N; j += MXR_NT) dst[j] = zf[j];
+}
+
+// Filter spectra of one level: workgroup (p, pair) transforms
+// [0_N | h_pair[T + pN .. T + pN + N)] (zero past the taps), scales by 1/2N
+// and stores H[k][p][o][i] (pair = o n_in + i; h is [n_out][n_in][taps]).
+template <int LOGN>
+__global__ void __launch_bounds__(MXR_NT) k_mxr_spec(MxrLevel L, const float *__restrict__ h, int taps, int n_in,
+                                                      int n_out, const cf *__restrict__ tw)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ cf mxr_lds[];
+    cf *bufA = mxr_lds, *bufB = mxr_lds + N;
+    const int p = blockIdx.x, pair = blockIdx.y, tid = threadIdx.x;
+    const int o = pair / n_in, i = pair - o * n_in;
+    const float *hp = h + (size_t)pair * taps;
+    const long long t0 = L.T + (long long)p * N;
+    float *win = reinterpret_cast<float *>(bufA);
+    for (int n = tid; n < 2 * N; n += MXR_NT) {
+        const long long t = t0 + n - N;
+        win[n] = (n >= N && t < taps) ? hp[t] : 0.f;
+    }
+    __syncthreads();
+    const cf *X = small_forward<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+    const float sc = 1.0f / (float)(2 * N);
+    cf *H = const_cast<cf *>(L.H);
+    for (int k = tid; k < N; k += MXR_NT)
+        H[(((size_t)k * L.P + p) * n_out + o) * n_in + i] = make_float2(X[k].x * sc, X[k].y * sc);
+}

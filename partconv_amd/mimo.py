@@ -1,4 +1,5 @@
-"""MIMO throughput form on the matrix cores (pcx_* entry points).
+"""MIMO forms: the throughput form on the matrix cores (pcx_* entry points) and
+the real-time engine (pcxr_*).
 
     mx = MimoConv([256, 4096], [30, 0], taps=65536, n_in=32, n_out=32)
     mx.set_filters(h)        # h: [n_out, n_in, taps] float32 cuda
@@ -6,8 +7,16 @@
 
 y[o] = sum_i conv(x[i], h[o][i]) over whole device-resident signals: every
 frequency bin's complex n_out x n_in product, for all output blocks at once,
-runs on v_mfma_f32_32x32x2f32.  The real-time form of the same computation is
-PartConvMulti with one instance per (input, output) pair.
+runs on v_mfma_f32_32x32x2f32.
+
+    rt = MimoRealtime(48000, 256, [256, 4096], [30, 0], taps=65536, n_in=32, n_out=32)
+    rt.set_filters(h)
+    rt.run_device(y, x, nframes)   # x: [nframes*256, n_in], y: [nframes*256, n_out] cuda, interleaved
+
+runs the same network tick by tick in real time: the output equals one
+PartConvMulti instance per (input, output) pair summed per output (the
+reference's formulation), with one transform and one delay line per input
+and a per-bin matrix-vector product instead.
 """
 from __future__ import annotations
 
@@ -53,6 +62,66 @@ class MimoConv:
         try:
             if self._h:
                 lib().pcx_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class MimoRealtime:
+    def __init__(self, FS, buffer_size, N, P, taps, n_in, n_out):
+        n = len(N)
+        Na = (ctypes.c_int * n)(*[int(v) for v in N])
+        Pa = (ctypes.c_int * n)(*[int(v) for v in P])
+        self._h = lib().pcxr_create(int(FS), int(buffer_size), n, Na, Pa, int(taps), int(n_in), int(n_out))
+        if not self._h:
+            raise ValueError(last_error())
+        self.buffer_size, self.taps, self.n_in, self.n_out = int(buffer_size), int(taps), int(n_in), int(n_out)
+
+    def set_filters(self, h):
+        h = h.contiguous()
+        if tuple(h.shape) != (self.n_out, self.n_in, self.taps) or h.dtype != torch.float32 or not h.is_cuda:
+            raise ValueError("h must be float32 cuda [n_out, n_in, taps]")
+        torch.cuda.synchronize()
+        if lib().pcxr_set_filters(self._h, ctypes.c_void_p(h.data_ptr())) != 0:
+            raise RuntimeError(last_error())
+
+    def run_device(self, y, x, nframes):
+        """nframes ticks over device-resident interleaved frames (x: [nframes *
+        buffer_size, n_in], y: [nframes * buffer_size, n_out], float32 cuda)."""
+        n = int(nframes) * self.buffer_size
+        if x.shape[0] < n or x.shape[1] != self.n_in or y.shape[0] < n or y.shape[1] != self.n_out:
+            raise ValueError("frame buffers too small")
+        if not (x.is_contiguous() and y.is_contiguous()):
+            raise ValueError("frame buffers must be contiguous")
+        torch.cuda.synchronize()
+        if lib().pcxr_run_device(self._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                 int(nframes)) != 0:
+            raise RuntimeError(last_error())
+
+    def reset(self):
+        if lib().pcxr_reset(self._h) != 0:
+            raise RuntimeError(last_error())
+
+    def stats(self):
+        from ._lib import PcStats
+        s = PcStats()
+        if lib().pcxr_get_stats(self._h, ctypes.byref(s)) != 0:
+            raise RuntimeError(last_error())
+        return s
+
+    def clear_stats(self):
+        lib().pcxr_clear_stats(self._h)
+
+    def partition(self):
+        N = (ctypes.c_int * 16)()
+        P = (ctypes.c_int * 16)()
+        M = lib().pcxr_partition(self._h, N, P, 16)
+        return list(N)[:M], list(P)[:M]
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pcxr_destroy(self._h)
                 self._h = None
         except Exception:
             pass

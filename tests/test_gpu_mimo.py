@@ -29,3 +29,39 @@ def test_mimo_mfma_matches_fp64(n_in, n_out, taps, T, N, P):
         r = sum(O.direct_conv_f64(x[i], h[o, i]) for i in range(n_in))
         err = float(np.max(np.abs(y[o] - r)) / np.sqrt(np.mean(r ** 2)))
         assert err < 1e-5, (o, err)
+
+
+@pytest.mark.parametrize("n_in,n_out,taps,N,P,nfr", [
+    (5, 3, 10000, [128, 1024], [14, 0], 3 * 8 + 77),
+    (32, 32, 20000, [256, 4096], [30, 0], 3 * 16 + 7),
+])
+def test_mimo_realtime_matches_fp64(n_in, n_out, taps, N, P, nfr):
+    """Real-time MIMO engine (shared input transforms and delay lines, per-bin
+    matrix-vector products): every output against the fp64 sum of per-pair
+    convolutions (zero added latency); the ticks split over two calls; a reset
+    replays the same output bitwise."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    print(f"case mimo real time {n_in}x{n_out} taps={taps} N={N}", flush=True)
+    N0 = N[0]
+    irs = cfg4_irs(seed=7, n_in=n_in, n_out=n_out, taps=taps)
+    h = np.stack([np.stack(irs[o]) for o in range(n_out)]).astype(np.float32)
+    rng = np.random.default_rng(n_in * 10 + n_out)
+    x = rng.standard_normal((nfr * N0, n_in)).astype(np.float32)
+    rt = MimoRealtime(48000, N0, N, P, taps, n_in, n_out)
+    rt.set_filters(torch.from_numpy(h).cuda())
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros((nfr * N0, n_out), dtype=torch.float32, device="cuda")
+    a = nfr // 3
+    rt.run_device(yd[:a * N0], xd[:a * N0], a)
+    rt.run_device(yd[a * N0:], xd[a * N0:], nfr - a)
+    assert rt.stats().ticks == nfr
+    y = yd.cpu().numpy()
+    for o in range(n_out):
+        r = sum(O.direct_conv_f64(x[:, i], h[o, i]) for i in range(n_in))[:nfr * N0]
+        err = float(np.max(np.abs(y[:, o] - r)) / np.sqrt(np.mean(r ** 2)))
+        assert err < 1e-5, (o, err)
+    rt.reset()
+    y2 = torch.zeros_like(yd)
+    rt.run_device(y2, xd, nfr)
+    assert np.array_equal(y2.cpu().numpy(), y)
