@@ -3205,14 +3205,21 @@ static int mxr_setup(pc_mimo_rt *r)
         L.P = r->P[l];
         L.kshift = ilog2(r->N[l] / N0);
         L.T = r->T[l];
+        L.ni = (r->n_in + 3) & ~3;
         const size_t nb = (size_t)L.N * sizeof(cf);
         cf *H = nullptr;
         if ((rc = mxr_alloc(r, (void **)&L.fdl, (size_t)r->n_in * L.P * nb))) return rc;
-        if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * r->n_in * nb))) return rc;
+        if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * L.ni * nb))) return rc;
         L.H = H;
         if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)r->n_out * nb))) return rc;
         if ((rc = mxr_alloc(r, (void **)&L.Y, (size_t)r->n_out * nb))) return rc;
         if (l > 0 && (rc = mxr_alloc(r, (void **)&L.ybuf, (size_t)r->n_out * 2 * L.N * sizeof(float)))) return rc;
+    }
+    {
+        MxrLevel *d_lv = nullptr;
+        if ((rc = mxr_alloc(r, (void **)&d_lv, (size_t)M * sizeof(MxrLevel)))) return rc;
+        HIPCHK(hipMemcpy(d_lv, A.lv, (size_t)M * sizeof(MxrLevel), hipMemcpyHostToDevice));
+        A.lvd = d_lv;
     }
     if ((rc = mxr_alloc(r, (void **)&r->d_bar, 64))) return rc;
     HIPCHK(hipStreamCreateWithFlags(&r->s_main, hipStreamNonBlocking));
@@ -3354,7 +3361,7 @@ static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
     MxrArgs a = r->A;
     const cf *t = r->tw;
     void *args[] = {(void *)&a, (void *)&t};
-    return hipLaunchCooperativeKernel((const void *)k_mxr_persist<LOGN>, dim3(G), dim3(MXR_NT), args,
+    return hipLaunchCooperativeKernel((const void *)k_mxr_persist<LOGN>, dim3(G), dim3(MXR_NTT), args,
                                       (unsigned)(2 * (1 << LOGN) * sizeof(cf)), r->s_main);
 }
 

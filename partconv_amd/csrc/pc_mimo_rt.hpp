@@ -16,28 +16,33 @@
 // Level 0 runs in one persistent cooperative kernel (k_mxr_persist), per tick:
 //   1. release (host word), input workgroups: frame -> ring, window -> r2c ->
 //      level-0 delay line of that input;
-//   2. grid barrier; every workgroup: p = 0 product of its bins -> Y;
-//   3. grid barrier; output workgroups: c2r of Y_o + the level blocks due ->
-//      output frame, completion flag;
-//   4. every workgroup: S for the next tick (its bins), streaming the
-//      filter spectra -- overlapped with the host's round trip.
+//   2. grid barrier; output workgroups: Y_o = S_o + p = 0 product over every
+//      bin, c2r + the level blocks due -> output frame, completion flag (one
+//      barrier per tick: a second one, after a p = 0 product spread over all
+//      workgroups, measured slower);
+//   3. every workgroup: S for the next tick (its bins), streaming the filter
+//      spectra -- overlapped with the host's round trip.
 // Levels >= 1 are host-launched per block on their own streams: k_mxr_fwd
 // (one workgroup per input), k_mxr_comb (per bin), k_mxr_inv (per output),
 // k_mxr_part (per bin, the next block's S).
 //
 // Layouts (device, bins in natural order, packed bin 0 = (X_0, X_N)):
 //   ring[i]   : 2R floats (doubled ring, as the instance engine's)
-//   fdl       : [n_in][P][N]     H : [N][P][n_out][n_in] (scaled by 1/2N)
+//   fdl       : [n_in][P][N]     H : [N][P][n_out][ni] (scaled by 1/2N; ni = n_in
+//               rounded up to 4, zero-padded, so a lane's four filters are one
+//               aligned 32-byte load)
 //   S, Y      : [n_out][N]       ybuf : [n_out][2][N] (levels >= 1)
 // Frames in and out are interleaved as PartConvMulti::run's buffers.
 #pragma once
 #include "pc_kernels.hpp"
 
-#define MXR_NT 256          // threads of every real-time MIMO kernel
-#define MXR_MAX_IO 32       // inputs and outputs: lane t -> output t/8, inputs 4(t%8) .. +3
+#define MXR_NT 256          // threads of the level kernels
+#define MXR_NTT 256         // threads of the persistent tick kernel's workgroups
+#define MXR_MAX_IO 32       // inputs and outputs (32 outputs x NT/32 lanes per bin)
 
 struct MxrLevel {
     int N, P, kshift;
+    int ni;                 // H row stride: n_in rounded up to 4 (zero-padded)
     long long T;            // taps before this level
     cf *fdl;
     const cf *H;
@@ -51,6 +56,9 @@ struct MxrArgs {
     float *ring;            // [n_in][2R]
     long long R, ring_mask;
     MxrLevel lv[PC_MAX_LEVELS];
+    const MxrLevel *lvd;    // the same levels in device memory: the kernel indexes
+                            // levels >= 1 with a loop variable, which on a by-value
+                            // argument array would move the array into registers
     const float *in;
     long long in_stride;
     float *out;
@@ -66,39 +74,67 @@ struct MxrArgs {
 
 // Bin k of level L, for every output: the sum over partitions p in [p_lo, p_hi)
 // and inputs of H_p[o][i][k] X_i^(b-p)[k], plus base[o][k] if given, into
-// dst[o][k].  Lane t handles output t/8 and inputs 4(t%8) .. 4(t%8)+3; the
-// eight partial sums of an output are added in a fixed tree (lanes 8o .. 8o+7
-// are consecutive), so the result does not depend on timing.
+// dst[o][k].  With NT threads, NT/32 lanes share an output (lane t: output
+// t / (NT/32)) and each takes II = 1024/NT consecutive inputs.  The loads of
+// PB partitions are issued together before they are summed (in partition,
+// then input order): the loop is otherwise one memory round trip per
+// partition.  The partial sums of an output are added in a fixed tree over
+// its consecutive lanes, so the result does not depend on timing.
+template <int NT, int PB>
 __device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, int k, uint64_t b, int p_lo,
                                         int p_hi, const cf *base, cf *dst, int tid)
 {
-    const int o = tid >> 3, ig = tid & 7;
+    constexpr int LPO = NT / 32, II = 32 / LPO;  // lanes per output, inputs per lane
+    static_assert(II == 2 || II == 4, "256 or 512 threads");
+    const int o = tid / LPO, ig = tid % LPO;
     const bool bin0 = (k == 0);
+    const bool act = (o < n_out) && (II * ig < n_in);
     cf acc = make_float2(0.f, 0.f);
-    if (o < n_out) {
+    cf s0 = make_float2(0.f, 0.f);
+    if (base && o < n_out && ig == 0) s0 = gload2(base + (size_t)o * L.N + k);
+    if (act) {
         const int bm = (int)(b % (uint64_t)L.P);
-        for (int p = p_lo; p < p_hi; p++) {
-            int r = bm - p;
-            if (r < 0) r += L.P;
-            const cf *h = L.H + (((size_t)k * L.P + p) * n_out + o) * n_in;
+        for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
+            float4 hv[PB][II / 2];
+            cf xv[PB][II];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int i = 4 * ig + q;
-                if (i < n_in) acc = pmac(acc, gload2(L.fdl + ((size_t)i * L.P + r) * L.N + k), gload2(h + i), bin0);
+            for (int j = 0; j < PB; j++) {
+                const int p = p0 + j;
+                if (p < p_hi) {
+                    int r = bm - p;
+                    if (r < 0) r += L.P;
+                    const float4 *h = reinterpret_cast<const float4 *>(
+                        L.H + (((size_t)k * L.P + p) * n_out + o) * L.ni + II * ig);
+#pragma unroll
+                    for (int q = 0; q < II / 2; q++) hv[j][q] = h[q];
+#pragma unroll
+                    for (int q = 0; q < II; q++) {
+                        const int i = II * ig + q;
+                        xv[j][q] = (i < n_in) ? gload2(L.fdl + ((size_t)i * L.P + r) * L.N + k)
+                                              : make_float2(0.f, 0.f);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PB; j++) {
+                if (p0 + j < p_hi) {
+#pragma unroll
+                    for (int q = 0; q < II / 2; q++) {
+                        acc = pmac(acc, xv[j][2 * q], make_float2(hv[j][q].x, hv[j][q].y), bin0);
+                        acc = pmac(acc, xv[j][2 * q + 1], make_float2(hv[j][q].z, hv[j][q].w), bin0);
+                    }
+                }
             }
         }
     }
 #pragma unroll
-    for (int d = 1; d < 8; d <<= 1) {
+    for (int d = 1; d < LPO; d <<= 1) {
         acc.x += __shfl_xor(acc.x, d);
         acc.y += __shfl_xor(acc.y, d);
     }
     if (o < n_out && ig == 0) {
-        if (base) {
-            const cf s = base[(size_t)o * L.N + k];
-            acc.x += s.x;
-            acc.y += s.y;
-        }
+        acc.x += s0.x;
+        acc.y += s0.y;
         dst[(size_t)o * L.N + k] = acc;
     }
 }
@@ -131,7 +167,7 @@ __device__ __forceinline__ int mxr_grid_sync(unsigned long long *ctr, unsigned l
 }
 
 template <int LOGN>
-__global__ void __launch_bounds__(MXR_NT) k_mxr_persist(MxrArgs A, const cf *__restrict__ tw)
+__global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__restrict__ tw)
 {
     constexpr int N0 = 1 << LOGN;
     extern __shared__ cf mxr_lds[];
@@ -166,7 +202,7 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_persist(MxrArgs A, const cf *__r
             const float *fr = A.in + (long long)(f - A.fbase) * A.in_stride + w;
             float *win = reinterpret_cast<float *>(bufA);
             const long long t0 = (long long)f * N0;
-            for (int j = tid; j < N0; j += MXR_NT) {
+            for (int j = tid; j < N0; j += MXR_NTT) {
                 const float x = fr[(long long)j * A.n_in];
                 const long long p = (t0 + j) & A.ring_mask;
                 ring[p] = x;
@@ -175,26 +211,58 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_persist(MxrArgs A, const cf *__r
                 win[N0 + j] = x;
             }
             __syncthreads();
-            const cf *X = small_forward<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+            const cf *X = small_forward<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
             cf *row = L0.fdl + ((size_t)w * L0.P + (size_t)(f % (uint64_t)L0.P)) * N0;
-            for (int k = tid; k < N0; k += MXR_NT) row[k] = X[k];
+            for (int k = tid; k < N0; k += MXR_NTT) row[k] = X[k];
         }
         if (!mxr_grid_sync(A.bar, ++nb, G, A.spin_limit, tid, &s_ok)) return;
-        // ---- 2. p = 0 term + S for every bin -> Y
-        for (int k = w; k < N0; k += G) mxr_bin(L0, A.n_in, A.n_out, k, f, 0, 1, L0.S, L0.Y, tid);
-        if (!mxr_grid_sync(A.bar, ++nb, G, A.spin_limit, tid, &s_ok)) return;
-        // ---- 3. output workgroups: c2r + the level blocks due -> output frame, flag
+        // ---- 2. output workgroups: Y_o = S_o + sum_i H_0[o][i] X_i (every bin,
+        //         inputs in order), c2r + the level blocks due -> output frame, flag
         if (w < A.n_out) {
-            for (int k = tid; k < N0; k += MXR_NT) bufA[k] = L0.Y[(size_t)w * N0 + k];
+            const int r0 = (int)(f % (uint64_t)L0.P);
+            // HV thread groups per bin, each a consecutive share of the inputs;
+            // group 0 adds the other groups' sums (fixed order) through LDS
+            constexpr int HV = (MXR_NTT >= 2 * N0) ? 2 : 1;
+            const int hv_g = tid / (MXR_NTT / HV), kt = tid % (MXR_NTT / HV);
+            const int ihalf = (A.n_in + HV - 1) / HV;
+            const int i_lo = hv_g * ihalf, i_hi = min(A.n_in, i_lo + ihalf);
+            for (int k = kt; k < N0; k += MXR_NTT / HV) {
+                const bool bin0 = (k == 0);
+                const cf *h = L0.H + ((size_t)k * L0.P * A.n_out + w) * L0.ni;  // partition 0
+                cf acc = (hv_g == 0) ? gload2(L0.S + (size_t)w * N0 + k) : make_float2(0.f, 0.f);
+                for (int i0 = i_lo; i0 < i_hi; i0 += 8) {
+                    cf hv[8], xv[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int i = i0 + q;
+                        if (i < i_hi) {
+                            hv[q] = gload2(h + i);
+                            xv[q] = gload2(L0.fdl + ((size_t)i * L0.P + r0) * N0 + k);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        if (i0 + q < i_hi) acc = pmac(acc, xv[q], hv[q], bin0);
+                }
+                if (hv_g == 0) bufA[k] = acc;
+                else bufB[k] = acc;
+            }
             __syncthreads();
-            const cf *z = small_inverse<LOGN, MXR_NT>(bufA, bufB, tw, tid);
+            if constexpr (HV == 2) {
+                for (int k = tid; k < N0; k += MXR_NTT) {
+                    bufA[k].x += bufB[k].x;
+                    bufA[k].y += bufB[k].y;
+                }
+                __syncthreads();
+            }
+            const cf *z = small_inverse<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
             const float *zf = reinterpret_cast<const float *>(z);
             const uint64_t fr = f + 1;  // strict: block fr/k - 1, sub-block fr % k
             float *out = A.out + (long long)(f - A.fbase) * A.out_stride + w;
-            for (int j = tid; j < N0; j += MXR_NT) {
+            for (int j = tid; j < N0; j += MXR_NTT) {
                 float y = zf[j];
                 for (int l = 1; l < A.M; l++) {
-                    const MxrLevel &L = A.lv[l];
+                    const MxrLevel &L = A.lvd[l];
                     const uint64_t qb = fr >> L.kshift;
                     if (qb == 0) continue;
                     const uint64_t sub = fr & ((1ull << L.kshift) - 1);
@@ -207,8 +275,8 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_persist(MxrArgs A, const cf *__r
                 __hip_atomic_store(A.flags + (size_t)w * 8, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
         }
-        // ---- 4. S for tick f + 1: partitions 1 .. P-1 of every bin
-        for (int k = w; k < N0; k += G) mxr_bin(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, L0.S, tid);
+        // ---- 3. S for tick f + 1: partitions 1 .. P-1 of every bin
+        for (int k = w; k < N0; k += G) mxr_bin<MXR_NTT, 8>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, L0.S, tid);
         __syncthreads();
     }
 }
@@ -236,13 +304,13 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_fwd(MxrLevel L, const float *__r
 // Level >= 1, block b: p = 0 term + S -> Y (one workgroup per bin).
 __global__ void __launch_bounds__(MXR_NT) k_mxr_comb(MxrLevel L, int n_in, int n_out, uint64_t b)
 {
-    mxr_bin(L, n_in, n_out, blockIdx.x, b, 0, 1, L.S, L.Y, threadIdx.x);
+    mxr_bin<MXR_NT, 8>(L, n_in, n_out, blockIdx.x, b, 0, 1, L.S, L.Y, threadIdx.x);
 }
 
 // Level >= 1, S for block b (partitions 1 .. P-1; one workgroup per bin).
 __global__ void __launch_bounds__(MXR_NT) k_mxr_part(MxrLevel L, int n_in, int n_out, uint64_t b)
 {
-    mxr_bin(L, n_in, n_out, blockIdx.x, b, 1, L.P, nullptr, L.S, threadIdx.x);
+    mxr_bin<MXR_NT, 8>(L, n_in, n_out, blockIdx.x, b, 1, L.P, nullptr, L.S, threadIdx.x);
 }
 
 // Level >= 1, block m: output workgroup o: c2r of Y_o -> double-buffer half m & 1.
@@ -285,5 +353,5 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_spec(MxrLevel L, const float *__
     const float sc = 1.0f / (float)(2 * N);
     cf *H = const_cast<cf *>(L.H);
     for (int k = tid; k < N; k += MXR_NT)
-        H[(((size_t)k * L.P + p) * n_out + o) * n_in + i] = make_float2(X[k].x * sc, X[k].y * sc);
+        H[(((size_t)k * L.P + p) * n_out + o) * L.ni + i] = make_float2(X[k].x * sc, X[k].y * sc);
 }
