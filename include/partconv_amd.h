@@ -364,6 +364,12 @@ int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st);
 int pcxr_clear_stats(pc_mimo_rt *r);
 /* partition after the last count is recounted; returns the level count */
 int pcxr_partition(const pc_mimo_rt *r, int *N, int *P, int cap);
+/* Time the delay-line-sum kernel of level `level` (>= 1; -1 disables) with HIP
+ * events (up to 64 launches); read back the mean duration, launch count and
+ * algorithmic bytes per launch (filter rows p >= 1 of every pair, delay-line
+ * rows of every input, one S row per output). */
+int pcxr_probe(pc_mimo_rt *r, int level);
+int pcxr_probe_read(pc_mimo_rt *r, double *avg_us, long long *launches, double *bytes_per_launch);
 void pcxr_destroy(pc_mimo_rt *r);
 
 #ifdef __cplusplus

@@ -114,6 +114,9 @@ SIGNATURES = {
     "pcxr_clear_stats": (ctypes.c_int, [_vp]),
     "pcxr_partition": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int]),
     "pcxr_destroy": (None, [_vp]),
+    "pcxr_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcxr_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
+                                       ctypes.POINTER(ctypes.c_double)]),
 }
 
 

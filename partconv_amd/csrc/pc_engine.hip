@@ -3144,10 +3144,19 @@ struct pc_mimo_rt {
     unsigned long long ticks = 0, misses = 0;
     double lat_sum = 0, lat_max = 0;
     bool filters = false;
+    // probe: HIP events around the delay-line-sum kernel of one level
+    int probe_level = -1;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pev;
+    int pev_used = 0;
 };
 
 static void mxr_free(pc_mimo_rt *r)
 {
+    for (auto &e : r->pev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    r->pev.clear();
     if (r->s_main) hipStreamSynchronize(r->s_main);
     for (hipStream_t s : r->s_lvl)
         if (s) hipStreamSynchronize(s);
@@ -3333,7 +3342,13 @@ static void mxr_level_t(pc_mimo_rt *r, int l, uint64_t m)
                        r->N0, m, r->tw);
     hipLaunchKernelGGL(k_mxr_comb, dim3(L.N), dim3(MXR_NT), 0, s, L, r->n_in, r->n_out, m);
     hipLaunchKernelGGL(k_mxr_inv<LOGN>, dim3(r->n_out), dim3(MXR_NT), lds, s, L, m, r->tw);
-    hipLaunchKernelGGL(k_mxr_part, dim3(L.N), dim3(MXR_NT), 0, s, L, r->n_in, r->n_out, m + 1);
+    if (l == r->probe_level && r->pev_used < (int)r->pev.size()) {
+        const auto &e = r->pev[r->pev_used++];
+        hipExtLaunchKernelGGL(k_mxr_part, dim3(L.N), dim3(MXR_NT), 0, s, e.first, e.second, 0, L, r->n_in,
+                              r->n_out, (uint64_t)(m + 1));
+    } else {
+        hipLaunchKernelGGL(k_mxr_part, dim3(L.N), dim3(MXR_NT), 0, s, L, r->n_in, r->n_out, m + 1);
+    }
 }
 
 // Level l's block that starts at tick f (after that tick's frame is in the ring).
@@ -3501,6 +3516,52 @@ PC_API int pcxr_clear_stats(pc_mimo_rt *r)
     r->lat.clear();
     r->ticks = r->misses = 0;
     r->lat_sum = r->lat_max = 0;
+    return PC_OK;
+}
+
+// Time the delay-line-sum kernel (k_mxr_part) of level `level` (>= 1) with HIP
+// events from now on (up to 64 launches); -1 disables.
+PC_API int pcxr_probe(pc_mimo_rt *r, int level)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    if (level >= r->M || level == 0) return fail(PC_ERR_ARG, "level out of range");
+    for (int l = 1; l < r->M; l++) HIPCHK(hipStreamSynchronize(r->s_lvl[l]));
+    if (r->pev.empty())
+        for (int q = 0; q < 64; q++) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            r->pev.push_back({a, b});
+        }
+    r->pev_used = 0;
+    r->probe_level = level;
+    return PC_OK;
+}
+
+// Mean duration, launch count and algorithmic bytes per launch of the probed
+// kernel: the P-1 filter-spectrum rows of every pair, the P-1 delay-line rows
+// of every input, and the S row of every output.
+PC_API int pcxr_probe_read(pc_mimo_rt *r, double *avg_us, long long *launches, double *bytes_per_launch)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    double ms = 0;
+    for (int q = 0; q < r->pev_used; q++) {
+        HIPCHK(hipEventSynchronize(r->pev[q].second));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r->pev[q].first, r->pev[q].second));
+        ms += t;
+    }
+    if (avg_us) *avg_us = r->pev_used ? 1e3 * ms / r->pev_used : 0.0;
+    if (launches) *launches = r->pev_used;
+    if (bytes_per_launch) {
+        double b = 0;
+        if (r->probe_level > 0) {
+            const int l = r->probe_level;
+            const double N = r->N[l], P = r->P[l];
+            b = 8.0 * N * (P - 1) * ((double)r->n_out * r->n_in + r->n_in) + 8.0 * N * r->n_out;
+        }
+        *bytes_per_launch = b;
+    }
     return PC_OK;
 }
 

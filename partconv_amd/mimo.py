@@ -112,6 +112,19 @@ class MimoRealtime:
     def clear_stats(self):
         lib().pcxr_clear_stats(self._h)
 
+    def probe(self, level):
+        """Time the delay-line-sum kernel of one level (>= 1; -1 disables)."""
+        if lib().pcxr_probe(self._h, int(level)) != 0:
+            raise RuntimeError(last_error())
+
+    def probe_read(self):
+        """(avg_us, launches, algorithmic_bytes_per_launch) of the probed kernel."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_longlong()
+        if lib().pcxr_probe_read(self._h, ctypes.byref(a), ctypes.byref(n), ctypes.byref(b)) != 0:
+            raise RuntimeError(last_error())
+        return a.value, n.value, b.value
+
     def partition(self):
         N = (ctypes.c_int * 16)()
         P = (ctypes.c_int * 16)()
