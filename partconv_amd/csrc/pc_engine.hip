@@ -3220,7 +3220,8 @@ static int mxr_setup(pc_mimo_rt *r)
         if ((rc = mxr_alloc(r, (void **)&L.fdl, (size_t)r->n_in * L.P * nb))) return rc;
         if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * L.ni * nb))) return rc;
         L.H = H;
-        if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)r->n_out * nb))) return rc;
+        // level 0's S twice: the persistent kernel double-buffers it by tick parity
+        if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)(l == 0 ? 2 : 1) * r->n_out * nb))) return rc;
         if ((rc = mxr_alloc(r, (void **)&L.Y, (size_t)r->n_out * nb))) return rc;
         if (l > 0 && (rc = mxr_alloc(r, (void **)&L.ybuf, (size_t)r->n_out * 2 * L.N * sizeof(float)))) return rc;
     }
@@ -3230,7 +3231,7 @@ static int mxr_setup(pc_mimo_rt *r)
         HIPCHK(hipMemcpy(d_lv, A.lv, (size_t)M * sizeof(MxrLevel), hipMemcpyHostToDevice));
         A.lvd = d_lv;
     }
-    if ((rc = mxr_alloc(r, (void **)&r->d_bar, 64))) return rc;
+    if ((rc = mxr_alloc(r, (void **)&r->d_bar, 128))) return rc;  // two counters, own lines
     HIPCHK(hipStreamCreateWithFlags(&r->s_main, hipStreamNonBlocking));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, r->dev));
@@ -3376,8 +3377,19 @@ static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
     MxrArgs a = r->A;
     const cf *t = r->tw;
     void *args[] = {(void *)&a, (void *)&t};
+    // two FFT buffers, plus the output workgroups' filter cache when it leaves
+    // room for two workgroups per CU
+    const size_t fft = 2 * (size_t)(1 << LOGN) * sizeof(cf);
+    const size_t hc = (size_t)(1 << LOGN) * r->n_in * sizeof(cf);
+    a.hcache = (fft + hc <= 80 * 1024) ? 1 : 0;
+    const size_t lds = fft + (a.hcache ? hc : 0);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_mxr_persist<LOGN>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     return hipLaunchCooperativeKernel((const void *)k_mxr_persist<LOGN>, dim3(G), dim3(MXR_NTT), args,
-                                      (unsigned)(2 * (1 << LOGN) * sizeof(cf)), r->s_main);
+                                      (unsigned)lds, r->s_main);
 }
 
 static void mxr_store_go(pc_mimo_rt *r, unsigned long long v)
@@ -3407,9 +3419,10 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     A.fbase = (long long)r->frame;
     A.f0 = r->frame;
     A.nticks = nframes;
-    // the grid: at least every input and output, at most one workgroup per bin
-    const int G = std::max(std::max(r->n_in, r->n_out), std::min(N0, 256));
-    HIPCHK(hipMemsetAsync(r->d_bar, 0, 64, r->s_main));
+    // the grid: input and output workgroups, then one delay-line-sum workgroup
+    // per bin (at most 256)
+    const int G = r->n_in + r->n_out + std::min(N0, 256);
+    HIPCHK(hipMemsetAsync(r->d_bar, 0, 128, r->s_main));
     mxr_store_go(r, r->frame);
     hipError_t le = hipErrorInvalidValue;
     switch (ilog2(N0)) {
@@ -3481,7 +3494,7 @@ PC_API int pcxr_reset(pc_mimo_rt *r)
         const MxrLevel &L = A.lv[l];
         const size_t nb = (size_t)L.N * sizeof(cf);
         HIPCHK(hipMemsetAsync(L.fdl, 0, (size_t)r->n_in * L.P * nb, r->s_main));
-        HIPCHK(hipMemsetAsync(L.S, 0, (size_t)r->n_out * nb, r->s_main));
+        HIPCHK(hipMemsetAsync(L.S, 0, (size_t)(l == 0 ? 2 : 1) * r->n_out * nb, r->s_main));
         if (L.ybuf) HIPCHK(hipMemsetAsync(L.ybuf, 0, (size_t)r->n_out * 2 * L.N * sizeof(float), r->s_main));
     }
     HIPCHK(hipStreamSynchronize(r->s_main));

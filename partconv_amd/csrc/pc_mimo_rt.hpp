@@ -13,15 +13,9 @@
 // the instance engine: Y = (p = 0 term) + S, S = sum_{p>=1} precomputed after
 // block m-1, off the path from a block's input to its output.
 //
-// Level 0 runs in one persistent cooperative kernel (k_mxr_persist), per tick:
-//   1. release (host word), input workgroups: frame -> ring, window -> r2c ->
-//      level-0 delay line of that input;
-//   2. grid barrier; output workgroups: Y_o = S_o + p = 0 product over every
-//      bin, c2r + the level blocks due -> output frame, completion flag (one
-//      barrier per tick: a second one, after a p = 0 product spread over all
-//      workgroups, measured slower);
-//   3. every workgroup: S for the next tick (its bins), streaming the filter
-//      spectra -- overlapped with the host's round trip.
+// Level 0 runs in one persistent cooperative kernel (k_mxr_persist) whose
+// workgroups have roles -- input, output, delay-line sum -- and synchronise
+// through two monotonic counters instead of grid barriers (see the kernel).
 // Levels >= 1 are host-launched per block on their own streams: k_mxr_fwd
 // (one workgroup per input), k_mxr_comb (per bin), k_mxr_inv (per output),
 // k_mxr_part (per bin, the next block's S).
@@ -31,7 +25,7 @@
 //   fdl       : [n_in][P][N]     H : [N][P][n_out][ni] (scaled by 1/2N; ni = n_in
 //               rounded up to 4, zero-padded, so a lane's four filters are one
 //               aligned 32-byte load)
-//   S, Y      : [n_out][N]       ybuf : [n_out][2][N] (levels >= 1)
+//   S, Y      : [n_out][N] (level 0: S twice, by tick parity)   ybuf : [n_out][2][N] (levels >= 1)
 // Frames in and out are interleaved as PartConvMulti::run's buffers.
 #pragma once
 #include "pc_kernels.hpp"
@@ -70,6 +64,7 @@ struct MxrArgs {
     unsigned long long *flags;  // host-visible, one per output workgroup, 8 words apart
     unsigned long long *bar;    // grid barrier counter (zeroed before each launch)
     unsigned long long spin_limit;
+    int hcache;             // output workgroups keep H_0[o] in LDS (after the two FFT buffers)
 };
 
 // Bin k of level L, for every output: the sum over partitions p in [p_lo, p_hi)
@@ -139,17 +134,12 @@ __device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, 
     }
 }
 
-// Grid barrier number nb (1, 2, ... within one launch) of G workgroups:
-// every wave's stores complete, device-scope release, arrive, wait (bounded),
+// Workgroup-wide bounded wait until *ctr >= target (thread 0 polls), then a
 // device-scope acquire.  Returns 0 on timeout.
-__device__ __forceinline__ int mxr_grid_sync(unsigned long long *ctr, unsigned long long nb, int G,
-                                             unsigned long long limit, int tid, int *s_ok)
+__device__ __forceinline__ int mxr_wait(const unsigned long long *ctr, unsigned long long target,
+                                        unsigned long long limit, int tid, int *s_ok)
 {
-    wg_stores_done();
     if (tid == 0) {
-        __threadfence();
-        atomicAdd(ctr, 1ull);
-        const unsigned long long target = nb * (unsigned long long)G;
         const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + limit;
         int ok = 1;
         while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -166,6 +156,31 @@ __device__ __forceinline__ int mxr_grid_sync(unsigned long long *ctr, unsigned l
     return *s_ok;
 }
 
+// Every wave's stores complete, device-scope release, then count one arrival.
+__device__ __forceinline__ void mxr_arrive(unsigned long long *ctr, int tid)
+{
+    wg_stores_done();
+    if (tid == 0) {
+        __threadfence();
+        atomicAdd(ctr, 1ull);
+    }
+}
+
+// Workgroups by role: [0, n_in) input, [n_in, n_in + n_out) output, the rest
+// (G4 of them) the delay-line sum.  Per tick t of the launch (f = f0 + t):
+//   input:  release; frame -> ring, window -> r2c -> delay-line row f % P;
+//           arrive at cX;
+//   output: release; wait cX >= (t+1) n_in (this tick's rows) and
+//           cS >= t G4 (S for tick f complete); Y_o = S_o + p = 0 product
+//           over every bin, c2r + the level blocks due -> output, flag;
+//   sum:    release; wait cX >= (t+1) n_in; S for tick f + 1 of its bins;
+//           arrive at cS.
+// S is double-buffered by tick parity (tick f reads S[f & 1]; its sum writes
+// S[(f+1) & 1]), so the sum for the next tick runs while this tick's outputs
+// are formed and while the host's round trip and the next tick's input
+// transforms proceed: the delay-line sum is off the critical path.  (A grid
+// barrier per tick instead, with every workgroup summing after it, measured
+// 43 us per tick against 32 x 32 channels.)
 template <int LOGN>
 __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__restrict__ tw)
 {
@@ -173,9 +188,21 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
     extern __shared__ cf mxr_lds[];
     __shared__ int s_ok;
     cf *bufA = mxr_lds, *bufB = mxr_lds + N0;
-    const int tid = threadIdx.x, w = blockIdx.x, G = gridDim.x;
+    const int tid = threadIdx.x, w = blockIdx.x;
+    const int n_io = A.n_in + A.n_out, G4 = (int)gridDim.x - n_io;
     const MxrLevel &L0 = A.lv[0];
-    unsigned long long nb = 0;
+    unsigned long long *cX = A.bar, *cS = A.bar + 8;
+    // output workgroups: their partition-0 filter spectra, input-major
+    // (Hc[i N0 + k]: lanes read consecutive words), loaded once per launch
+    cf *Hc = mxr_lds + 2 * N0;
+    if (A.hcache && w >= A.n_in && w < n_io) {
+        const int o = w - A.n_in;
+        for (int e = tid; e < N0 * A.n_in; e += MXR_NTT) {
+            const int k = e / A.n_in, i = e - k * A.n_in;
+            Hc[(size_t)i * N0 + k] = L0.H[((size_t)k * L0.P * A.n_out + o) * L0.ni + i];
+        }
+        __syncthreads();
+    }
     for (long long t = 0; t < A.nticks; t++) {
         const uint64_t f = A.f0 + (uint64_t)t;
         // ---- release
@@ -196,8 +223,8 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
         }
         __syncthreads();
         if (!s_ok) return;
-        // ---- 1. input workgroups: frame -> ring, window [frame f-1 | frame f] -> r2c -> delay line
         if (w < A.n_in) {
+            // ---- input: frame -> ring, window [frame f-1 | frame f] -> r2c -> delay line
             float *ring = A.ring + (size_t)w * 2 * A.R;
             const float *fr = A.in + (long long)(f - A.fbase) * A.in_stride + w;
             float *win = reinterpret_cast<float *>(bufA);
@@ -214,51 +241,41 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
             const cf *X = small_forward<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
             cf *row = L0.fdl + ((size_t)w * L0.P + (size_t)(f % (uint64_t)L0.P)) * N0;
             for (int k = tid; k < N0; k += MXR_NTT) row[k] = X[k];
-        }
-        if (!mxr_grid_sync(A.bar, ++nb, G, A.spin_limit, tid, &s_ok)) return;
-        // ---- 2. output workgroups: Y_o = S_o + sum_i H_0[o][i] X_i (every bin,
-        //         inputs in order), c2r + the level blocks due -> output frame, flag
-        if (w < A.n_out) {
+            mxr_arrive(cX, tid);
+        } else if (w < n_io) {
+            // ---- output o: Y_o = S_o + sum_i H_0[o][i] X_i (inputs in order), c2r + levels
+            const int o = w - A.n_in;
+            if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
+            if (!mxr_wait(cS, (unsigned long long)t * G4, A.spin_limit, tid, &s_ok)) return;
+            const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0;
             const int r0 = (int)(f % (uint64_t)L0.P);
-            // HV thread groups per bin, each a consecutive share of the inputs;
-            // group 0 adds the other groups' sums (fixed order) through LDS
-            constexpr int HV = (MXR_NTT >= 2 * N0) ? 2 : 1;
-            const int hv_g = tid / (MXR_NTT / HV), kt = tid % (MXR_NTT / HV);
-            const int ihalf = (A.n_in + HV - 1) / HV;
-            const int i_lo = hv_g * ihalf, i_hi = min(A.n_in, i_lo + ihalf);
-            for (int k = kt; k < N0; k += MXR_NTT / HV) {
+            for (int k = tid; k < N0; k += MXR_NTT) {
                 const bool bin0 = (k == 0);
-                const cf *h = L0.H + ((size_t)k * L0.P * A.n_out + w) * L0.ni;  // partition 0
-                cf acc = (hv_g == 0) ? gload2(L0.S + (size_t)w * N0 + k) : make_float2(0.f, 0.f);
-                for (int i0 = i_lo; i0 < i_hi; i0 += 8) {
+                const cf *h = L0.H + ((size_t)k * L0.P * A.n_out + o) * L0.ni;  // partition 0
+                cf acc = gload2(S + (size_t)o * N0 + k);
+                for (int i0 = 0; i0 < A.n_in; i0 += 8) {
                     cf hv[8], xv[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         const int i = i0 + q;
-                        if (i < i_hi) {
-                            hv[q] = gload2(h + i);
+                        if (i < A.n_in) {
                             xv[q] = gload2(L0.fdl + ((size_t)i * L0.P + r0) * N0 + k);
+                            if (!A.hcache) hv[q] = gload2(h + i);
                         }
                     }
 #pragma unroll
-                    for (int q = 0; q < 8; q++)
-                        if (i0 + q < i_hi) acc = pmac(acc, xv[q], hv[q], bin0);
+                    for (int q = 0; q < 8; q++) {
+                        const int i = i0 + q;
+                        if (i < A.n_in) acc = pmac(acc, xv[q], A.hcache ? Hc[(size_t)i * N0 + k] : hv[q], bin0);
+                    }
                 }
-                if (hv_g == 0) bufA[k] = acc;
-                else bufB[k] = acc;
+                bufA[k] = acc;
             }
             __syncthreads();
-            if constexpr (HV == 2) {
-                for (int k = tid; k < N0; k += MXR_NTT) {
-                    bufA[k].x += bufB[k].x;
-                    bufA[k].y += bufB[k].y;
-                }
-                __syncthreads();
-            }
             const cf *z = small_inverse<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
             const float *zf = reinterpret_cast<const float *>(z);
             const uint64_t fr = f + 1;  // strict: block fr/k - 1, sub-block fr % k
-            float *out = A.out + (long long)(f - A.fbase) * A.out_stride + w;
+            float *out = A.out + (long long)(f - A.fbase) * A.out_stride + o;
             for (int j = tid; j < N0; j += MXR_NTT) {
                 float y = zf[j];
                 for (int l = 1; l < A.M; l++) {
@@ -266,17 +283,22 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                     const uint64_t qb = fr >> L.kshift;
                     if (qb == 0) continue;
                     const uint64_t sub = fr & ((1ull << L.kshift) - 1);
-                    y += gmem(L.ybuf)[((size_t)w * 2 + (size_t)((qb - 1) & 1)) * L.N + sub * N0 + j];
+                    y += gmem(L.ybuf)[((size_t)o * 2 + (size_t)((qb - 1) & 1)) * L.N + sub * N0 + j];
                 }
                 out[(long long)j * A.n_out] = y;
             }
             wg_stores_done();
             if (tid == 0)
-                __hip_atomic_store(A.flags + (size_t)w * 8, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
+                __hip_atomic_store(A.flags + (size_t)o * 8, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
+        } else {
+            // ---- delay-line sum for tick f + 1: partitions 1 .. P-1 of this workgroup's bins
+            if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
+            cf *Sn = L0.S + (size_t)((f + 1) & 1) * A.n_out * N0;
+            for (int k = w - n_io; k < N0; k += G4)
+                mxr_bin<MXR_NTT, 8>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, Sn, tid);
+            mxr_arrive(cS, tid);
         }
-        // ---- 3. S for tick f + 1: partitions 1 .. P-1 of every bin
-        for (int k = w; k < N0; k += G) mxr_bin<MXR_NTT, 8>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, L0.S, tid);
         __syncthreads();
     }
 }
