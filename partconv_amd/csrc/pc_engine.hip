@@ -3217,7 +3217,7 @@ static int mxr_setup(pc_mimo_rt *r)
         L.ni = (r->n_in + 3) & ~3;
         const size_t nb = (size_t)L.N * sizeof(cf);
         cf *H = nullptr;
-        if ((rc = mxr_alloc(r, (void **)&L.fdl, (size_t)r->n_in * L.P * nb))) return rc;
+        if ((rc = mxr_alloc(r, (void **)&L.fdl, (size_t)L.ni * L.P * nb))) return rc;
         if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * L.ni * nb))) return rc;
         L.H = H;
         // level 0's S twice: the persistent kernel double-buffers it by tick parity
@@ -3377,9 +3377,9 @@ static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
     MxrArgs a = r->A;
     const cf *t = r->tw;
     void *args[] = {(void *)&a, (void *)&t};
-    // two FFT buffers, plus the output workgroups' filter cache when it leaves
-    // room for two workgroups per CU
-    const size_t fft = 2 * (size_t)(1 << LOGN) * sizeof(cf);
+    // two FFT buffers and the twiddles, plus the output workgroups' filter
+    // cache when it leaves room for two workgroups per CU
+    const size_t fft = 5 * (size_t)(1 << LOGN) * sizeof(cf);
     const size_t hc = (size_t)(1 << LOGN) * r->n_in * sizeof(cf);
     a.hcache = (fft + hc <= 80 * 1024) ? 1 : 0;
     const size_t lds = fft + (a.hcache ? hc : 0);
@@ -3493,7 +3493,7 @@ PC_API int pcxr_reset(pc_mimo_rt *r)
     for (int l = 0; l < r->M; l++) {
         const MxrLevel &L = A.lv[l];
         const size_t nb = (size_t)L.N * sizeof(cf);
-        HIPCHK(hipMemsetAsync(L.fdl, 0, (size_t)r->n_in * L.P * nb, r->s_main));
+        HIPCHK(hipMemsetAsync(L.fdl, 0, (size_t)L.ni * L.P * nb, r->s_main));
         HIPCHK(hipMemsetAsync(L.S, 0, (size_t)(l == 0 ? 2 : 1) * r->n_out * nb, r->s_main));
         if (L.ybuf) HIPCHK(hipMemsetAsync(L.ybuf, 0, (size_t)r->n_out * 2 * L.N * sizeof(float), r->s_main));
     }

@@ -22,9 +22,10 @@
 //
 // Layouts (device, bins in natural order, packed bin 0 = (X_0, X_N)):
 //   ring[i]   : 2R floats (doubled ring, as the instance engine's)
-//   fdl       : [n_in][P][N]     H : [N][P][n_out][ni] (scaled by 1/2N; ni = n_in
-//               rounded up to 4, zero-padded, so a lane's four filters are one
-//               aligned 32-byte load)
+//   fdl       : [P][N][ni]       H : [N][P][n_out][ni] (scaled by 1/2N; ni = n_in
+//               rounded up to 4, zero-padded, so a lane's four inputs or four
+//               filters are one aligned 32-byte load; a bin's inputs share two
+//               cache lines)
 //   S, Y      : [n_out][N] (level 0: S twice, by tick parity)   ybuf : [n_out][2][N] (levels >= 1)
 // Frames in and out are interleaved as PartConvMulti::run's buffers.
 #pragma once
@@ -36,7 +37,7 @@
 
 struct MxrLevel {
     int N, P, kshift;
-    int ni;                 // H row stride: n_in rounded up to 4 (zero-padded)
+    int ni;                 // fdl and H row stride: n_in rounded up to 4 (zero-padded)
     long long T;            // taps before this level
     cf *fdl;
     const cf *H;
@@ -64,7 +65,7 @@ struct MxrArgs {
     unsigned long long *flags;  // host-visible, one per output workgroup, 8 words apart
     unsigned long long *bar;    // grid barrier counter (zeroed before each launch)
     unsigned long long spin_limit;
-    int hcache;             // output workgroups keep H_0[o] in LDS (after the two FFT buffers)
+    int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
 };
 
 // Bin k of level L, for every output: the sum over partitions p in [p_lo, p_hi)
@@ -90,8 +91,7 @@ __device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, 
     if (act) {
         const int bm = (int)(b % (uint64_t)L.P);
         for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
-            float4 hv[PB][II / 2];
-            cf xv[PB][II];
+            float4 hv[PB][II / 2], xv[PB][II / 2];
 #pragma unroll
             for (int j = 0; j < PB; j++) {
                 const int p = p0 + j;
@@ -100,13 +100,12 @@ __device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, 
                     if (r < 0) r += L.P;
                     const float4 *h = reinterpret_cast<const float4 *>(
                         L.H + (((size_t)k * L.P + p) * n_out + o) * L.ni + II * ig);
+                    const float4 *x = reinterpret_cast<const float4 *>(
+                        L.fdl + ((size_t)r * L.N + k) * L.ni + II * ig);
 #pragma unroll
-                    for (int q = 0; q < II / 2; q++) hv[j][q] = h[q];
-#pragma unroll
-                    for (int q = 0; q < II; q++) {
-                        const int i = II * ig + q;
-                        xv[j][q] = (i < n_in) ? gload2(L.fdl + ((size_t)i * L.P + r) * L.N + k)
-                                              : make_float2(0.f, 0.f);
+                    for (int q = 0; q < II / 2; q++) {
+                        hv[j][q] = h[q];
+                        xv[j][q] = x[q];
                     }
                 }
             }
@@ -115,8 +114,10 @@ __device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, 
                 if (p0 + j < p_hi) {
 #pragma unroll
                     for (int q = 0; q < II / 2; q++) {
-                        acc = pmac(acc, xv[j][2 * q], make_float2(hv[j][q].x, hv[j][q].y), bin0);
-                        acc = pmac(acc, xv[j][2 * q + 1], make_float2(hv[j][q].z, hv[j][q].w), bin0);
+                        acc = pmac(acc, make_float2(xv[j][q].x, xv[j][q].y), make_float2(hv[j][q].x, hv[j][q].y),
+                                   bin0);
+                        acc = pmac(acc, make_float2(xv[j][q].z, xv[j][q].w), make_float2(hv[j][q].z, hv[j][q].w),
+                                   bin0);
                     }
                 }
             }
@@ -156,13 +157,38 @@ __device__ __forceinline__ int mxr_wait(const unsigned long long *ctr, unsigned 
     return *s_ok;
 }
 
+// The same for two counters at once (both loads in flight per poll).
+__device__ __forceinline__ int mxr_wait2(const unsigned long long *c1, unsigned long long t1,
+                                         const unsigned long long *c2, unsigned long long t2,
+                                         unsigned long long limit, int tid, int *s_ok)
+{
+    if (tid == 0) {
+        const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + limit;
+        int ok = 1;
+        for (;;) {
+            const unsigned long long v1 = __hip_atomic_load(c1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long v2 = __hip_atomic_load(c2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v1 >= t1 && v2 >= t2) break;
+            if (__builtin_amdgcn_s_memrealtime() > t_end) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok;
+}
+
 // Every wave's stores complete, device-scope release, then count one arrival.
 __device__ __forceinline__ void mxr_arrive(unsigned long long *ctr, int tid)
 {
     wg_stores_done();
     if (tid == 0) {
-        __threadfence();
-        atomicAdd(ctr, 1ull);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -192,14 +218,20 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
     const int n_io = A.n_in + A.n_out, G4 = (int)gridDim.x - n_io;
     const MxrLevel &L0 = A.lv[0];
     unsigned long long *cX = A.bar, *cS = A.bar + 8;
-    // output workgroups: their partition-0 filter spectra, input-major
-    // (Hc[i N0 + k]: lanes read consecutive words), loaded once per launch
-    cf *Hc = mxr_lds + 2 * N0;
-    if (A.hcache && w >= A.n_in && w < n_io) {
-        const int o = w - A.n_in;
-        for (int e = tid; e < N0 * A.n_in; e += MXR_NTT) {
-            const int k = e / A.n_in, i = e - k * A.n_in;
-            Hc[(size_t)i * N0 + k] = L0.H[((size_t)k * L0.P * A.n_out + o) * L0.ni + i];
+    // input and output workgroups: the transforms' twiddles (entries below
+    // 2 N0 + N0 / 2 + 1) in LDS, so the transforms make no global loads;
+    // output workgroups also keep their partition-0 filter spectra,
+    // input-major (Hc[i N0 + k]: lanes read consecutive words)
+    cf *twl = mxr_lds + 2 * N0;
+    cf *Hc = mxr_lds + 5 * N0;
+    if (w < n_io) {
+        for (int e = tid; e <= 2 * N0 + N0 / 2; e += MXR_NTT) twl[e] = tw[e];
+        if (A.hcache && w >= A.n_in) {
+            const int o = w - A.n_in;
+            for (int e = tid; e < N0 * A.n_in; e += MXR_NTT) {
+                const int k = e / A.n_in, i = e - k * A.n_in;
+                Hc[(size_t)i * N0 + k] = L0.H[((size_t)k * L0.P * A.n_out + o) * L0.ni + i];
+            }
         }
         __syncthreads();
     }
@@ -238,54 +270,73 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                 win[N0 + j] = x;
             }
             __syncthreads();
-            const cf *X = small_forward<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
-            cf *row = L0.fdl + ((size_t)w * L0.P + (size_t)(f % (uint64_t)L0.P)) * N0;
-            for (int k = tid; k < N0; k += MXR_NTT) row[k] = X[k];
+            const cf *X = small_forward<LOGN, MXR_NTT>(bufA, bufB, twl, tid);
+            cf *row = L0.fdl + (size_t)(f % (uint64_t)L0.P) * N0 * L0.ni + w;
+            for (int k = tid; k < N0; k += MXR_NTT) row[(size_t)k * L0.ni] = X[k];
             mxr_arrive(cX, tid);
         } else if (w < n_io) {
             // ---- output o: Y_o = S_o + sum_i H_0[o][i] X_i (inputs in order), c2r + levels
             const int o = w - A.n_in;
-            if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
-            if (!mxr_wait(cS, (unsigned long long)t * G4, A.spin_limit, tid, &s_ok)) return;
+            const uint64_t fr = f + 1;  // strict: block fr/k - 1, sub-block fr % k
+            // the level blocks' outputs were complete before the release (the
+            // host waits for them): fetch them while the inputs transform
+            constexpr int JO = (N0 + MXR_NTT - 1) / MXR_NTT;
+            float ylv[JO];
+            if (A.M > 1) {
+                if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __syncthreads();
+            }
+#pragma unroll
+            for (int q = 0; q < JO; q++) {
+                const int j = tid + q * MXR_NTT;
+                ylv[q] = 0.f;
+                if (j < N0)
+                    for (int l = 1; l < A.M; l++) {
+                        const MxrLevel &L = A.lvd[l];
+                        const uint64_t qb = fr >> L.kshift;
+                        if (qb == 0) continue;
+                        const uint64_t sub = fr & ((1ull << L.kshift) - 1);
+                        ylv[q] += gmem(L.ybuf)[((size_t)o * 2 + (size_t)((qb - 1) & 1)) * L.N + sub * N0 + j];
+                    }
+            }
+            // this tick's delay-line rows and S for this tick
+            if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * G4, A.spin_limit,
+                           tid, &s_ok))
+                return;
             const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0;
             const int r0 = (int)(f % (uint64_t)L0.P);
             for (int k = tid; k < N0; k += MXR_NTT) {
                 const bool bin0 = (k == 0);
                 const cf *h = L0.H + ((size_t)k * L0.P * A.n_out + o) * L0.ni;  // partition 0
                 cf acc = gload2(S + (size_t)o * N0 + k);
-                for (int i0 = 0; i0 < A.n_in; i0 += 8) {
-                    cf hv[8], xv[8];
+                // the bin's inputs (contiguous) in rounds of 16
+                const float4 *xk = reinterpret_cast<const float4 *>(L0.fdl + ((size_t)r0 * N0 + k) * L0.ni);
+                for (int i0 = 0; i0 < A.n_in; i0 += 16) {
+                    float4 xv[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        if (i0 + 2 * q < A.n_in) xv[q] = xk[i0 / 2 + q];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
-                        const int i = i0 + q;
-                        if (i < A.n_in) {
-                            xv[q] = gload2(L0.fdl + ((size_t)i * L0.P + r0) * N0 + k);
-                            if (!A.hcache) hv[q] = gload2(h + i);
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const int i = i0 + q;
-                        if (i < A.n_in) acc = pmac(acc, xv[q], A.hcache ? Hc[(size_t)i * N0 + k] : hv[q], bin0);
+                        const int i = i0 + 2 * q;
+                        if (i < A.n_in)
+                            acc = pmac(acc, make_float2(xv[q].x, xv[q].y),
+                                       A.hcache ? Hc[(size_t)i * N0 + k] : h[i], bin0);
+                        if (i + 1 < A.n_in)
+                            acc = pmac(acc, make_float2(xv[q].z, xv[q].w),
+                                       A.hcache ? Hc[(size_t)(i + 1) * N0 + k] : h[i + 1], bin0);
                     }
                 }
                 bufA[k] = acc;
             }
             __syncthreads();
-            const cf *z = small_inverse<LOGN, MXR_NTT>(bufA, bufB, tw, tid);
+            const cf *z = small_inverse<LOGN, MXR_NTT>(bufA, bufB, twl, tid);
             const float *zf = reinterpret_cast<const float *>(z);
-            const uint64_t fr = f + 1;  // strict: block fr/k - 1, sub-block fr % k
             float *out = A.out + (long long)(f - A.fbase) * A.out_stride + o;
-            for (int j = tid; j < N0; j += MXR_NTT) {
-                float y = zf[j];
-                for (int l = 1; l < A.M; l++) {
-                    const MxrLevel &L = A.lvd[l];
-                    const uint64_t qb = fr >> L.kshift;
-                    if (qb == 0) continue;
-                    const uint64_t sub = fr & ((1ull << L.kshift) - 1);
-                    y += gmem(L.ybuf)[((size_t)o * 2 + (size_t)((qb - 1) & 1)) * L.N + sub * N0 + j];
-                }
-                out[(long long)j * A.n_out] = y;
+#pragma unroll
+            for (int q = 0; q < JO; q++) {
+                const int j = tid + q * MXR_NTT;
+                if (j < N0) out[(long long)j * A.n_out] = zf[j] + ylv[q];
             }
             wg_stores_done();
             if (tid == 0)
@@ -319,8 +370,8 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_fwd(MxrLevel L, const float *__r
     for (int n = tid; n < N; n += MXR_NT) bufA[n] = wsrc[n];
     __syncthreads();
     const cf *X = small_forward<LOGN, MXR_NT>(bufA, bufB, tw, tid);
-    cf *row = L.fdl + ((size_t)i * L.P + (size_t)(m % (uint64_t)L.P)) * N;
-    for (int k = tid; k < N; k += MXR_NT) row[k] = X[k];
+    cf *row = L.fdl + (size_t)(m % (uint64_t)L.P) * N * L.ni + i;
+    for (int k = tid; k < N; k += MXR_NT) row[(size_t)k * L.ni] = X[k];
 }
 
 // Level >= 1, block b: p = 0 term + S -> Y (one workgroup per bin).
