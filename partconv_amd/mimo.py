@@ -93,6 +93,8 @@ class MimoRealtime:
             raise ValueError("frame buffers too small")
         if not (x.is_contiguous() and y.is_contiguous()):
             raise ValueError("frame buffers must be contiguous")
+        if x.dtype != torch.float32 or y.dtype != torch.float32 or not (x.is_cuda and y.is_cuda):
+            raise ValueError("frame buffers must be float32 cuda tensors")
         torch.cuda.synchronize()
         if lib().pcxr_run_device(self._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                                  int(nframes)) != 0:
