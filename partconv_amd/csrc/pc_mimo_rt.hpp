@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
             if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
             cf *Sn = L0.S + (size_t)((f + 1) & 1) * A.n_out * N0;
             for (int k = w - n_io; k < N0; k += G4)
-                mxr_bin<MXR_NTT, 8>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, Sn, tid);
+                mxr_bin<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, Sn, tid);
             mxr_arrive(cS, tid);
         }
         __syncthreads();
@@ -381,9 +381,13 @@ __global__ void __launch_bounds__(MXR_NT) k_mxr_comb(MxrLevel L, int n_in, int n
 }
 
 // Level >= 1, S for block b (partitions 1 .. P-1; one workgroup per bin).
+// It runs beside the persistent kernel, so both batch four partitions per
+// round of loads: 84 and about 156 VGPRs, and several of its workgroups fit
+// on a CU that holds a persistent one.  (Eight per round in both: 155 and
+// 245 VGPRs, 1.45 TB/s here against 2.6 TB/s, measured at 32 x 32 pairs.)
 __global__ void __launch_bounds__(MXR_NT) k_mxr_part(MxrLevel L, int n_in, int n_out, uint64_t b)
 {
-    mxr_bin<MXR_NT, 8>(L, n_in, n_out, blockIdx.x, b, 1, L.P, nullptr, L.S, threadIdx.x);
+    mxr_bin<MXR_NT, 4>(L, n_in, n_out, blockIdx.x, b, 1, L.P, nullptr, L.S, threadIdx.x);
 }
 
 // Level >= 1, block m: output workgroup o: c2r of Y_o -> double-buffer half m & 1.
