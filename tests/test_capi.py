@@ -13,7 +13,7 @@ HEADER = ROOT / "include" / "partconv_amd.h"
 def declared_functions():
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"\b(pc[mf]?_[a-z_]+)\s*\(", text)
+    names = re.findall(r"\b(pc[a-z]*_[a-z_0-9]+)\s*\(", text)
     return sorted(set(names))
 
 
@@ -49,6 +49,30 @@ def test_errors_are_returned_not_fatal():
     assert L.pcm_setup(h, b"/nonexistent/pc.config", 1, 0, 0.0) == 2
     assert "Unable to open config file" in _lib.last_error()
     L.pcm_destroy(h)
+
+
+def test_mimo_realtime_rejects_bad_shapes_without_gpu():
+    """pcxr_create validates sizes and the partition before touching the GPU
+    (the partition rules are the reference's, src/partconv.cpp:94-158)."""
+    from partconv_amd import _lib
+    L = _lib.lib()
+
+    def create(FS, bs, N, P, taps, n_in, n_out):
+        Na = (ctypes.c_int * len(N))(*N)
+        Pa = (ctypes.c_int * len(P))(*P)
+        return L.pcxr_create(FS, bs, len(N), Na, Pa, taps, n_in, n_out)
+
+    assert not create(48000, 256, [256, 4096], [30, 0], 65536, 33, 32)
+    assert "1..32" in _lib.last_error()
+    assert not create(48000, 256, [256, 4096], [30, 0], 65536, 32, 0)
+    assert not create(48000, 128, [256, 4096], [30, 0], 65536, 4, 4)
+    assert "buffer_size" in _lib.last_error()
+    assert not create(48000, 32, [32, 256], [8, 0], 4096, 4, 4)
+    assert not create(48000, 256, [256, 8192], [64, 0], 65536, 4, 4)
+    assert "4096" in _lib.last_error()
+    # a later block smaller than an earlier one violates the reference's rules
+    assert not create(48000, 256, [256, 128], [4, 0], 4096, 4, 4)
+    assert not create(0, 256, [256], [0], 4096, 4, 4)
 
 
 def test_config_parse_errors_without_gpu(tmp_path):

@@ -34,6 +34,10 @@ def test_mimo_mfma_matches_fp64(n_in, n_out, taps, T, N, P):
 @pytest.mark.parametrize("n_in,n_out,taps,N,P,nfr", [
     (5, 3, 10000, [128, 1024], [14, 0], 3 * 8 + 77),
     (32, 32, 20000, [256, 4096], [30, 0], 3 * 16 + 7),
+    (3, 2, 3000, [128], [0], 50),                              # level 0 only
+    (4, 4, 40000, [64, 512, 4096], [14, 14, 0], 5 * 64 + 13), # three levels, buffer 64
+    (1, 32, 5000, [512, 2048], [6, 0], 3 * 4 + 5),            # one input, buffer 512
+    (32, 1, 5000, [1024], [0], 9),                             # one output, buffer 1024
 ])
 def test_mimo_realtime_matches_fp64(n_in, n_out, taps, N, P, nfr):
     """Real-time MIMO engine (shared input transforms and delay lines, per-bin
