@@ -47,6 +47,8 @@
 #define PC_SESSION_IDLE_S 1.0       // host sessions restart after this long without a release
 #define PC_TICK_TIMEOUT_S 10.0
 #define PC_LL_MAX_WORDS 16384       // host sessions use tagged input words up to nIn * N0 of these
+#define PC_DH_PT 8                  // dL/dh: partitions per thread (register window)
+#define PC_DH_MC 64                 // dL/dh: blocks per chunk (one workgroup column each)
 
 static thread_local std::string g_err;
 
@@ -2958,12 +2960,17 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
                 if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, s))) return rc;
                 c->recomputed_levels++;
             }
-            if ((rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * L.N))) return rc;
-            const int kb = (L.N / 2 + 255) / 256;
-            hipLaunchKernelGGL(k_off_dh_part, dim3(L.P * C * kb), dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P);
+            // partial sums per clip and chunk of PC_DH_MC blocks, PC_DH_PT partitions per thread
+            const int nch = (M + PC_DH_MC - 1) / PC_DH_MC;
+            if ((rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * nch * L.N))) return rc;
+            const long long items = (long long)C * (L.N / 2);
+            hipLaunchKernelGGL(k_off_dh_part<PC_DH_PT>,
+                               dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
+                                    (unsigned)nch),
+                               dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P, PC_DH_MC);
             const long long nd = (long long)L.P * L.N;
             hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, c->dpart, Wsp,
-                               C, L.N, L.P);
+                               C * nch, L.N, L.P);
             HIPCHK(hipGetLastError());
             InvDst D{};
             D.dst = d_dh;

@@ -336,40 +336,63 @@ k_off_mac(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y
     reinterpret_cast<float4 *>(Y)[i] = acc;
 }
 
-// D partial sums: part[p][j][k] = sum_{(c,m) in chunk j, m >= p} G[c,m] conj(X[c,m-p]);
-// chunk j = clip j (grid: P x C x ceil(N/2/256)).
+// D partial sums, one chunk of MC blocks per workgroup column:
+//   part[p][c nch + j][k] = sum_{m in chunk j, m >= p} G[c,m,k] conj(X[c,m-p,k])
+// (nch = ceil(M / MC) chunks, m ascending).  A thread owns one (clip, bin
+// pair) and the PT partitions p0 .. p0+PT-1 (p0 = PT blockIdx.y): each step of
+// m loads G[c,m] and X[c,m-p0] once and shifts X through a register window
+// (xw[t] = X[c,m-p0-t]; zero before block 0), so a spectrum is read once per
+// group of PT partitions, not once per partition.  A chunk first shifts in the
+// PT-1 blocks before it.  Grid: (ceil(C N/2 / 256), ceil(P / PT), nch).
+template <int PT>
 __global__ void __launch_bounds__(256)
 k_off_dh_part(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict__ part, int C, int M, int N,
-              int P)
+              int P, int MC)
 {
     const int half = N / 2;
-    const int kb = (half + 255) / 256;
-    const int b = blockIdx.x;
-    const int p = b / (C * kb);
-    const int rem = b - p * C * kb;
-    const int c = rem / kb;
-    const int k2 = (rem - c * kb) * 256 + threadIdx.x;
-    if (k2 >= half) return;
-    const float4 *Gr = reinterpret_cast<const float4 *>(G) + (size_t)c * M * half;
-    const float4 *Xr = reinterpret_cast<const float4 *>(X) + (size_t)c * M * half;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int m = p; m < M; m++) {
-        const float4 g = Gr[(size_t)m * half + k2];
-        const float4 x = Xr[(size_t)(m - p) * half + k2];
-        if (k2 == 0) {
-            acc.x += g.x * x.x;
-            acc.y += g.y * x.y;
-        } else {
-            acc.x += g.x * x.x + g.y * x.y;
-            acc.y += g.y * x.x - g.x * x.y;
-        }
-        acc.z += g.z * x.z + g.w * x.w;
-        acc.w += g.w * x.z - g.z * x.w;
+    const int nch = (M + MC - 1) / MC;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)C * half) return;
+    const int c = (int)(e / half), k2 = (int)(e - (long long)c * half);
+    const int p0 = blockIdx.y * PT, j = blockIdx.z;
+    const float4 *Gr = reinterpret_cast<const float4 *>(G) + (size_t)c * M * half + k2;
+    const float4 *Xr = reinterpret_cast<const float4 *>(X) + (size_t)c * M * half + k2;
+    const int m0 = j * MC, m1 = min(M, m0 + MC);
+    const bool bin0 = (k2 == 0);
+    float4 xw[PT], acc[PT];
+#pragma unroll
+    for (int t = 0; t < PT; t++) {
+        xw[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    reinterpret_cast<float4 *>(part)[((size_t)p * C + c) * half + k2] = acc;
+#pragma unroll 4
+    for (int m = max(p0, m0 - (PT - 1)); m < m1; m++) {
+#pragma unroll
+        for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
+        xw[0] = Xr[(size_t)(m - p0) * half];
+        if (m < m0) continue;
+        const float4 g = Gr[(size_t)m * half];
+#pragma unroll
+        for (int t = 0; t < PT; t++) {
+            const float4 x = xw[t];
+            if (bin0) {
+                acc[t].x += g.x * x.x;
+                acc[t].y += g.y * x.y;
+            } else {
+                acc[t].x += g.x * x.x + g.y * x.y;
+                acc[t].y += g.y * x.x - g.x * x.y;
+            }
+            acc[t].z += g.z * x.z + g.w * x.w;
+            acc[t].w += g.w * x.z - g.z * x.w;
+        }
+    }
+    float4 *dst = reinterpret_cast<float4 *>(part) + ((size_t)c * nch + j) * half + k2;
+#pragma unroll
+    for (int t = 0; t < PT; t++)
+        if (p0 + t < P) dst[(size_t)(p0 + t) * C * nch * half] = acc[t];
 }
 
-// D[p][k] = sum_j part[p][j][k] (clip order)
+// D[p][k] = sum_j part[p][j][k] (j: clip-major, then chunk)
 __global__ void __launch_bounds__(256)
 k_off_dh_reduce(const cf *__restrict__ part, cf *__restrict__ D, int C, int N, int P)
 {
