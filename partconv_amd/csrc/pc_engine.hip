@@ -2969,7 +2969,7 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
                                     (unsigned)nch),
                                dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P, PC_DH_MC);
             const long long nd = (long long)L.P * L.N;
-            hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, c->dpart, Wsp,
+            hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, c->dpart, Wsp,
                                C * nch, L.N, L.P);
             HIPCHK(hipGetLastError());
             InvDst D{};

@@ -392,20 +392,39 @@ k_off_dh_part(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict
         if (p0 + t < P) dst[(size_t)(p0 + t) * C * nch * half] = acc[t];
 }
 
-// D[p][k] = sum_j part[p][j][k] (j: clip-major, then chunk)
+// D[p][k] = sum_j part[p][j][k] over the J = C nch partial sums (j: clip-major,
+// then chunk).  A workgroup takes 16 consecutive outputs; its 16 thread rows
+// each sum one contiguous segment of j in order, and the segments are added
+// in order: a fixed order, independent of timing.  Grid: ceil(P N / 16).
 __global__ void __launch_bounds__(256)
-k_off_dh_reduce(const cf *__restrict__ part, cf *__restrict__ D, int C, int N, int P)
+k_off_dh_reduce(const cf *__restrict__ part, cf *__restrict__ D, int J, int N, int P)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)P * N) return;
-    const int p = (int)(i / N), k = (int)(i % N);
+    __shared__ cf sp[16][16];
+    const int kk = threadIdx.x & 15, sg = threadIdx.x >> 4;
+    const long long i = (long long)blockIdx.x * 16 + kk;
+    const bool ok = i < (long long)P * N;
+    const int p = ok ? (int)(i / N) : 0, k = ok ? (int)(i % N) : 0;
+    const int len = (J + 15) / 16, j0 = sg * len, j1 = min(J, j0 + len);
     cf acc = make_float2(0.f, 0.f);
-    for (int c = 0; c < C; c++) {
-        const cf v = part[((size_t)p * C + c) * N + k];
-        acc.x += v.x;
-        acc.y += v.y;
+    if (ok) {
+        const cf *src = part + (size_t)p * J * N + k;
+#pragma unroll 8
+        for (int j = j0; j < j1; j++) {
+            const cf v = src[(size_t)j * N];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
     }
-    D[i] = acc;
+    sp[sg][kk] = acc;
+    __syncthreads();
+    if (sg == 0 && ok) {
+        cf t = sp[0][kk];
+        for (int q = 1; q < 16; q++) {
+            t.x += sp[q][kk].x;
+            t.y += sp[q][kk].y;
+        }
+        D[i] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
