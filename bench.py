@@ -448,11 +448,14 @@ def bench_diff(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    conv.probe(0)  # the forward spectral product of the 256 level, timed on its stream
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
+    p_avg_us, p_launches, p_bytes = conv.probe_read()
+    conv.probe(-1)
     # long IR, spectra recomputed in backward
     taps2 = 1 << 20
     conv2 = PartitionedConv([256, 2048, 16384], [14, 14, 0], taps2, budget_bytes=0)
@@ -476,19 +479,21 @@ def bench_diff(args):
     with torch.no_grad():
         yc = conv(x, h0)[3].cpu().numpy()
     err = float(np.max(np.abs(yc - r)) / np.sqrt(np.mean(r ** 2)))
-    # whole-step HBM roofline: the forward and the two adjoint products (dL/dx,
-    # dL/dh) each stream the partition spectra of every block once
+    # per-kernel roofline: the forward spectral product of the 256 level
+    # (64 clips x 512 blocks, 14 partitions), its unique bytes over its
+    # HIP-event duration
+    roof = kernel_roofline(p_avg_us, p_launches, p_bytes,
+                           "k_off_mac (256 level, forward: Y_m = sum_p X_{m-p} H_p for 64 clips x 512 blocks, "
+                           "14 partitions)")
+    # whole step in the canonical per-block accounting (the partition spectra
+    # are shared by the 64 clips and served from cache, so this can exceed
+    # the HBM peak): forward, dL/dx and dL/dh each stream them once per block
     Ns, Ps = conv.N, conv.P
     fwd_per_sample = sum(block_bytes(nl, pl) / nl for nl, pl in zip(Ns, Ps))
     step_bytes = 3.0 * fwd_per_sample * C * T
-    gbps = step_bytes * args.steps / secs / 1e9
-    roof = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
-            "traffic": None, "kernel": "whole step (forward + dL/dx + dL/dh kernels)",
-            "alg_bytes_per_step": step_bytes, "alg_bytes_per_clip_sample": 3.0 * fwd_per_sample,
-            "partition": [list(Ns), list(Ps)],
-            "note": "canonical block bytes 8(N+1)(2P+1)+12N for every block of every clip, three passes; the "
-                    "partition spectra are shared by the 64 clips and served from cache, so this can exceed "
-                    "the HBM peak"}
+    step_canon = {"alg_bytes_per_step": step_bytes, "achieved_GBps": step_bytes * args.steps / secs / 1e9,
+                  "partition": [list(Ns), list(Ps)],
+                  "note": "canonical block bytes 8(N+1)(2P+1)+12N for every block of every clip, three passes"}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_diff(x[:8].cpu().numpy(), h0.detach().cpu().numpy(), tgt[:8].cpu().numpy(),
@@ -504,7 +509,7 @@ def bench_diff(args):
         "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
                               "ms_per_step": 1e3 * secs2 / args.steps,
                               "recomputed_levels": int(conv2.recomputed_levels)},
-        "roofline": roof, "cpu_baseline": cpu}), flush=True)
+        "roofline": roof, "whole_step_canonical_bytes": step_canon, "cpu_baseline": cpu}), flush=True)
 
 
 def cpu_baseline_diff(x, h, tgt, budget_s):

@@ -321,6 +321,14 @@ int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, fl
 int pcc_set_budget(pc_conv *c, long long bytes);
 /* Number of level spectra recomputed by backward so far. */
 long long pcc_recomputed_levels(const pc_conv *c);
+/* Measurement hook, as pcm_probe (no reference counterpart): time with HIP
+ * events stamped by the launch itself the forward spectral product (k_off_mac)
+ * of level `level` each time it launches (-1 disables); read back the mean
+ * duration (us), the launch count and the algorithmic bytes one launch moves
+ * (every block spectrum of every clip read once, the level's filter spectra
+ * once, the products written once). */
+int pcc_probe(pc_conv *c, int level);
+int pcc_probe_read(pc_conv *c, double *avg_us, long long *launches, double *bytes_per_launch);
 void pcc_destroy(pc_conv *c);
 
 /* ----------------------------------------------------------------------------

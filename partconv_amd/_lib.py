@@ -100,6 +100,9 @@ SIGNATURES = {
     "pcc_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_set_budget": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),
+    "pcc_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcc_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(ctypes.c_double)]),
     "pcc_destroy": (None, [_vp]),
     "pcx_create": (_vp, [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pcx_set_filters": (ctypes.c_int, [_vp, _vp, _vp]),

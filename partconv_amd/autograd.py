@@ -61,6 +61,17 @@ class PartitionedConv:
         if rc != 0:
             raise RuntimeError(last_error())
 
+    def probe(self, level):
+        """Time the forward spectral product of one level with HIP events (-1 disables)."""
+        self._check(lib().pcc_probe(self._h, int(level)))
+
+    def probe_read(self):
+        """(avg_us, launches, algorithmic_bytes_per_launch) of the probed kernel."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(lib().pcc_probe_read(self._h, ctypes.byref(a), ctypes.byref(n), ctypes.byref(b)))
+        return a.value, n.value, b.value
+
     def set_filter(self, h):
         key = (h.data_ptr(), h._version)
         if key != self._filter_key:

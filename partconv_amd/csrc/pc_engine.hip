@@ -2655,7 +2655,62 @@ struct pc_conv {
     int keep_C = 0, keep_T = 0;
     bool keep_valid = false;
     long long recomputed_levels = 0;
+    // probe: HIP events around the forward spectral product of one level
+    int probe_level = -1;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
+    double probe_ms = 0, probe_bytes = 0;
+    long long probe_n = 0;
 };
+
+// Spectral products of one level over every block of every clip (dir +1:
+// causal, -1: adjoint): the register-window kernel (window of 4, 8 or 16)
+// for up to 16 partitions, else one thread per (clip, block, bin pair).  With events (e0, e1), the
+// launch stamps them.
+#define PC_MAC_MC 32  // blocks per chunk of the register-window kernel
+template <typename K, typename... Args>
+static void off_launch(K kern, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0)
+        hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, s, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, args...);
+}
+
+static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLevel &L, int dir, hipStream_t s,
+                    hipEvent_t e0, hipEvent_t e1)
+{
+    const long long items = (long long)C * (L.N / 2);
+    const dim3 grid((unsigned)((items + 255) / 256), (unsigned)((M + PC_MAC_MC - 1) / PC_MAC_MC));
+    const int mc = PC_MAC_MC;
+    if (L.P <= 4) {
+        if (dir > 0) off_launch(k_off_mac_w<4, 1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+        else off_launch(k_off_mac_w<4, -1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+    } else if (L.P <= 8) {
+        if (dir > 0) off_launch(k_off_mac_w<8, 1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+        else off_launch(k_off_mac_w<8, -1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+    } else if (L.P <= 16) {
+        if (dir > 0) off_launch(k_off_mac_w<16, 1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+        else off_launch(k_off_mac_w<16, -1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
+    } else {
+        off_launch(k_off_mac, dim3((unsigned)((items * M + 255) / 256)), s, e0, e1, X, H, Y, C, M, L.N, L.P, dir);
+    }
+}
+
+// Fold the probe's pending events into its totals (waits for them).
+static int conv_probe_fold(pc_conv *c)
+{
+    for (auto &e : c->probe_ev) {
+        HIPCHK(hipEventSynchronize(e.second));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        c->probe_ms += ms;
+        c->probe_n++;
+        HIPCHK(hipEventDestroy(e.first));
+        HIPCHK(hipEventDestroy(e.second));
+    }
+    c->probe_ev.clear();
+    return PC_OK;
+}
 
 static int conv_grow(cf **p, size_t *cap, size_t need)
 {
@@ -2872,9 +2927,18 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
         }
         if ((rc = conv_grow(&c->work, &c->work_cap, n))) return rc;
         if ((rc = off_rfft(c, L, W, C, X, s))) return rc;
-        const long long nthr = (long long)C * M * (L.N / 2);
-        hipLaunchKernelGGL(k_off_mac, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, X, L.H, c->work, C, M,
-                           L.N, L.P, 1);
+        if (c->probe_level == (int)li) {
+            if (c->probe_ev.size() >= 64 && (rc = conv_probe_fold(c))) return rc;
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            off_mac(X, L.H, c->work, C, M, L, 1, s, e0, e1);
+            c->probe_ev.push_back({e0, e1});
+            // the block spectra read once, the filter spectra once, the products written once
+            c->probe_bytes = 2.0 * 8.0 * (double)C * M * L.N + 8.0 * (double)L.P * L.N;
+        } else {
+            off_mac(X, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
+        }
         HIPCHK(hipGetLastError());
         InvDst D{};
         D.dst = d_y;
@@ -2930,9 +2994,7 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
         Wg.scale = 1.f;
         if ((rc = off_rfft(c, L, Wg, C, G, s))) return rc;
         if (d_dx) {
-            const long long nthr = (long long)C * M * (L.N / 2);
-            hipLaunchKernelGGL(k_off_mac, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, G, L.H, Wsp, C,
-                               M, L.N, L.P, -1);
+            off_mac(G, L.H, Wsp, C, M, L, -1, s, nullptr, nullptr);
             HIPCHK(hipGetLastError());
             InvDst D{};
             D.dst = d_dx;
@@ -2990,10 +3052,35 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
     return PC_OK;
 }
 
+PC_API int pcc_probe(pc_conv *c, int level)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    if (level < -1 || level >= (int)c->lv.size()) return fail(PC_ERR_ARG, "probe level out of range");
+    int rc = conv_probe_fold(c);
+    if (rc) return rc;
+    c->probe_level = level;
+    c->probe_ms = 0;
+    c->probe_n = 0;
+    c->probe_bytes = 0;
+    return PC_OK;
+}
+
+PC_API int pcc_probe_read(pc_conv *c, double *avg_us, long long *launches, double *bytes_per_launch)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    int rc = conv_probe_fold(c);
+    if (rc) return rc;
+    if (avg_us) *avg_us = c->probe_n ? 1e3 * c->probe_ms / (double)c->probe_n : 0.0;
+    if (launches) *launches = c->probe_n;
+    if (bytes_per_launch) *bytes_per_launch = c->probe_bytes;
+    return PC_OK;
+}
+
 PC_API void pcc_destroy(pc_conv *c)
 {
     if (!c) return;
     hipDeviceSynchronize();
+    conv_probe_fold(c);
     for (ConvLevel &L : c->lv) hipFree(L.H);
     for (cf *p : c->keep) hipFree(p);
     hipFree(c->work);

@@ -336,6 +336,70 @@ k_off_mac(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y
     reinterpret_cast<float4 *>(Y)[i] = acc;
 }
 
+// The same products for P <= PW partitions, one chunk of MC blocks per
+// workgroup column: a thread owns one (clip, bin pair), keeps the level's PW
+// filter spectra of its bins in registers and shifts the block spectra
+// through a register window (causal: xw[t] = X[c, m-t], m ascending;
+// adjoint: xw[t] = X[c, m+t], m descending), so each block spectrum is read
+// once (plus the P-1 blocks before a chunk) instead of once per partition.
+// Terms are summed in partition order, as k_off_mac does.  Grid:
+// (ceil(C N/2 / 256), ceil(M / MC)).
+template <int PW, int DIR>
+__global__ void __launch_bounds__(256)
+k_off_mac_w(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y, int C, int M, int N, int P,
+            int MC)
+{
+    const int half = N / 2;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)C * half) return;
+    const int c = (int)(e / half), k2 = (int)(e - (long long)c * half);
+    const float4 *Xr = reinterpret_cast<const float4 *>(X) + (size_t)c * M * half + k2;
+    float4 *Yr = reinterpret_cast<float4 *>(Y) + (size_t)c * M * half + k2;
+    const float4 *Hr = reinterpret_cast<const float4 *>(H) + k2;
+    const bool bin0 = (k2 == 0);
+    float4 hr[PW], xw[PW];
+#pragma unroll
+    for (int t = 0; t < PW; t++) {
+        hr[t] = (t < P) ? Hr[(size_t)t * half] : make_float4(0.f, 0.f, 0.f, 0.f);
+        xw[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const int m0 = blockIdx.y * MC, m1 = min(M, m0 + MC);
+    const int mfirst = (DIR > 0) ? max(0, m0 - (P - 1)) : min(M - 1, m1 - 1 + (P - 1));
+    const int nstep = (DIR > 0) ? (m1 - mfirst) : (mfirst - m0 + 1);
+#pragma unroll 2
+    for (int s = 0; s < nstep; s++) {
+        const int m = mfirst + DIR * s;
+#pragma unroll
+        for (int t = PW - 1; t > 0; t--) xw[t] = xw[t - 1];
+        xw[0] = Xr[(size_t)m * half];
+        if ((DIR > 0) ? (m < m0) : (m >= m1)) continue;
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int t = 0; t < PW; t++) {
+            // hr[t] is zero for t >= P: those terms add exact zeros
+            const float4 a = xw[t], h = hr[t];
+            if (bin0) {
+                acc.x += a.x * h.x;
+                acc.y += a.y * h.y;
+            } else if (DIR > 0) {
+                acc.x += a.x * h.x - a.y * h.y;
+                acc.y += a.x * h.y + a.y * h.x;
+            } else {
+                acc.x += a.x * h.x + a.y * h.y;
+                acc.y += a.y * h.x - a.x * h.y;
+            }
+            if (DIR > 0) {
+                acc.z += a.z * h.z - a.w * h.w;
+                acc.w += a.z * h.w + a.w * h.z;
+            } else {
+                acc.z += a.z * h.z + a.w * h.w;
+                acc.w += a.w * h.z - a.z * h.w;
+            }
+        }
+        Yr[(size_t)m * half] = acc;
+    }
+}
+
 // D partial sums, one chunk of MC blocks per workgroup column:
 //   part[p][c nch + j][k] = sum_{m in chunk j, m >= p} G[c,m,k] conj(X[c,m-p,k])
 // (nch = ceil(M / MC) chunks, m ascending).  A thread owns one (clip, bin

@@ -3,7 +3,9 @@
     python -m partconv_amd.prof_summary trace   <kernel_trace.csv>
     python -m partconv_amd.prof_summary counter <counter_collection.csv> [--calib-grid G]
 
-`trace` prints calls / mean / min / max duration (us) per (kernel, grid, block).
+`trace` prints calls / mean / min / max duration (us) per (kernel, grid, block);
+--split NAME K splits the launches of one shape of the kernels named NAME by
+call index mod K (e.g. the levels of a pass that launches one kernel per level).
 `counter` prints the mean counter value (KiB per dispatch for FETCH_SIZE and
 WRITE_SIZE) per (kernel, grid, counter); with --calib-grid the copy kernel of
 that grid size (a 64 MiB device copy issued by bench.py --calib-copy) gives the
@@ -20,12 +22,19 @@ def _short(name, n=60):
     return name if len(name) <= n else name[:n - 3] + "..."
 
 
-def trace(path):
+def trace(path, split=None):
     rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     d = collections.defaultdict(list)
+    seen = collections.Counter()
     for r in rows:
         key = (r["Kernel_Name"], r.get("Grid_Size_X", r.get("Grid_Size", "")),
                r.get("Workgroup_Size_X", r.get("Workgroup_Size", "")))
+        if split and split[0] in key[0]:
+            # launches of one shape in time order, phase = index mod K
+            ph = seen[key] % split[1]
+            seen[key] += 1
+            key = (f"[call%{split[1]}={ph}] " + key[0], key[1], key[2])
         d[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     print(f"{'kernel':60s} {'grid':>9s} {'block':>5s} {'calls':>6s} {'mean_us':>9s} {'min_us':>8s} {'max_us':>8s}")
     for (n, g, b), v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
@@ -56,9 +65,11 @@ def main():
     ap.add_argument("kind", choices=["trace", "counter"])
     ap.add_argument("path")
     ap.add_argument("--calib-grid", type=int, default=None)
+    ap.add_argument("--split", nargs=2, metavar=("NAME", "K"), default=None,
+                    help="trace: split the launches of kernels whose name contains NAME by call index mod K")
     a = ap.parse_args()
     if a.kind == "trace":
-        trace(a.path)
+        trace(a.path, (a.split[0], int(a.split[1])) if a.split else None)
     else:
         counter(a.path, a.calib_grid)
 

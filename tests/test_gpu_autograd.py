@@ -39,6 +39,7 @@ def relmax(a, b):
     ([256, 2048, 16384], [14, 14, 0], 65536, 40000, 3),
     ([64, 512, 4096], [14, 14, 0], 20000, 15000, 4),
     ([256, 2048], [14, 0], 9000, 8000, 2),
+    ([128, 512], [6, 0], 4000, 7000, 2),        # 5..8 partitions per level
 ])
 def test_forward_and_gradients_match_fp64(N, P, taps, T, C):
     import torch
