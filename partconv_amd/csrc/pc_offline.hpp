@@ -54,6 +54,30 @@ __device__ __forceinline__ float win_sample(const WinSrc &W, int r, int m, int u
     return (i >= 0 && i < W.len) ? W.src[(long long)r * W.row_stride + i] : 0.f;
 }
 
+// The window of transform (r, m) as n contiguous floats, when it covers
+// samples [0, n), lies inside the signal and starts on a 16-byte boundary
+// (every block but the first and last few), else null.
+__device__ __forceinline__ const float4 *win_fast(const WinSrc &W, int r, int m, int n)
+{
+    if (W.ulo != 0 || W.uhi < n) return nullptr;
+    const long long i0 = W.base0 + (long long)m * W.base_step;
+    if (i0 < 0 || i0 + n > W.len) return nullptr;
+    const float *p = W.src + (long long)r * W.row_stride + i0;
+    return (reinterpret_cast<uintptr_t>(p) & 15) ? nullptr : reinterpret_cast<const float4 *>(p);
+}
+
+// The destination of samples [ulo, uhi) of transform (r, m) as contiguous
+// floats, when they all lie inside the signal, start on a 16-byte boundary
+// and number a multiple of 4, else null.
+__device__ __forceinline__ float4 *inv_fast(const InvDst &D, int r, int m)
+{
+    if ((D.uhi - D.ulo) & 3) return nullptr;
+    const long long i0 = D.base0 + (long long)m * D.base_step + D.ulo;
+    if (i0 < 0 || i0 + (D.uhi - D.ulo) > D.len) return nullptr;
+    float *p = D.dst + (long long)r * D.row_stride + i0;
+    return (reinterpret_cast<uintptr_t>(p) & 15) ? nullptr : reinterpret_cast<float4 *>(p);
+}
+
 __device__ __forceinline__ void inv_store(const InvDst &D, int r, int m, int u, float v)
 {
     if (u < D.ulo || u >= D.uhi) return;
@@ -76,8 +100,16 @@ k_off_rfft_small(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict_
     const int t = blockIdx.x;
     const int r = t / W.nb, m = t - r * W.nb;
     const int tid = threadIdx.x;
-    for (int n = tid; n < N; n += NT)
-        bufA[n] = make_float2(win_sample(W, r, m, 2 * n), win_sample(W, r, m, 2 * n + 1));
+    if (const float4 *w4 = win_fast(W, r, m, 2 * N)) {
+        for (int q = tid; q < N / 2; q += NT) {
+            const float4 v = w4[q];
+            bufA[2 * q] = make_float2(v.x, v.y);
+            bufA[2 * q + 1] = make_float2(v.z, v.w);
+        }
+    } else {
+        for (int n = tid; n < N; n += NT)
+            bufA[n] = make_float2(win_sample(W, r, m, 2 * n), win_sample(W, r, m, 2 * n + 1));
+    }
     __syncthreads();
     cf *Z = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
     cf *o = X + (size_t)t * N;
@@ -106,7 +138,20 @@ k_off_irfft_small(const cf *__restrict__ Y, InvDst D, const cf *__restrict__ tw)
     __syncthreads();
     cf *z = small_inverse<LOGN, NT>(bufA, bufB, tw, tid);
     const float *zf = reinterpret_cast<const float *>(z);
-    for (int u = tid; u < 2 * N; u += NT) inv_store(D, r, m, u, zf[u]);
+    if (float4 *d4 = inv_fast(D, r, m)) {
+        const int nq = (D.uhi - D.ulo) / 4;
+        for (int q = tid; q < nq; q += NT) {
+            const float *zq = zf + D.ulo + 4 * q;
+            float4 v = make_float4(zq[0] * D.scale, zq[1] * D.scale, zq[2] * D.scale, zq[3] * D.scale);
+            if (D.accumulate) {
+                const float4 o = d4[q];
+                v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+            }
+            d4[q] = v;
+        }
+    } else {
+        for (int u = tid; u < 2 * N; u += NT) inv_store(D, r, m, u, zf[u]);
+    }
 }
 
 // ---- big N (four-step, as k_big1..3) ---------------------------------------
