@@ -54,14 +54,16 @@ __device__ __forceinline__ float win_sample(const WinSrc &W, int r, int m, int u
     return (i >= 0 && i < W.len) ? W.src[(long long)r * W.row_stride + i] : 0.f;
 }
 
-// The window of transform (r, m) as n contiguous floats, when it covers
-// samples [0, n), lies inside the signal and starts on a 16-byte boundary
-// (every block but the first and last few), else null.
-__device__ __forceinline__ const float4 *win_fast(const WinSrc &W, int r, int m, int n)
+// The window of transform (r, m) as contiguous floats, when its nonzero
+// samples are [0, nz) with nz = min(uhi, n) a multiple of 4, lie inside the
+// signal and start on a 16-byte boundary (every block but the first and last
+// few), else null.
+__device__ __forceinline__ const float4 *win_fast(const WinSrc &W, int r, int m, int n, int *nz)
 {
-    if (W.ulo != 0 || W.uhi < n) return nullptr;
+    *nz = min(W.uhi, n);
+    if (W.ulo != 0 || (*nz & 3)) return nullptr;
     const long long i0 = W.base0 + (long long)m * W.base_step;
-    if (i0 < 0 || i0 + n > W.len) return nullptr;
+    if (i0 < 0 || i0 + *nz > W.len) return nullptr;
     const float *p = W.src + (long long)r * W.row_stride + i0;
     return (reinterpret_cast<uintptr_t>(p) & 15) ? nullptr : reinterpret_cast<const float4 *>(p);
 }
@@ -100,9 +102,10 @@ k_off_rfft_small(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict_
     const int t = blockIdx.x;
     const int r = t / W.nb, m = t - r * W.nb;
     const int tid = threadIdx.x;
-    if (const float4 *w4 = win_fast(W, r, m, 2 * N)) {
+    int nz;
+    if (const float4 *w4 = win_fast(W, r, m, 2 * N, &nz)) {
         for (int q = tid; q < N / 2; q += NT) {
-            const float4 v = w4[q];
+            const float4 v = (4 * q < nz) ? w4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
             bufA[2 * q] = make_float2(v.x, v.y);
             bufA[2 * q + 1] = make_float2(v.z, v.w);
         }
