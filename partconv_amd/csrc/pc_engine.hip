@@ -3631,7 +3631,7 @@ PC_API int pcxr_clear_stats(pc_mimo_rt *r)
 PC_API int pcxr_probe(pc_mimo_rt *r, int level)
 {
     if (!r) return fail(PC_ERR_ARG, "null handle");
-    if (level >= r->M || level == 0) return fail(PC_ERR_ARG, "level out of range");
+    if (level >= r->M || level == 0 || level < -1) return fail(PC_ERR_ARG, "level out of range");
     for (int l = 1; l < r->M; l++) HIPCHK(hipStreamSynchronize(r->s_lvl[l]));
     if (r->pev.empty())
         for (int q = 0; q < 64; q++) {

@@ -139,3 +139,30 @@ def test_long_ir_recompute_mode():
     dh = ht.grad.cpu().numpy()
     assert relmax(dh, f64_corr_h(g, x, taps)) < 1e-4
     assert conv.recomputed_levels == 3
+
+
+def test_probe_times_the_forward_product():
+    """pcc_probe: one timed launch per forward of the probed level, the unique
+    bytes of that product (block spectra in, products out, filter spectra),
+    and a probed forward equal to an unprobed one."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    C, T, taps = 3, 20000, 9000
+    conv = PartitionedConv([256, 2048], [14, 0], taps)
+    x = torch.randn((C, T), device="cuda")
+    h = torch.from_numpy(decaying_noise_ir(taps, 0.3, 48000, 9)).cuda()
+    with torch.no_grad():
+        y0 = conv(x, h)
+        conv.probe(0)
+        for _ in range(3):
+            y1 = conv(x, h)
+        avg_us, n, nbytes = conv.probe_read()
+        conv.probe(-1)
+        y2 = conv(x, h)
+    assert n == 3 and avg_us > 0
+    N, P = conv.N[0], conv.P[0]
+    M = -(-T // N)  # blocks of the 256 level covering the clip
+    assert nbytes == 2 * 8 * C * M * N + 8 * P * N, (nbytes, M)
+    assert torch.equal(y0, y1) and torch.equal(y0, y2)
+    with pytest.raises(RuntimeError):
+        conv.probe(5)

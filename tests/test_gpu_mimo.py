@@ -69,3 +69,31 @@ def test_mimo_realtime_matches_fp64(n_in, n_out, taps, N, P, nfr):
     y2 = torch.zeros_like(yd)
     rt.run_device(y2, xd, nfr)
     assert np.array_equal(y2.cpu().numpy(), y)
+
+
+def test_mimo_realtime_probe():
+    """pcxr_probe times the delay-line sum of a level >= 1 on its stream and
+    reports that launch's algorithmic bytes; probing leaves the output
+    unchanged (bitwise, after a reset)."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    n_in, n_out, taps, N, P, nfr = 5, 3, 10000, [128, 1024], [14, 0], 101
+    irs = cfg4_irs(seed=3, n_in=n_in, n_out=n_out, taps=taps)
+    h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n_out)]).astype(np.float32)).cuda()
+    x = torch.randn((nfr * 128, n_in), device="cuda")
+    rt = MimoRealtime(48000, 128, N, P, taps, n_in, n_out)
+    rt.set_filters(h)
+    y0 = torch.zeros((nfr * 128, n_out), device="cuda")
+    rt.run_device(y0, x, nfr)
+    rt.reset()
+    rt.probe(1)
+    y1 = torch.zeros_like(y0)
+    rt.run_device(y1, x, nfr)
+    avg_us, n, nbytes = rt.probe_read()
+    rt.probe(-1)
+    Nl, Pl = rt.partition()
+    assert n >= nfr // 8 - 1 and avg_us > 0
+    assert nbytes == 8.0 * Nl[1] * (Pl[1] - 1) * (n_out * n_in + n_in) + 8.0 * Nl[1] * n_out
+    assert torch.equal(y0, y1)
+    with pytest.raises(RuntimeError):
+        rt.probe(0)
