@@ -627,11 +627,9 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
         if (!*s_last) return;
         if (tid == 0) {
             trace_stamp(A, f, 6);
-            if (A.done_flag) {
-                __threadfence_system();
+            if (A.done_flag)  // the release store is itself the system-scope release
                 __hip_atomic_store(A.done_flag, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
-            }
         }
         return;
     }
@@ -673,11 +671,9 @@ __device__ __forceinline__ void tick_arrive(const TickJob &TJ, const TickArgs &A
     if (tid == 0) {
         *A.ticket = 0u;
         trace_stamp(A, f, 6);
-        if (A.done_flag) {
-            __threadfence_system();
+        if (A.done_flag)  // the release store is itself the system-scope release
             __hip_atomic_store(A.done_flag, (unsigned long long)(f + 1), __ATOMIC_RELEASE,
                                __HIP_MEMORY_SCOPE_SYSTEM);
-        }
     }
 }
 
