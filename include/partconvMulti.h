@@ -217,4 +217,53 @@ private:
     pc_multi *h_;
 };
 
+/* Real-time MIMO, shared input transforms (no reference counterpart: the
+ * reference runs an n_in x n_out matrix as n_in * n_out PartConvMulti
+ * instances, instance (o, i) with inCh = i, outCh = o, summed into the
+ * outputs).  Each input is transformed once and has one delay line; frames
+ * are device-resident and interleaved as run()'s buffers.  Filters: device
+ * float32 [n_out][n_in][taps]. */
+class PartConvMimo
+{
+public:
+    PartConvMimo() = default;
+    PartConvMimo(const PartConvMimo &) = delete;
+    PartConvMimo &operator=(const PartConvMimo &) = delete;
+    ~PartConvMimo() { cleanup(); }
+
+    int setup(int inSampleRate, int buffer_size, int M, const int *N, const int *P, int taps, int n_in, int n_out)
+    {
+        cleanup();
+        h_ = pcxr_create(inSampleRate, buffer_size, M, N, P, taps, n_in, n_out);
+        if (!h_) {
+            fprintf(stderr, "%s\n", pc_last_error());
+            return 1;
+        }
+        return 0;
+    }
+    int set_filters(const float *d_h) { return report(pcxr_set_filters(h_, d_h)); }
+    /* nframes ticks, each released by the host and waited for */
+    int run_device(float *d_out, const float *d_in, long long nframes)
+    {
+        return report(pcxr_run_device(h_, d_out, d_in, nframes));
+    }
+    int reset() { return report(pcxr_reset(h_)); }
+    int stats(pc_stats *st) const { return pcxr_get_stats(h_, st); }
+    int cleanup()
+    {
+        if (h_) pcxr_destroy(h_);
+        h_ = nullptr;
+        return 0;
+    }
+    pc_mimo_rt *handle() { return h_; }
+
+private:
+    static int report(int rc)
+    {
+        if (rc) fprintf(stderr, "%s\n", pc_last_error());
+        return rc;
+    }
+    pc_mimo_rt *h_ = nullptr;
+};
+
 #endif /* PARTCONV_AMD_PARTCONVMULTI_H */

@@ -141,6 +141,10 @@ int main()
     const float *ins_c[2] = {a, b};
     float **ins = const_cast<float **>(ins_c);
     int rc3 = mx.frameNum == 0 ? 0 : 1;
+    PartConvMimo mimo;
+    int Nm[2] = {256, 4096}, Pm[2] = {30, 0};
+    int rc4 = mimo.setup(48000, 256, 2, Nm, Pm, 65536, 33, 32);  // 33 inputs: rejected
+    if (rc4 == 0) return 2;
     (void)f;
     (void)pr;
     (void)outs;
