@@ -63,7 +63,7 @@ struct MxrArgs {
     long long nticks;
     const unsigned long long *go;
     unsigned long long *flags;  // host-visible, one per output workgroup, 8 words apart
-    unsigned long long *bar;    // grid barrier counter (zeroed before each launch)
+    unsigned long long *bar;    // the tick counters cX and cS, 8 words apart (zeroed before each launch)
     unsigned long long spin_limit;
     int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
 };
