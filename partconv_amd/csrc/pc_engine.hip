@@ -47,6 +47,8 @@
 #define PC_SESSION_IDLE_S 1.0       // host sessions restart after this long without a release
 #define PC_TICK_TIMEOUT_S 10.0
 #define PC_LL_MAX_WORDS 16384       // host sessions use tagged input words up to nIn * N0 of these
+#define PC_WG_FLAGS_MAX 1024        // direct mix: per-workgroup host flags up to this many instances
+                                    // (the host polls them; beyond, device tickets raise one flag)
 #define PC_DH_PT 8                  // dL/dh: partitions per thread (register window)
 #define PC_DH_MC 64                 // dL/dh: blocks per chunk (one workgroup column each)
 
@@ -446,7 +448,9 @@ struct Engine {
     unsigned long long *ll_in = nullptr;
     bool ll_active = false;
     int nwg = 0;
-    bool use_wgflags = false;   // direct mix with few instances: one host flag per workgroup
+    bool use_wgflags = false;   // direct mix: one host flag per workgroup (up to PC_WG_FLAGS_MAX)
+    int wg_scan = 0;            // tick_done: first workgroup not yet seen done for wg_scan_key
+    unsigned long long wg_scan_key = ~0ull;
     uint64_t frame = 0;
     // kernel probe: HIP events around one kernel of one global level
     int probe_g = -1;
@@ -1032,11 +1036,12 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         E.go_d = E.d_flag_map + PC_GO_WORD;
     }
     E.nwg = (E.mode == PC_BATCHED) ? (int)E.inst.size() : 0;
-    E.use_wgflags = (E.mix == 1 && E.nwg <= 32);
+    E.use_wgflags = (E.mix == 1 && E.nwg <= PC_WG_FLAGS_MAX);
     if (E.use_wgflags) {
         HIPCHK(hipHostMalloc(&E.h_wg, wg_flag_words(E) * sizeof(unsigned long long),
                              hipHostMallocCoherent | hipHostMallocMapped));
         memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
+        E.wg_scan_key = ~0ull;
         HIPCHK(hipHostGetDevicePointer((void **)&E.d_wg_map, E.h_wg, 0));
     }
     if (E.mode == PC_BATCHED && E.mix != 0 && (size_t)nIn * N0 <= PC_LL_MAX_WORDS)
@@ -1092,6 +1097,7 @@ static int engine_reset(Engine &E)
         for (size_t g = 0; g < E.gsize.size(); g++) E.lau->done[g].store(0);
     *E.h_flag = 0;
     if (E.h_wg) memset(E.h_wg, 0, wg_flag_words(E) * sizeof(unsigned long long));
+    E.wg_scan_key = ~0ull;  // the flags restart from zero
     return PC_OK;
 }
 
@@ -1111,13 +1117,24 @@ static void record_latency(Engine &E, double us)
 
 // Completion of tick target-1: every per-workgroup flag (direct mix) or the
 // single flag raised by the last workgroup / the mix kernel.
-static bool tick_done(const Engine &E, unsigned long long target)
+static bool tick_done(Engine &E, unsigned long long target)
 {
     if (E.use_wgflags) {
         const size_t stride = E.wg_persist ? PC_WG_FLAG_STRIDE : 1;
         const volatile unsigned long long *w = E.h_wg + (E.wg_persist ? 0 : (size_t)E.nwg * PC_WG_FLAG_STRIDE);
-        for (int i = 0; i < E.nwg; i++)
-            if (w[(size_t)i * stride] < target) return false;
+        // flags only grow: a poll resumes at the first workgroup not yet seen
+        // done for this target (and flag set)
+        const unsigned long long key = 2 * target + (E.wg_persist ? 1 : 0);
+        if (E.wg_scan_key != key) {
+            E.wg_scan_key = key;
+            E.wg_scan = 0;
+        }
+        for (int i = E.wg_scan; i < E.nwg; i++)
+            if (w[(size_t)i * stride] < target) {
+                E.wg_scan = i;
+                return false;
+            }
+        E.wg_scan = E.nwg;
         return true;
     }
     return *(const volatile unsigned long long *)E.h_flag >= target;
