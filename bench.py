@@ -345,6 +345,23 @@ def bench_mimo(args):
     rsecs = time.perf_counter() - t2
     rst = rt.stats()
     r_avg_us, r_launches, r_bytes = rt.probe_read()
+    rt.probe(-1)
+    # host buffers, one call per tick (frames cross the bus; one launch per call)
+    xh = xd[:tps * n0].cpu().numpy()
+    oh = np.zeros((n0, n), dtype=np.float32)
+    hcalls = 4 * tps
+    hlat = []
+    rt.reset()
+    t3 = time.perf_counter()
+    for c in range(hcalls):
+        tc = time.perf_counter()
+        rt.run(oh, np.ascontiguousarray(xh[(c % tps) * n0:(c % tps + 1) * n0]))
+        hlat.append(1e6 * (time.perf_counter() - tc))
+    hsecs = time.perf_counter() - t3
+    host_rt = {"value": n * n0 * hcalls / hsecs, "p50_us": float(np.percentile(hlat, 50)),
+               "p99_us": float(np.percentile(hlat, 99)), "calls": hcalls,
+               "note": "pcxr_run per tick: frame into pinned memory, one persistent launch of one tick, "
+                       "output from pinned memory"}
     del rt
     # throughput form on the matrix cores: same filters, same input
     mx = MimoConv([256, 4096], [30, 0], taps, n, n)
@@ -404,6 +421,7 @@ def bench_mimo(args):
                        "misses": int(rst.deadline_misses), "ticks": int(rst.ticks)},
         "p99_below_period": bool(rst.p99_us < rst.period_us),
         "max_rel_err_vs_fp64": err_rt,
+        "host_buffers_pcie_inclusive": host_rt,
         "roofline": kernel_roofline(r_avg_us, r_launches, r_bytes,
                                     "k_mxr_part (4096 level: the next block's delay-line sum, 32x32 pairs x 14 "
                                     "partitions, one workgroup per bin)"),

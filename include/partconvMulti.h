@@ -247,6 +247,11 @@ public:
     {
         return report(pcxr_run_device(h_, d_out, d_in, nframes));
     }
+    /* one tick on host buffers (interleaved, frameCount == buffer_size) */
+    int run(float *const output, const float *const input, const int frameCount)
+    {
+        return report(pcxr_run(h_, output, input, frameCount));
+    }
     int reset() { return report(pcxr_reset(h_)); }
     int stats(pc_stats *st) const { return pcxr_get_stats(h_, st); }
     int cleanup()

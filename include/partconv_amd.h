@@ -367,6 +367,10 @@ int pcxr_set_filters(pc_mimo_rt *r, const float *d_h);
 /* nframes ticks; d_in / d_out: frames of buffer_size x n_in / n_out interleaved
  * samples on the device (PartConvMulti::run's layout, frame after frame) */
 int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long long nframes);
+/* One tick on host buffers, interleaved as PartConvMulti::run's
+ * (src/partconvMulti.cpp:815-928; frameCount == buffer_size, else an error):
+ * what a real-time callback calls for a MIMO network. */
+int pcxr_run(pc_mimo_rt *r, float *out, const float *in, int frameCount);
 int pcxr_reset(pc_mimo_rt *r);
 int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st);
 int pcxr_clear_stats(pc_mimo_rt *r);

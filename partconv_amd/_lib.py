@@ -112,6 +112,7 @@ SIGNATURES = {
                           ctypes.c_int]),
     "pcxr_set_filters": (ctypes.c_int, [_vp, _vp]),
     "pcxr_run_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong]),
+    "pcxr_run": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
     "pcxr_reset": (ctypes.c_int, [_vp]),
     "pcxr_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
     "pcxr_clear_stats": (ctypes.c_int, [_vp]),

@@ -3259,6 +3259,8 @@ struct pc_mimo_rt {
     int probe_level = -1;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pev;
     int pev_used = 0;
+    // host-buffer ticks: pinned frames the kernel reads and writes in place
+    float *h_in = nullptr, *h_out = nullptr, *d_in_map = nullptr, *d_out_map = nullptr;
 };
 
 static void mxr_free(pc_mimo_rt *r)
@@ -3283,6 +3285,9 @@ static void mxr_free(pc_mimo_rt *r)
     if (r->h_flags) hipHostFree(r->h_flags);
     if (r->h_go_pinned) hipHostFree(r->h_go_pinned);
     r->h_flags = r->h_go_pinned = nullptr;
+    if (r->h_in) hipHostFree(r->h_in);
+    if (r->h_out) hipHostFree(r->h_out);
+    r->h_in = r->h_out = r->d_in_map = r->d_out_map = nullptr;
     release_slot_put(r->dev, r->go_slot);
     r->go_slot = -1;
 }
@@ -3592,6 +3597,28 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     if (err) mxr_store_go(r, PC_GO_STOP);  // the kernel's waits end at once
     HIPCHK(hipStreamSynchronize(r->s_main));
     return err;
+}
+
+// One tick on host buffers, interleaved as PartConvMulti::run's
+// (src/partconvMulti.cpp:815-928, frameCount == buffer_size): the frame is
+// copied into pinned memory, which the kernel reads in place, and the kernel
+// writes the output frame to pinned memory, copied out after the tick.
+PC_API int pcxr_run(pc_mimo_rt *r, float *out, const float *in, int frameCount)
+{
+    if (!r || !out || !in) return fail(PC_ERR_ARG, "null argument");
+    if (frameCount != r->N0) return fail(PC_ERR_ARG, "frameCount must equal buffer_size");
+    const size_t nin = (size_t)r->n_in * r->N0 * sizeof(float), nout = (size_t)r->n_out * r->N0 * sizeof(float);
+    if (!r->h_in) {
+        HIPCHK(hipHostMalloc(&r->h_in, nin, hipHostMallocCoherent | hipHostMallocMapped));
+        HIPCHK(hipHostMalloc(&r->h_out, nout, hipHostMallocCoherent | hipHostMallocMapped));
+        HIPCHK(hipHostGetDevicePointer((void **)&r->d_in_map, r->h_in, 0));
+        HIPCHK(hipHostGetDevicePointer((void **)&r->d_out_map, r->h_out, 0));
+    }
+    memcpy(r->h_in, in, nin);
+    const int rc = pcxr_run_device(r, r->d_out_map, r->d_in_map, 1);
+    if (rc) return rc;
+    memcpy(out, r->h_out, nout);
+    return PC_OK;
 }
 
 PC_API int pcxr_reset(pc_mimo_rt *r)

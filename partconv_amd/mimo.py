@@ -12,6 +12,7 @@ runs on v_mfma_f32_32x32x2f32.
     rt = MimoRealtime(48000, 256, [256, 4096], [30, 0], taps=65536, n_in=32, n_out=32)
     rt.set_filters(h)
     rt.run_device(y, x, nframes)   # x: [nframes*256, n_in], y: [nframes*256, n_out] cuda, interleaved
+    rt.run(out, inp)               # or one tick on host arrays [256, n_in] -> [256, n_out]
 
 runs the same network tick by tick in real time: the output equals one
 PartConvMulti instance per (input, output) pair summed per output (the
@@ -98,6 +99,19 @@ class MimoRealtime:
         torch.cuda.synchronize()
         if lib().pcxr_run_device(self._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                                  int(nframes)) != 0:
+            raise RuntimeError(last_error())
+
+    def run(self, out, inp):
+        """One tick on host buffers: inp float32 [buffer_size, n_in], out float32
+        [buffer_size, n_out], C-contiguous numpy arrays (PartConvMulti.run's
+        interleaved layout)."""
+        import numpy as np
+        if (not isinstance(inp, np.ndarray) or not isinstance(out, np.ndarray) or inp.dtype != np.float32
+                or out.dtype != np.float32 or inp.shape != (self.buffer_size, self.n_in)
+                or out.shape != (self.buffer_size, self.n_out) or not inp.flags.c_contiguous
+                or not out.flags.c_contiguous):
+            raise ValueError("run takes float32 C-contiguous [buffer_size, n_in] / [buffer_size, n_out] arrays")
+        if lib().pcxr_run(self._h, out.ctypes.data, inp.ctypes.data, self.buffer_size) != 0:
             raise RuntimeError(last_error())
 
     def reset(self):

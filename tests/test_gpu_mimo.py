@@ -97,3 +97,29 @@ def test_mimo_realtime_probe():
     assert torch.equal(y0, y1)
     with pytest.raises(RuntimeError):
         rt.probe(0)
+
+
+def test_mimo_realtime_host_buffer_ticks():
+    """pcxr_run: one tick per call on host buffers (PartConvMulti::run's
+    interleaved layout) gives the same output as the device-resident run,
+    bitwise; a wrong frameCount is an error."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    n_in, n_out, taps, N, P, nfr, n0 = 5, 3, 10000, [128, 1024], [14, 0], 41, 128
+    irs = cfg4_irs(seed=5, n_in=n_in, n_out=n_out, taps=taps)
+    h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n_out)]).astype(np.float32)).cuda()
+    x = np.random.default_rng(4).standard_normal((nfr * n0, n_in)).astype(np.float32)
+    rt = MimoRealtime(48000, n0, N, P, taps, n_in, n_out)
+    rt.set_filters(h)
+    y_host = np.zeros((nfr * n0, n_out), dtype=np.float32)
+    out = np.zeros((n0, n_out), dtype=np.float32)
+    for f in range(nfr):
+        rt.run(out, np.ascontiguousarray(x[f * n0:(f + 1) * n0]))
+        y_host[f * n0:(f + 1) * n0] = out
+    assert rt.stats().ticks == nfr
+    rt.reset()
+    yd = torch.zeros((nfr * n0, n_out), device="cuda")
+    rt.run_device(yd, torch.from_numpy(x).cuda(), nfr)
+    assert np.array_equal(yd.cpu().numpy(), y_host)
+    with pytest.raises(ValueError):
+        rt.run(np.zeros((64, n_out), np.float32), np.zeros((64, n_in), np.float32))
