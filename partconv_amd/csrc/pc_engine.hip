@@ -3261,10 +3261,94 @@ struct pc_mimo_rt {
     int pev_used = 0;
     // host-buffer ticks: pinned frames the kernel reads and writes in place
     float *h_in = nullptr, *h_out = nullptr, *d_in_map = nullptr, *d_out_map = nullptr;
+    // level launches off the tick thread (as the instance engine's helper):
+    // queued (level, start tick); done[l] = start tick + 1 once issued
+    std::thread lth;
+    std::mutex lmu;
+    std::condition_variable lcv;
+    std::vector<std::pair<int, uint64_t>> lq;
+    bool lstop = false;
+    int lbusy = 0;
+    std::unique_ptr<std::atomic<uint64_t>[]> ldone;
+    std::vector<uint64_t> lstart;
+    int lerr = 0;
+    std::string lmsg;
 };
+
+static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f);
+
+static void mxr_launcher_main(pc_mimo_rt *r)
+{
+    (void)hipSetDevice(r->dev);
+    for (;;) {
+        std::pair<int, uint64_t> job;
+        {
+            std::unique_lock<std::mutex> lk(r->lmu);
+            r->lcv.wait(lk, [&] { return r->lstop || !r->lq.empty(); });
+            if (r->lq.empty()) return;  // stopped with nothing left
+            job = r->lq.front();
+            r->lq.erase(r->lq.begin());
+        }
+        const int rc = mxr_start_level(r, job.first, job.second);
+        std::lock_guard<std::mutex> lk(r->lmu);
+        if (rc && !r->lerr) {
+            r->lerr = rc;
+            r->lmsg = g_err;
+        }
+        r->ldone[job.first].store(job.second + 1, std::memory_order_release);
+        r->lbusy--;
+        r->lcv.notify_all();
+    }
+}
+
+static void mxr_launcher_start(pc_mimo_rt *r)
+{
+    if (r->lth.joinable()) return;
+    r->ldone.reset(new std::atomic<uint64_t>[r->M]);
+    for (int l = 0; l < r->M; l++) r->ldone[l].store(0);
+    r->lstart.assign(r->M, 0);
+    r->lstop = false;
+    r->lth = std::thread(mxr_launcher_main, r);
+}
+
+static void mxr_launcher_queue(pc_mimo_rt *r, int l, uint64_t f)
+{
+    {
+        std::lock_guard<std::mutex> lk(r->lmu);
+        r->lq.push_back({l, f});
+        r->lbusy++;
+    }
+    r->lcv.notify_all();
+}
+
+// Wait until every queued launch is issued; returns its first error.
+static int mxr_launcher_drain(pc_mimo_rt *r)
+{
+    if (!r->lth.joinable()) return PC_OK;
+    std::unique_lock<std::mutex> lk(r->lmu);
+    r->lcv.wait(lk, [&] { return r->lbusy == 0; });
+    if (r->lerr) {
+        const int rc = r->lerr;
+        r->lerr = 0;
+        return fail(rc, r->lmsg);
+    }
+    return PC_OK;
+}
+
+static void mxr_launcher_stop(pc_mimo_rt *r)
+{
+    if (!r->lth.joinable()) return;
+    {
+        std::lock_guard<std::mutex> lk(r->lmu);
+        r->lstop = true;
+    }
+    r->lcv.notify_all();
+    r->lth.join();
+}
 
 static void mxr_free(pc_mimo_rt *r)
 {
+    mxr_launcher_stop(r);
     for (auto &e : r->pev) {
         hipEventDestroy(e.first);
         hipEventDestroy(e.second);
@@ -3483,7 +3567,6 @@ static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(r->ev_lvl[l], r->s_lvl[l]));
-    r->pending[l] = true;
     return PC_OK;
 }
 
@@ -3549,6 +3632,7 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     case 10: le = mxr_launch_t<10>(r, G); break;
     }
     if (le != hipSuccess) return fail(PC_ERR_HIP, std::string("real-time MIMO launch: ") + hipGetErrorString(le));
+    if (r->M > 1) mxr_launcher_start(r);
     const double period = 1e6 * N0 / r->FS;
     int err = PC_OK;
     for (long long t = 0; t < nframes && !err; t++) {
@@ -3558,6 +3642,14 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
         for (int l = 1; l < r->M && !err; l++) {
             const uint64_t k = 1ull << A.lv[l].kshift;
             if ((f + 1) % k != 0 || !r->pending[l]) continue;
+            // the helper thread has issued the block's launches and its event
+            while (r->ldone[l].load(std::memory_order_acquire) < r->lstart[l] + 1) {
+                if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
+                    err = fail(PC_ERR_TIMEOUT, "level block was not launched");
+                    break;
+                }
+            }
+            if (err) break;
             for (;;) {
                 const hipError_t q = hipEventQuery(r->ev_lvl[l]);
                 if (q == hipSuccess) break;
@@ -3592,8 +3684,14 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
         if (us > period) r->misses++;
         r->frame++;
         for (int l = 1; l < r->M && !err; l++)
-            if (f % (1ull << A.lv[l].kshift) == 0) err = mxr_start_level(r, l, f);
+            if (f % (1ull << A.lv[l].kshift) == 0) {
+                r->lstart[l] = f;
+                r->pending[l] = true;
+                mxr_launcher_queue(r, l, f);
+            }
     }
+    const int lrc = mxr_launcher_drain(r);
+    if (!err && lrc) err = lrc;
     if (err) mxr_store_go(r, PC_GO_STOP);  // the kernel's waits end at once
     HIPCHK(hipStreamSynchronize(r->s_main));
     return err;
@@ -3624,8 +3722,12 @@ PC_API int pcxr_run(pc_mimo_rt *r, float *out, const float *in, int frameCount)
 PC_API int pcxr_reset(pc_mimo_rt *r)
 {
     if (!r) return fail(PC_ERR_ARG, "null handle");
+    (void)mxr_launcher_drain(r);  // a failed launch is moot: the state is cleared below
     HIPCHK(hipStreamSynchronize(r->s_main));
     for (int l = 1; l < r->M; l++) HIPCHK(hipStreamSynchronize(r->s_lvl[l]));
+    // the tick counter restarts: so do the launch marks
+    if (r->ldone)
+        for (int l = 0; l < r->M; l++) r->ldone[l].store(0, std::memory_order_relaxed);
     const MxrArgs &A = r->A;
     HIPCHK(hipMemsetAsync(A.ring, 0, (size_t)r->n_in * 2 * A.R * sizeof(float), r->s_main));
     for (int l = 0; l < r->M; l++) {
