@@ -73,10 +73,13 @@ class PartitionedConv:
         return a.value, n.value, b.value
 
     def set_filter(self, h):
+        # under graph capture the filter spectra are always recomputed, so a
+        # replay sees the filter's values at replay time
+        capturing = torch.cuda.is_current_stream_capturing()
         key = (h.data_ptr(), h._version)
-        if key != self._filter_key:
+        if capturing or key != self._filter_key:
             self._check(lib().pcc_set_filter(self._h, _p(h), _stream()))
-            self._filter_key = key
+            self._filter_key = None if capturing else key
 
     def forward_raw(self, x, h, y):
         self.set_filter(h)
