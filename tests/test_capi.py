@@ -160,3 +160,26 @@ int main()
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.split() == ["1", "1", "0"]
+
+
+def test_prof_summary_splits_calls_by_phase(tmp_path, capsys):
+    """prof_summary trace --split: launches of one kernel shape are grouped by
+    call index mod K in time order (one row per level of a per-level pass)."""
+    from partconv_amd import prof_summary
+    rows = ["Kernel_Name,Grid_Size_X,Workgroup_Size_X,Start_Timestamp,End_Timestamp"]
+    t = 0
+    for call in range(6):  # durations 1, 2, 3, 1, 2, 3 us
+        d = 1000 * (call % 3 + 1)
+        rows.append(f"k_mac(int),256,256,{t},{t + d}")
+        t += d + 100
+    rows.append(f"k_other(int),64,64,{t},{t + 500}")
+    p = tmp_path / "trace.csv"
+    p.write_text("\n".join(rows) + "\n")
+    prof_summary.trace(str(p), ("k_mac", 3))
+    out = capsys.readouterr().out
+    means = {}
+    for line in out.splitlines():
+        if line.startswith("[call%3="):
+            means[int(line[8])] = float(line.split()[-3])
+    assert means == {0: 1.0, 1: 2.0, 2: 3.0}
+    assert "k_other(int)" in out
