@@ -3431,7 +3431,7 @@ static int mxr_setup(pc_mimo_rt *r)
         HIPCHK(hipMemcpy(d_lv, A.lv, (size_t)M * sizeof(MxrLevel), hipMemcpyHostToDevice));
         A.lvd = d_lv;
     }
-    if ((rc = mxr_alloc(r, (void **)&r->d_bar, 128))) return rc;  // two counters, own lines
+    if ((rc = mxr_alloc(r, (void **)&r->d_bar, 640))) return rc;  // cX and 8 cS groups, own lines
     HIPCHK(hipStreamCreateWithFlags(&r->s_main, hipStreamNonBlocking));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, r->dev));
@@ -3621,7 +3621,7 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     // the grid: input and output workgroups, then one delay-line-sum workgroup
     // per bin (at most 256)
     const int G = r->n_in + r->n_out + std::min(N0, 256);
-    HIPCHK(hipMemsetAsync(r->d_bar, 0, 128, r->s_main));
+    HIPCHK(hipMemsetAsync(r->d_bar, 0, 640, r->s_main));
     mxr_store_go(r, r->frame);
     hipError_t le = hipErrorInvalidValue;
     switch (ilog2(N0)) {

@@ -63,7 +63,8 @@ struct MxrArgs {
     long long nticks;
     const unsigned long long *go;
     unsigned long long *flags;  // host-visible, one per output workgroup, 8 words apart
-    unsigned long long *bar;    // the tick counters cX and cS, 8 words apart (zeroed before each launch)
+    unsigned long long *bar;    // the tick counters: cX, then cS in 8 groups, 8 words apart (zeroed
+                                // before each launch)
     unsigned long long spin_limit;
     int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
 };
@@ -157,7 +158,9 @@ __device__ __forceinline__ int mxr_wait(const unsigned long long *ctr, unsigned 
     return *s_ok;
 }
 
-// The same for two counters at once (both loads in flight per poll).
+// The same for counter c1 and the MXR_SG group counters c2[8 g] at once
+// (every load in flight per poll).
+#define MXR_SG 8
 __device__ __forceinline__ int mxr_wait2(const unsigned long long *c1, unsigned long long t1,
                                          const unsigned long long *c2, unsigned long long t2,
                                          unsigned long long limit, int tid, int *s_ok)
@@ -167,7 +170,10 @@ __device__ __forceinline__ int mxr_wait2(const unsigned long long *c1, unsigned 
         int ok = 1;
         for (;;) {
             const unsigned long long v1 = __hip_atomic_load(c1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned long long v2 = __hip_atomic_load(c2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long v2 = ~0ull;
+#pragma unroll
+            for (int g = 0; g < MXR_SG; g++)
+                v2 = min(v2, __hip_atomic_load(c2 + 8 * g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             if (v1 >= t1 && v2 >= t2) break;
             if (__builtin_amdgcn_s_memrealtime() > t_end) {
                 ok = 0;
@@ -300,7 +306,7 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                     }
             }
             // this tick's delay-line rows and S for this tick
-            if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * G4, A.spin_limit,
+            if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * (G4 / MXR_SG), A.spin_limit,
                            tid, &s_ok))
                 return;
             const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0;
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
             cf *Sn = L0.S + (size_t)((f + 1) & 1) * A.n_out * N0;
             for (int k = w - n_io; k < N0; k += G4)
                 mxr_bin<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, Sn, tid);
-            mxr_arrive(cS, tid);
+            mxr_arrive(cS + 8 * ((w - n_io) % MXR_SG), tid);  // group counters: 8 lines
         }
         __syncthreads();
     }
