@@ -457,6 +457,9 @@ def bench_diff(args):
     with torch.no_grad():
         tgt = conv(x, hs) + 1e-3 * torch.randn((C, T), dtype=torch.float32, device="cuda", generator=g)
 
+    # chunk sizes of the spectral products and dL/dh chosen by timing on these clips
+    tuning = conv.tune(x, h0.detach())
+
     def step():
         h0.grad = None
         loss = torch.mean((conv(x, h0) - tgt) ** 2)
@@ -560,6 +563,7 @@ def bench_diff(args):
         "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
                               "ms_per_step": 1e3 * secs2 / args.steps,
                               "recomputed_levels": int(conv2.recomputed_levels)},
+        "tuned_chunks": {"levels": list(conv.N), "mac_dh_blocks_per_chunk": tuning},
         "hip_graph_replay": {"ms_per_step": 1e3 * gsecs / args.steps, "value": C * T * args.steps / gsecs,
                              "grad_equals_eager": graph_ok,
                              "note": "forward + adjoint captured once (torch.cuda.graph) and replayed; loss "

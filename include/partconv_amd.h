@@ -321,6 +321,14 @@ int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, fl
 int pcc_set_budget(pc_conv *c, long long bytes);
 /* Number of level spectra recomputed by backward so far. */
 long long pcc_recomputed_levels(const pc_conv *c);
+/* Autotuner (the reference tunes partitions only, src/tune.py): choose each
+ * level's chunk sizes (blocks per workgroup column) of the spectral products
+ * and of the dL/dh partial sums by timing candidates with HIP events on the
+ * caller's clips d_x [C][T].  Setup-time: allocates and synchronises.  The
+ * forward output does not depend on the choice; dL/dh sums are grouped by it
+ * (deterministic for a fixed choice).  pcc_level_tuning reads the choice. */
+int pcc_tune(pc_conv *c, const float *d_x, int C, int T, void *stream);
+int pcc_level_tuning(const pc_conv *c, int level, int *mac_mc, int *dh_mc);
 /* Measurement hook, as pcm_probe (no reference counterpart): time with HIP
  * events stamped by the launch itself the forward spectral product (k_off_mac)
  * of level `level` each time it launches (-1 disables); read back the mean

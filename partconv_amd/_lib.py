@@ -100,6 +100,8 @@ SIGNATURES = {
     "pcc_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_set_budget": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),
+    "pcc_tune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pcc_level_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pcc_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcc_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_double)]),

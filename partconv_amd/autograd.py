@@ -61,6 +61,20 @@ class PartitionedConv:
         if rc != 0:
             raise RuntimeError(last_error())
 
+    def tune(self, x, h):
+        """Choose each level's chunk sizes by timing candidates on clips x [C, T]
+        with filter h; returns [(mac_chunk, dh_chunk)] per level."""
+        x = _prep(x, "x")
+        self.set_filter(_prep(h, "h"))
+        torch.cuda.synchronize()
+        self._check(lib().pcc_tune(self._h, _p(x), x.shape[0], x.shape[1], _stream()))
+        out = []
+        for l in range(len(self.N)):
+            a, b = ctypes.c_int(), ctypes.c_int()
+            self._check(lib().pcc_level_tuning(self._h, l, ctypes.byref(a), ctypes.byref(b)))
+            out.append((a.value, b.value))
+        return out
+
     def probe(self, level):
         """Time the forward spectral product of one level with HIP events (-1 disables)."""
         self._check(lib().pcc_probe(self._h, int(level)))

@@ -50,7 +50,6 @@
 #define PC_WG_FLAGS_MAX 1024        // direct mix: per-workgroup host flags up to this many instances
                                     // (the host polls them; beyond, device tickets raise one flag)
 #define PC_DH_PT 8                  // dL/dh: partitions per thread (register window)
-#define PC_DH_MC 64                 // dL/dh: blocks per chunk (one workgroup column each)
 
 static thread_local std::string g_err;
 
@@ -2651,6 +2650,9 @@ struct ConvLevel {
     bool big;
     BigShape sh;
     cf *H = nullptr;
+    // blocks per chunk (one workgroup column each) of the spectral products
+    // and of the dL/dh partial sums: defaults, or chosen by pcc_tune
+    int mac_mc = 32, dh_mc = 64;
 };
 
 struct pc_conv {
@@ -2683,7 +2685,6 @@ struct pc_conv {
 // causal, -1: adjoint): the register-window kernel (window of 4, 8 or 16)
 // for up to 16 partitions, else one thread per (clip, block, bin pair).  With events (e0, e1), the
 // launch stamps them.
-#define PC_MAC_MC 32  // blocks per chunk of the register-window kernel
 template <typename K, typename... Args>
 static void off_launch(K kern, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args)
 {
@@ -2697,8 +2698,8 @@ static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLev
                     hipEvent_t e0, hipEvent_t e1)
 {
     const long long items = (long long)C * (L.N / 2);
-    const dim3 grid((unsigned)((items + 255) / 256), (unsigned)((M + PC_MAC_MC - 1) / PC_MAC_MC));
-    const int mc = PC_MAC_MC;
+    const int mc = L.mac_mc;
+    const dim3 grid((unsigned)((items + 255) / 256), (unsigned)((M + mc - 1) / mc));
     if (L.P <= 4) {
         if (dir > 0) off_launch(k_off_mac_w<4, 1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
         else off_launch(k_off_mac_w<4, -1>, grid, s, e0, e1, X, H, Y, C, M, L.N, L.P, mc);
@@ -2712,6 +2713,10 @@ static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLev
         off_launch(k_off_mac, dim3((unsigned)((items * M + 255) / 256)), s, e0, e1, X, H, Y, C, M, L.N, L.P, dir);
     }
 }
+
+// dL/dh of one level into D (P spectra): partial sums per clip and chunk of
+// L.dh_mc blocks, PC_DH_PT partitions per thread, then the fixed-order reduce.
+static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s);
 
 // Fold the probe's pending events into its totals (waits for them).
 static int conv_probe_fold(pc_conv *c)
@@ -2736,6 +2741,23 @@ static int conv_grow(cf **p, size_t *cap, size_t need)
     *p = nullptr;
     HIPCHK(hipMalloc(p, need * sizeof(cf)));
     *cap = need;
+    return PC_OK;
+}
+
+static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s)
+{
+    const int nch = (M + L.dh_mc - 1) / L.dh_mc;
+    int rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * nch * L.N);
+    if (rc) return rc;
+    const long long items = (long long)C * (L.N / 2);
+    hipLaunchKernelGGL(k_off_dh_part<PC_DH_PT>,
+                       dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
+                            (unsigned)nch),
+                       dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+    const long long nd = (long long)L.P * L.N;
+    hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, c->dpart, D, C * nch,
+                       L.N, L.P);
+    HIPCHK(hipGetLastError());
     return PC_OK;
 }
 
@@ -3039,18 +3061,7 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
                 if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, s))) return rc;
                 c->recomputed_levels++;
             }
-            // partial sums per clip and chunk of PC_DH_MC blocks, PC_DH_PT partitions per thread
-            const int nch = (M + PC_DH_MC - 1) / PC_DH_MC;
-            if ((rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * nch * L.N))) return rc;
-            const long long items = (long long)C * (L.N / 2);
-            hipLaunchKernelGGL(k_off_dh_part<PC_DH_PT>,
-                               dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
-                                    (unsigned)nch),
-                               dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P, PC_DH_MC);
-            const long long nd = (long long)L.P * L.N;
-            hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, c->dpart, Wsp,
-                               C * nch, L.N, L.P);
-            HIPCHK(hipGetLastError());
+            if ((rc = off_dh(c, L, G, X, Wsp, C, M, s))) return rc;
             InvDst D{};
             D.dst = d_dh;
             D.row_stride = 0;
@@ -3066,6 +3077,85 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
             if ((rc = off_irfft(c, L, Wsp, D, 1, s))) return rc;
         }
     }
+    return PC_OK;
+}
+
+// Choose each level's chunk sizes by timing (HIP events) the spectral product
+// and the dL/dh partial sums on the caller's clips: 3 launches per candidate,
+// the fastest mean kept.  The products' results do not depend on the chunk
+// size; dL/dh's partial sums are grouped by it (a fixed order for a fixed
+// choice).  Setup-time: allocates, synchronises.
+PC_API int pcc_tune(pc_conv *c, const float *d_x, int C, int T, void *stream)
+{
+    if (!c || !d_x || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "tune before set_filter");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = PC_OK;
+    const int mac_c[] = {8, 16, 32, 64, 128}, dh_c[] = {16, 32, 64, 128, 256};
+    for (size_t li = 0; li < c->lv.size() && !rc; li++) {
+        ConvLevel &L = c->lv[li];
+        WinSrc W = conv_xwin(L, d_x, T);
+        const int M = W.nb;
+        const size_t n = (size_t)C * M * L.N;
+        if ((rc = conv_grow(&c->dsum, &c->dsum_cap, n))) break;
+        // products (n) or the level's dL/dh spectra (P N)
+        if ((rc = conv_grow(&c->work, &c->work_cap, std::max(n, (size_t)L.P * L.N)))) break;
+        if ((rc = off_rfft(c, L, W, C, c->dsum, s))) break;
+        auto time_it = [&](auto launch, double *best_ms, int *best, int cand) -> int {
+            float tot = 0.f;
+            for (int r = 0; r < 4; r++) {
+                HIPCHK(hipEventRecord(e0, s));
+                int lrc = launch();
+                if (lrc) return lrc;
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) tot += ms;  // the first launch warms up
+            }
+            if (tot < *best_ms) {
+                *best_ms = tot;
+                *best = cand;
+            }
+            return PC_OK;
+        };
+        if (L.P <= 16) {
+            double best_ms = 1e30;
+            int best = L.mac_mc;
+            for (int mc : mac_c) {
+                L.mac_mc = mc;
+                if ((rc = time_it([&]() { off_mac(c->dsum, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
+                                          return PC_OK; }, &best_ms, &best, mc)))
+                    break;
+            }
+            L.mac_mc = best;
+            if (rc) break;
+        }
+        double best_ms = 1e30;
+        int best = L.dh_mc;
+        // the block spectra stand in for the gradient spectra (same shape)
+        for (int mc : dh_c) {
+            L.dh_mc = mc;
+            if ((rc = time_it([&]() { return off_dh(c, L, c->dsum, c->dsum, c->work, C, M, s); }, &best_ms, &best,
+                              mc)))
+                break;
+        }
+        L.dh_mc = best;
+    }
+    c->keep_valid = false;  // the work space held tuning data
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+PC_API int pcc_level_tuning(const pc_conv *c, int level, int *mac_mc, int *dh_mc)
+{
+    if (!c || level < 0 || level >= (int)c->lv.size()) return fail(PC_ERR_ARG, "bad argument");
+    if (mac_mc) *mac_mc = c->lv[level].mac_mc;
+    if (dh_mc) *dh_mc = c->lv[level].dh_mc;
     return PC_OK;
 }
 

@@ -217,3 +217,33 @@ def test_forward_and_adjoint_replay_in_a_cuda_graph():
         assert torch.equal(ys, ye), r
         assert torch.equal(xs.grad, xe.grad), r
         assert torch.equal(hs.grad, he.grad), r
+
+
+def test_tuned_chunk_sizes_keep_results():
+    """pcc_tune picks each level's chunk sizes from its candidates by timing;
+    the forward output is bitwise that of the untuned form, and both gradients
+    stay within the fp64 bars."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    N, P, taps, T, C = [128, 512, 4096], [14, 14, 0], 20000, 15000, 3
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((C, T)).astype(np.float32)
+    h = decaying_noise_ir(taps, 0.8, 48000, 8)
+    g = rng.standard_normal((C, T)).astype(np.float32)
+    xt0 = torch.from_numpy(x).cuda()
+    ht0 = torch.from_numpy(h).cuda()
+    plain = PartitionedConv(N, P, taps)
+    tuned = PartitionedConv(N, P, taps)
+    choice = tuned.tune(xt0, ht0)
+    assert len(choice) == 3
+    for lvl, (mac, dh) in enumerate(choice):
+        assert dh in (16, 32, 64, 128, 256), choice
+        assert mac in (8, 16, 32, 64, 128), choice
+    with torch.no_grad():
+        assert torch.equal(plain(xt0, ht0), tuned(xt0, ht0))
+    xt = xt0.clone().requires_grad_(True)
+    ht = ht0.clone().requires_grad_(True)
+    tuned(xt, ht).backward(torch.from_numpy(g).cuda())
+    torch.cuda.synchronize()
+    assert relmax(xt.grad.cpu().numpy(), f64_corr_x(g, h)) < 1e-4
+    assert relmax(ht.grad.cpu().numpy(), f64_corr_h(g, x, taps)) < 1e-4
