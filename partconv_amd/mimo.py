@@ -154,3 +154,29 @@ class MimoRealtime:
                 self._h = None
         except Exception:
             pass
+
+
+def _throughput_demo():
+    """The MFMA throughput form at 32 x 32 pairs, 65536-tap filters,
+    [256 x 30, 4096 x 15], 172032 samples per call (three calls): a fixed
+    workload for hardware counters (rocprofv3 --pmc MfmaUtil MfmaFlopsF32 --
+    python -u -m partconv_amd.mimo)."""
+    import time
+    n, taps, T = 32, 65536, 172032
+    g = torch.Generator(device="cuda").manual_seed(4)
+    h = torch.randn((n, n, taps), device="cuda", generator=g) * (1.0 / 256)
+    x = torch.randn((n, T), device="cuda", generator=g)
+    mx = MimoConv([256, 4096], [30, 0], taps, n, n)
+    mx.set_filters(h)
+    mx(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        mx(x)
+    torch.cuda.synchronize()
+    print(f"throughput form: {1e3 * (time.perf_counter() - t0) / 3:.3f} ms per call of {T} samples x {n} outputs",
+          flush=True)
+
+
+if __name__ == "__main__":
+    _throughput_demo()
