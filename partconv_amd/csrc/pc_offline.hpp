@@ -597,6 +597,83 @@ k_mimo_mfma(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__restrict
     }
 }
 
+// The same product with MT waves per workgroup, one m-tile each: the
+// partition-p filter block H_k,p (n_in x 32 complex) is staged in LDS once for
+// the workgroup (double-buffered, the next partition's loads in flight during
+// this one's MFMAs) instead of being read from memory by every m-tile.  Each
+// wave runs exactly k_mimo_mfma's sequence of MFMAs, so the results are the
+// same bit for bit.  Grid: (N, ceil(ceil(M / 32) / MT)), block 64 MT.
+template <int MT>
+__global__ void __launch_bounds__(64 * MT)
+k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__restrict__ Y, int N, int M, int P,
+                int n_in, int n_out)
+{
+    constexpr int NT = 64 * MT, HS = 32 * 32;  // n_in <= 32
+    constexpr int PER = (HS + NT - 1) / NT;
+    __shared__ cf hs[2][HS];
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+    const int m0 = (blockIdx.y * MT + wv) * 32;
+    const bool wave_on = m0 < M;
+    const int lo = l & 31, hi = l >> 5;
+    const int nh = n_in * 32;
+    f32x16 accr, acci;
+    for (int v = 0; v < 16; v++) {
+        accr[v] = 0.f;
+        acci[v] = 0.f;
+    }
+    const bool bin0 = (k == 0);
+    const int m = m0 + lo;
+    const cf *Xk = Xt + (size_t)k * n_in * M;
+    const cf *Hk = Ht + (size_t)k * P * n_in * 32;
+    for (int e = tid; e < nh; e += NT) hs[0][e] = Hk[e];
+    __syncthreads();
+    for (int p = 0; p < P; p++) {
+        cf pre[PER];
+        const cf *Hn = Hk + (size_t)(p + 1) * n_in * 32;
+        if (p + 1 < P) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + q * NT;
+                if (e < nh) pre[q] = Hn[e];
+            }
+        }
+        if (wave_on) {
+            const cf *hp = hs[p & 1];
+            const int mm = m - p;
+            const bool mv = (mm >= 0 && mm < M);
+            for (int j = 0; j < n_in; j++) {
+                const cf h = hp[j * 32 + lo];
+                const cf x = mv ? Xk[(size_t)j * M + mm] : make_float2(0.f, 0.f);
+                float ar, ai;
+                if (!bin0) {
+                    ar = hi ? -h.y : h.x;
+                    ai = hi ? h.x : h.y;
+                } else {
+                    ar = hi ? 0.f : h.x;
+                    ai = hi ? h.y : 0.f;
+                }
+                const float b = hi ? x.y : x.x;
+                accr = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, b, accr, 0, 0, 0);
+                acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, b, acci, 0, 0, 0);
+            }
+        }
+        if (p + 1 < P) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = tid + q * NT;
+                if (e < nh) hs[(p + 1) & 1][e] = pre[q];
+            }
+        }
+        __syncthreads();
+    }
+    if (!wave_on || m >= M) return;
+    for (int v = 0; v < 16; v++) {
+        const int o = 8 * (v >> 2) + 4 * hi + (v & 3);
+        if (o < n_out) Y[((size_t)o * M + m) * N + k] = make_float2(accr[v], acci[v]);
+    }
+}
+
 // [R][M][N] -> [N][R][M] (spectra of R signals, M blocks, N bins)
 __global__ void k_spec_transpose(const cf *__restrict__ src, cf *__restrict__ dst, int R, int M, int N)
 {
