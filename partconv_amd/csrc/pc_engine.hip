@@ -3284,13 +3284,16 @@ PC_API int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *st
         const long long n = (long long)nx;
         hipLaunchKernelGGL(k_spec_transpose, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dsum, x->tmp,
                            x->n_in, M, L.N);
-        // up to 4 m-tiles per workgroup share the staged filter blocks
+        // up to 4 m-tiles per workgroup share the staged filter blocks and
+        // input window (while that fits 64 KB of LDS)
         const int tiles = (M + 31) / 32;
-        if (tiles >= 4)
-            hipLaunchKernelGGL(k_mimo_mfma_lds<4>, dim3(L.N, (tiles + 3) / 4), dim3(256), 0, s, x->Ht[li], x->tmp,
+        const int mt = tiles >= 4 ? 4 : 2;
+        const size_t lds = (2 * 32 * 32 + (size_t)x->n_in * (32 * mt + L.P - 1)) * sizeof(cf);
+        if (tiles >= 4 && lds <= 65536)
+            hipLaunchKernelGGL(k_mimo_mfma_lds<4>, dim3(L.N, (tiles + 3) / 4), dim3(256), lds, s, x->Ht[li], x->tmp,
                                c->work, L.N, M, L.P, x->n_in, x->n_out);
-        else if (tiles >= 2)
-            hipLaunchKernelGGL(k_mimo_mfma_lds<2>, dim3(L.N, (tiles + 1) / 2), dim3(128), 0, s, x->Ht[li], x->tmp,
+        else if (tiles >= 2 && lds <= 65536)
+            hipLaunchKernelGGL(k_mimo_mfma_lds<2>, dim3(L.N, (tiles + 1) / 2), dim3(128), lds, s, x->Ht[li], x->tmp,
                                c->work, L.N, M, L.P, x->n_in, x->n_out);
         else
             hipLaunchKernelGGL(k_mimo_mfma, dim3(L.N, tiles), dim3(64), 0, s, x->Ht[li], x->tmp, c->work, L.N, M,

@@ -600,9 +600,13 @@ k_mimo_mfma(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__restrict
 // The same product with MT waves per workgroup, one m-tile each: the
 // partition-p filter block H_k,p (n_in x 32 complex) is staged in LDS once for
 // the workgroup (double-buffered, the next partition's loads in flight during
-// this one's MFMAs) instead of being read from memory by every m-tile.  Each
-// wave runs exactly k_mimo_mfma's sequence of MFMAs, so the results are the
-// same bit for bit.  Grid: (N, ceil(ceil(M / 32) / MT)), block 64 MT.
+// this one's MFMAs) instead of being read from memory by every m-tile, and the
+// workgroup's input window (blocks m0 - P + 1 .. m0 + 32 MT - 1 of every input,
+// zero outside 0 .. M-1) is staged once, since partition p reads the blocks of
+// partition p-1 shifted by one.  Each wave runs exactly k_mimo_mfma's sequence
+// of MFMAs, so the results are the same bit for bit.  Grid: (N, ceil(ceil(M /
+// 32) / MT)), block 64 MT, dynamic LDS (2 * 32 * 32 + n_in * (32 MT + P - 1))
+// complex values.
 template <int MT>
 __global__ void __launch_bounds__(64 * MT)
 k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__restrict__ Y, int N, int M, int P,
@@ -610,7 +614,9 @@ k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__rest
 {
     constexpr int NT = 64 * MT, HS = 32 * 32;  // n_in <= 32
     constexpr int PER = (HS + NT - 1) / NT;
-    __shared__ cf hs[2][HS];
+    extern __shared__ cf mm_lds[];
+    cf *const hs0 = mm_lds, *const hs1 = mm_lds + HS;  // filter blocks, by partition parity
+    cf *const xs = mm_lds + 2 * HS;                      // input window [n_in][WL]
     const int k = blockIdx.x;
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
     const int m0 = (blockIdx.y * MT + wv) * 32;
@@ -626,7 +632,12 @@ k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__rest
     const int m = m0 + lo;
     const cf *Xk = Xt + (size_t)k * n_in * M;
     const cf *Hk = Ht + (size_t)k * P * n_in * 32;
-    for (int e = tid; e < nh; e += NT) hs[0][e] = Hk[e];
+    const int mlo = blockIdx.y * MT * 32 - (P - 1), WL = 32 * MT + P - 1;
+    for (int e = tid; e < nh; e += NT) hs0[e] = Hk[e];
+    for (int e = tid; e < n_in * WL; e += NT) {
+        const int i = e / WL, r = mlo + (e - i * WL);
+        xs[e] = (r >= 0 && r < M) ? Xk[(size_t)i * M + r] : make_float2(0.f, 0.f);
+    }
     __syncthreads();
     for (int p = 0; p < P; p++) {
         cf pre[PER];
@@ -639,12 +650,11 @@ k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__rest
             }
         }
         if (wave_on) {
-            const cf *hp = hs[p & 1];
-            const int mm = m - p;
-            const bool mv = (mm >= 0 && mm < M);
+            const cf *hp = (p & 1) ? hs1 : hs0;
+            const cf *xp = xs + (m - p - mlo);  // block m - p, zero outside the clip
             for (int j = 0; j < n_in; j++) {
                 const cf h = hp[j * 32 + lo];
-                const cf x = mv ? Xk[(size_t)j * M + mm] : make_float2(0.f, 0.f);
+                const cf x = xp[j * WL];
                 float ar, ai;
                 if (!bin0) {
                     ar = hi ? -h.y : h.x;
@@ -662,7 +672,7 @@ k_mimo_mfma_lds(const cf *__restrict__ Ht, const cf *__restrict__ Xt, cf *__rest
 #pragma unroll
             for (int q = 0; q < PER; q++) {
                 const int e = tid + q * NT;
-                if (e < nh) hs[(p + 1) & 1][e] = pre[q];
+                if (e < nh) ((p & 1) ? hs0 : hs1)[e] = pre[q];
             }
         }
         __syncthreads();
