@@ -2945,8 +2945,7 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
     if (!c || !d_x || !d_y || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
     if (!c->filter_ready) return fail(PC_ERR_STATE, "forward before set_filter");
     hipStream_t s = (hipStream_t)stream;
-    int rc = conv_fill_zero(d_y, (long long)C * T, s);
-    if (rc) return rc;
+    int rc = PC_OK;
     size_t need_keep = 0;
     for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
     const bool keep = (long long)need_keep <= c->budget;
@@ -2989,7 +2988,9 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
         D.ulo = 0;
         D.uhi = L.N;
         D.scale = 1.f;
-        D.accumulate = 1;
+        // level 0's blocks tile the clip: they write every sample once, the
+        // later levels add to it (no clearing pass)
+        D.accumulate = li > 0;
         D.parity = -1;
         if ((rc = off_irfft(c, L, c->work, D, C, s))) return rc;
     }
@@ -3270,8 +3271,7 @@ PC_API int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *st
     if (!x || !d_x || !d_y || T < 1) return fail(PC_ERR_ARG, "bad argument");
     hipStream_t s = (hipStream_t)stream;
     pc_conv *c = x->c;
-    int rc = conv_fill_zero(d_y, (long long)x->n_out * T, s);
-    if (rc) return rc;
+    int rc = PC_OK;
     for (size_t li = 0; li < c->lv.size(); li++) {
         ConvLevel &L = c->lv[li];
         WinSrc W = conv_xwin(L, d_x, T);
@@ -3309,7 +3309,7 @@ PC_API int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *st
         D.ulo = 0;
         D.uhi = L.N;
         D.scale = 1.f;
-        D.accumulate = 1;
+        D.accumulate = li > 0;  // level 0's blocks tile the signal: no clearing pass
         D.parity = -1;
         if ((rc = off_irfft(c, L, c->work, D, x->n_out, s))) return rc;
     }
