@@ -539,6 +539,11 @@ def bench_diff(args):
     roof = kernel_roofline(p_avg_us, p_launches, p_bytes,
                            "k_off_mac (256 level, forward: Y_m = sum_p X_{m-p} H_p for 64 clips x 512 blocks, "
                            "14 partitions)")
+    tpath = os.path.join(ROOT, "profiles", "k_off_mac_cfg5_traffic.json")
+    if roof and os.path.exists(tpath):
+        with open(tpath) as fh:
+            t = json.load(fh)
+        roof["traffic"] = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
     # whole step in the canonical per-block accounting (the partition spectra
     # are shared by the 64 clips and served from cache, so this can exceed
     # the HBM peak): forward, dL/dx and dL/dh each stream them once per block
@@ -626,6 +631,14 @@ def main():
               "(no collective on this path); run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy:
+        # a 64 MiB device copy first: the counter runs scale FETCH_SIZE / WRITE_SIZE by it
+        import torch
+        src = torch.ones(16 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "cfg3":
         bench_uneven(args)
         return
@@ -637,12 +650,6 @@ def main():
         return
     import torch
     from partconv_amd.workloads import cfg2_irs
-    if args.calib_copy:
-        src = torch.ones(16 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     irs = cfg2_irs(seed=2, channels=CHANNELS, taps=TAPS)
     total_steps = args.warmup + max(args.steps, args.plain_steps)
     n = total_steps * TICKS_PER_STEP * N0

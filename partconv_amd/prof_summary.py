@@ -7,9 +7,9 @@
 --split NAME K splits the launches of one shape of the kernels named NAME by
 call index mod K (e.g. the levels of a pass that launches one kernel per level).
 `counter` prints the mean counter value (KiB per dispatch for FETCH_SIZE and
-WRITE_SIZE) per (kernel, grid, counter); with --calib-grid the copy kernel of
-that grid size (a 64 MiB device copy issued by bench.py --calib-copy) gives the
-scale factor 65536 KiB / measured for each counter.
+WRITE_SIZE) per (kernel, grid, counter); with --calib-grid the first copy kernel
+of that grid size (the 64 MiB device copy bench.py --calib-copy issues first)
+gives the scale factor 65536 KiB / measured for each counter.
 """
 from __future__ import annotations
 
@@ -49,9 +49,16 @@ def counter(path, calib_grid=None):
             float(r["Counter_Value"]))
     scale = {}
     if calib_grid is not None:
-        for (n, g, c), v in d.items():
-            if "copyBuffer" in n and g == str(calib_grid):
-                scale[c] = 65536.0 / (sum(v) / len(v))
+        # the calibration copy is the first copy of that grid in the process
+        # (later copies of the same grid may move other sizes)
+        first = {}
+        for r in rows:
+            if "copyBuffer" in r["Kernel_Name"] and r.get("Grid_Size", r.get("Grid_Size_X", "")) == str(calib_grid):
+                c, i = r["Counter_Name"], int(r.get("Dispatch_Id", 0))
+                if c not in first or i < first[c][0]:
+                    first[c] = (i, float(r["Counter_Value"]))
+        for c, (_, v) in first.items():
+            scale[c] = 65536.0 / v
         for c, s in scale.items():
             print(f"calibration {c}: 64 MiB copy reads {65536.0 / s:.1f} KiB -> scale {s:.4f}")
     print(f"{'kernel':60s} {'grid':>9s} {'counter':>11s} {'calls':>6s} {'KiB/disp':>11s} {'scaled':>11s}")
