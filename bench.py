@@ -112,6 +112,17 @@ def kernel_roofline(avg_us, launches, bytes_per, kernel):
             "alg_bytes_per_launch": bytes_per}
 
 
+def with_traffic(roof, name):
+    """Attach the HBM bytes per launch measured by FETCH_SIZE / WRITE_SIZE
+    counters (copy-calibrated; profiles/<name>) to a kernel roofline."""
+    path = os.path.join(ROOT, "profiles", name)
+    if roof and os.path.exists(path):
+        with open(path) as fh:
+            t = json.load(fh)
+        roof["traffic"] = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
+    return roof
+
+
 def oracle_threads(units, budget_s, calib_ticks):
     """Time OracleMulti units (each (engine, input columns)) on threads over the
     host's cores, every unit processing the same number of ticks; the tick count
@@ -266,9 +277,11 @@ def bench_uneven(args):
                        "persistent_ticks": int(st.persistent_ticks)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err,
-        "roofline": kernel_roofline(avg_us, launches, bytes_per,
-                                    f"k_partial (largest level, {int(parts and max(max(N) for N, _ in parts))}, "
-                                    "delay-line sum of every instance that has it)"),
+        "roofline": with_traffic(kernel_roofline(avg_us, launches, bytes_per,
+                                                 f"k_partial (largest level, "
+                                                 f"{int(parts and max(max(N) for N, _ in parts))}, "
+                                                 "delay-line sum of every instance that has it)"),
+                                 "k_partial_cfg3_512_traffic.json"),
         "hbm_roofline": hbm_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0),
         "cpu_baseline": cpu}), flush=True)
 
@@ -539,11 +552,7 @@ def bench_diff(args):
     roof = kernel_roofline(p_avg_us, p_launches, p_bytes,
                            "k_off_mac (256 level, forward: Y_m = sum_p X_{m-p} H_p for 64 clips x 512 blocks, "
                            "14 partitions)")
-    tpath = os.path.join(ROOT, "profiles", "k_off_mac_cfg5_traffic.json")
-    if roof and os.path.exists(tpath):
-        with open(tpath) as fh:
-            t = json.load(fh)
-        roof["traffic"] = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
+    roof = with_traffic(roof, "k_off_mac_cfg5_traffic.json")
     # whole step in the canonical per-block accounting (the partition spectra
     # are shared by the 64 clips and served from cache, so this can exceed
     # the HBM peak): forward, dL/dx and dL/dh each stream them once per block
