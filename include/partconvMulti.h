@@ -1,8 +1,8 @@
 /*
- * partconvMulti.h -- C++ drop-in for the reference's PartConvMulti,
- * PartConvMultiRelaxed (src/partconvMulti.h:52-148), PartConvFilter
- * (src/partconv.h:40-114) and PartConvMax (src/partconv.h:117-175),
- * header-only over the C ABI of partconv_amd.h.
+ * partconvMulti.h -- C++ drop-in for the reference's PartConvMulti and
+ * PartConvMultiRelaxed (src/partconvMulti.h:52-148), header-only over the C
+ * ABI of partconv_amd.h.  Like the reference's header it includes partconv.h
+ * (PartConvFilter, PartConvMax, and Vector through buffers.h).
  *
  * Same class names, method names, argument meaning and public fields.  Where
  * the reference calls exit(-1) (bad config, unreadable IR, frameCount !=
@@ -22,7 +22,10 @@
 
 #include <stdio.h>
 
+#include "partconv.h"
 #include "partconv_amd.h"
+
+class Waiter; /* the reference's completion waiter type (src/partconvMulti.h:10) */
 
 class PartConvMulti
 {
@@ -39,11 +42,12 @@ public:
         return rc;
     }
     int cleanup() { return pcm_cleanup(h_); }
+    /* one tick; frameNum counts the ticks that completed */
     virtual int run(float *const output, const float *const input, const int frameCount)
     {
         int rc = pcm_run(h_, output, input, frameCount);
         if (rc) fprintf(stderr, "%s\n", pc_last_error());
-        frameNum++;
+        else frameNum++;
         return rc;
     }
     void reset()
@@ -64,7 +68,7 @@ public:
     int numInputChannels;
     int numOutputChannels;
     unsigned long frameNum;
-    void *doneWaiter = nullptr;
+    Waiter *doneWaiter = nullptr; /* no waiter: run() returns when the tick's output is ready */
     unsigned long lastFrame = 0;
 
 protected:
@@ -95,126 +99,6 @@ class PartConvMultiRelaxed : public PartConvMulti
 {
 public:
     PartConvMultiRelaxed(int inVerbosity = 0) : PartConvMulti(inVerbosity, 1) {}
-};
-
-class PartConvFilter
-{
-public:
-    PartConvFilter(int inVerbosity = 0, int inSampleRate = 44100)
-        : M(0), N(nullptr), num_parts(nullptr), FS(inSampleRate), verbosity(inVerbosity),
-          h_(pcf_create(inVerbosity, inSampleRate))
-    {
-    }
-    ~PartConvFilter()
-    {
-        pcf_destroy(h_);
-        delete[] N;
-        delete[] num_parts;
-    }
-    /* impulse: IR samples (impulse_len of them), or nullptr for an all-ones IR */
-    int setup(int M_in, int *N_in, int *num_parts_in, const float *impulse = nullptr, int impulse_len = 0,
-              int fftwOptLevel = 1, const char *wisdom = nullptr)
-    {
-        pcf_set_fs(h_, FS);
-        int rc = pcf_setup(h_, M_in, N_in, num_parts_in, impulse, impulse_len, fftwOptLevel, wisdom);
-        if (rc) {
-            fprintf(stderr, "%s\n", pc_last_error());
-            return 1;
-        }
-        delete[] N;
-        delete[] num_parts;
-        M = M_in;
-        N = new int[M];
-        num_parts = new int[M];
-        for (int i = 0; i < M; i++) {
-            N[i] = N_in[i];
-            num_parts[i] = num_parts_in[i];
-        }
-        return 0;
-    }
-    double bench(const int L, const int trials = 100, const int pollute_every = 0, const float max_time = 0.1,
-                 double *benchData = nullptr)
-    {
-        return pcf_bench(h_, L, trials, pollute_every, max_time, benchData);
-    }
-    void reset() { pcf_reset(h_); }
-    void sync_levels() { pcf_sync_levels(h_); }
-    void sync_levels_relaxed() { pcf_sync_levels_relaxed(h_); }
-    void input(const float *const input) { pcf_input(h_, input); }
-    void output(float *const output) { pcf_output(h_, output); }
-    void output_relaxed(float *const output) { pcf_output_relaxed(h_, output); }
-    int runLevel(int L) { return pcf_run_level(h_, L); }
-    int cleanup() { return pcf_cleanup(h_); }
-
-    int M;
-    int *N;
-    int *num_parts;
-    int FS;
-    int verbosity;
-
-private:
-    pc_filter *h_;
-};
-
-/* PartConvMax (src/partconv.h:117-175): the in-memory front end.  Channel
- * modes from (n, m, k), IRs from one array, one greedy partition over
- * blockSizes, per-channel buffers (float** run). */
-class PartConvMax
-{
-public:
-    PartConvMax(int inVerbosity = 0) : verbosity(inVerbosity), h_(pcm_create(inVerbosity, 0)) {}
-    ~PartConvMax() { pcm_destroy(h_); }
-
-    int setup(int inSampleRate, int n, int m, int k, float *impulses, int filterLength, int stride, int *blockSizes,
-              int levels, int fftwOptLevel = 1, const char *wisdom = nullptr, int max_threads_per_level = 1,
-              int max_level0_threads = 0)
-    {
-        (void)fftwOptLevel;
-        (void)wisdom;
-        (void)max_threads_per_level;
-        (void)max_level0_threads;
-        int rc = pcm_setup_max(h_, inSampleRate, n, m, k, impulses, filterLength, stride, blockSizes, levels);
-        if (rc) {
-            fprintf(stderr, "%s\n", pc_last_error());
-            return 1;
-        }
-        pc_multi_info i;
-        if (pcm_get_info(h_, &i) == PC_OK) {
-            FS = i.FS;
-            buffer_size = i.buffer_size;
-            max_block_size = i.max_block_size;
-            numInputChannels = i.numInputChannels;
-            numOutputChannels = i.numOutputChannels;
-        }
-        return 0;
-    }
-    int cleanup() { return pcm_cleanup(h_); }
-    int run(float **const output, float **const input)
-    {
-        int rc = pcm_run_channels(h_, output, input);
-        if (rc) fprintf(stderr, "%s\n", pc_last_error());
-        frameNum++;
-        return rc;
-    }
-    void reset()
-    {
-        pcm_reset(h_);
-        frameNum = 0;
-    }
-    pc_multi *handle() { return h_; }
-
-    float **outbuffers = nullptr; /* kept for source compatibility */
-    float **inbuffers = nullptr;
-    int FS = 0;
-    int buffer_size = 0;
-    int max_block_size = 0;
-    int verbosity;
-    int numInputChannels = 0;
-    int numOutputChannels = 0;
-    unsigned long frameNum = 0;
-
-private:
-    pc_multi *h_;
 };
 
 /* Real-time MIMO, shared input transforms (no reference counterpart: the

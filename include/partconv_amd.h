@@ -86,8 +86,33 @@ typedef struct pc_filter pc_filter;
 /* Last error message of the calling thread ("" if none). */
 const char *pc_last_error(void);
 
+/* End the library's use in this process (no reference counterpart: the
+ * reference frees per object in cleanup(), src/partconvMulti.cpp:720-804).
+ * Destroys every handle still live -- stopping resident persistent tick
+ * kernels and joining the engine's launcher threads -- returns the library's
+ * host-writable device memory and twiddle tables, and pairs its hsa_init with
+ * hsa_shut_down.  Idempotent.  Runs by itself at process exit (an atexit
+ * handler registered after the library's first HIP call, so before the HIP
+ * runtime's own teardown; the Python loader also calls it from its atexit
+ * hook).  Afterwards *_destroy of a handle it destroyed is a no-op and every
+ * *_create fails with PC_ERR_STATE. */
+void pc_shutdown(void);
+
 /* Library / device information: writes the HIP device name into buf. */
 int pc_device_name(char *buf, int len);
+
+/* readImpulseResponse(Vector&, filename, verbosity) (src/sndtools.cpp:3-47):
+ * the first channel of a RIFF/WAVE file as float.  Integer PCM of 8 (unsigned),
+ * 16, 24 or 32 bits is scaled by 2^(bits-1); IEEE float of 32 or 64 bits is
+ * taken as is; WAVE_FORMAT_EXTENSIBLE resolves to its subformat.  Writes up to
+ * cap samples into out (out may be null with cap 0 to query the length);
+ * *frames gets the file's frame count and *sample_rate its rate (either may be
+ * null).  verbosity > 0 prints the file's description as the reference does.
+ * PC_ERR_IO for a missing, unsupported or malformed file (a header whose frame
+ * size cannot hold one sample per channel is rejected before any sample is
+ * read). */
+int pc_read_ir(const char *filename, float *out, long long cap, long long *frames, int *sample_rate,
+               int verbosity);
 
 /* Greedy partition of PartConvMax::setup (src/partconv.cpp:712-728): over the
  * candidate block sizes block_sizes[0..levels), P_i = 2*N_{i+1}/N_i - 2, stopping
@@ -249,10 +274,12 @@ pc_filter *pcf_create(int verbosity, int sample_rate);
 
 /* PartConvFilter::setup(M, N, num_parts, impulse, fftwOptLevel, wisdom)
  * (src/partconv.cpp:70-306).  impulse == NULL gives an all-ones IR of sum N*P
- * taps (src/partconv.cpp:132-142).  num_parts is updated in place with the
- * recounted last level, as the reference's member array is.  fftwOptLevel and
- * wisdom are accepted and ignored (no planner). */
-int pcf_setup(pc_filter *h, int M, const int *N, int *num_parts, const float *impulse,
+ * taps (src/partconv.cpp:132-142).  N and num_parts are read, never written
+ * (the reference copies them into its members, src/partconv.cpp:83-84, and
+ * recounts the last level there, :169); pcf_get_levels returns the recounted
+ * partition.  Partition errors are reported before any GPU call.  fftwOptLevel
+ * and wisdom are accepted and ignored (no planner). */
+int pcf_setup(pc_filter *h, int M, const int *N, const int *num_parts, const float *impulse,
               int impulse_len, int fftwOptLevel, const char *wisdom);
 
 /* PartConvFilter::runLevel(L) (src/partconv.cpp:354-490): enqueues level L's

@@ -5,8 +5,10 @@ a CPU implementation: if the library is missing, loading raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import pathlib
+import sys
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libpartconv_amd.so"
@@ -46,6 +48,9 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "pc_last_error": (ctypes.c_char_p, []),
+    "pc_shutdown": (None, []),
+    "pc_read_ir": (ctypes.c_int, [ctypes.c_char_p, _fp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     "pc_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "pc_greedy_partition": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]),
     "pcm_create": (_vp, [ctypes.c_int, ctypes.c_int]),
@@ -138,7 +143,37 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        # registered after torch (imported first by every caller that drives a
+        # GPU), so it runs before torch's own exit hooks
+        atexit.register(shutdown)
     return _lib
+
+
+_shut = False
+
+
+def shutdown():
+    """Destroy every live handle (resident kernels stopped, launcher threads
+    joined) and release the library's device resources: pc_shutdown().  Runs
+    at interpreter exit; afterwards destroy() is a no-op."""
+    global _shut
+    if _shut:
+        return
+    _shut = True
+    if _lib is not None:
+        _lib.pc_shutdown()
+
+
+def destroy(fn_name, handle):
+    """Destroy a handle from a __del__: a no-op once the library has shut down
+    or while the interpreter finalises (the library's exit hook owns teardown
+    then), so no destructor ever runs against a torn-down runtime."""
+    try:
+        if _shut or not handle or _lib is None or sys.is_finalizing():
+            return
+        getattr(_lib, fn_name)(handle)
+    except Exception:
+        pass
 
 
 def last_error() -> str:

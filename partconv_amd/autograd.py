@@ -21,6 +21,7 @@ import ctypes
 
 import torch
 
+from . import _lib
 from ._lib import last_error, lib
 
 
@@ -48,7 +49,10 @@ class PartitionedConv:
         Po = (ctypes.c_int * 16)()
         M = lib().pcc_levels(self._h, No, Po, 16)
         self.N, self.P = list(No)[:M], list(Po)[:M]
-        self._filter_key = None
+        # generation of the filter spectra held by the handle: bumped by every
+        # set_filter, so a backward can tell whether the spectra are still the
+        # ones its own forward computed
+        self._filter_gen = 0
 
     def set_budget(self, nbytes):
         lib().pcc_set_budget(self._h, int(nbytes))
@@ -87,20 +91,28 @@ class PartitionedConv:
         return a.value, n.value, b.value
 
     def set_filter(self, h):
-        # under graph capture the filter spectra are always recomputed, so a
-        # replay sees the filter's values at replay time
-        capturing = torch.cuda.is_current_stream_capturing()
-        key = (h.data_ptr(), h._version)
-        if capturing or key != self._filter_key:
-            self._check(lib().pcc_set_filter(self._h, _p(h), _stream()))
-            self._filter_key = None if capturing else key
+        """Filter spectra from h (always recomputed: a tensor's address and
+        version do not identify its contents, since the caching allocator hands
+        a freed block to the next tensor of the same size).  Returns the new
+        spectra generation."""
+        self._check(lib().pcc_set_filter(self._h, _p(h), _stream()))
+        self._filter_gen += 1
+        return self._filter_gen
 
     def forward_raw(self, x, h, y):
-        self.set_filter(h)
+        """Forward with fresh filter spectra; returns their generation."""
+        gen = self.set_filter(h)
         self._check(lib().pcc_forward(self._h, _p(x), _p(y), x.shape[0], x.shape[1], _stream()))
+        return gen
 
-    def backward_raw(self, x, h, g, dx, dh):
-        self.set_filter(h)
+    def backward_raw(self, x, h, g, dx, dh, gen=None):
+        """Adjoint.  The spectra are recomputed unless they are still generation
+        `gen` (the forward that saved h computed them and no set_filter ran
+        since; autograd rejects a saved h that was modified in place).  Under
+        graph capture they are always recomputed, so a replay sees the filter's
+        values at replay time."""
+        if gen is None or gen != self._filter_gen or torch.cuda.is_current_stream_capturing():
+            self.set_filter(h)
         self._check(lib().pcc_backward(self._h, _p(x), _p(g), None if dx is None else _p(dx),
                                        None if dh is None else _p(dh), x.shape[0], x.shape[1], _stream()))
 
@@ -108,12 +120,8 @@ class PartitionedConv:
         return PartConvFunction.apply(x, h, self)
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcc_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcc_destroy", h)
 
 
 def _prep(t, shape_msg):
@@ -130,7 +138,7 @@ class PartConvFunction(torch.autograd.Function):
         if x.dim() != 2 or h.dim() != 1 or h.numel() != conv.taps:
             raise ValueError("x must be [C, T] and h [taps]")
         y = torch.empty_like(x)
-        conv.forward_raw(x, h, y)
+        ctx.filter_gen = conv.forward_raw(x, h, y)
         ctx.save_for_backward(x, h)
         ctx.conv = conv
         return y
@@ -143,5 +151,5 @@ class PartConvFunction(torch.autograd.Function):
         dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
         dh = torch.empty_like(h) if ctx.needs_input_grad[1] else None
         if dx is not None or dh is not None:
-            conv.backward_raw(x, h, g, dx, dh)
+            conv.backward_raw(x, h, g, dx, dh, ctx.filter_gen)
         return dx, dh, None

@@ -136,6 +136,7 @@ struct BarDevice {
 };
 static std::mutex g_bar_mu;
 static BarDevice g_bar[64];
+static int g_hsa_inits = 0;  // successful hsa_init calls of bar_init (g_bar_mu held)
 
 struct HsaFind {
     uint32_t bdf = 0, domain = 0;
@@ -194,6 +195,7 @@ static bool bar_init(int dev)
         hipDeviceGetAttribute(&domain, hipDeviceAttributePciDomainID, dev) != hipSuccess)
         return false;
     if (hsa_init() != HSA_STATUS_SUCCESS) return false;
+    g_hsa_inits++;  // paired with hsa_shut_down in pc_shutdown
     HsaFind f;
     f.bdf = ((uint32_t)bus << 8) | ((uint32_t)devno << 3);
     f.domain = (uint32_t)domain;
@@ -277,6 +279,62 @@ static void release_slot_put(int dev, int slot)
     if (slot < 0 || dev < 0 || dev >= 64) return;
     std::lock_guard<std::mutex> lk(g_bar_mu);
     g_bar[dev].free_slots.push_back(slot);
+}
+
+// ----------------------------------------------------------------------------
+// Live handles and library shutdown.
+//
+// Every handle a *_create returns is recorded until its *_destroy takes it.
+// pc_shutdown() destroys the handles still live -- stopping resident
+// persistent kernels and joining the launcher threads -- then returns the BAR
+// blocks and the twiddle tables and pairs bar_init's hsa_init with
+// hsa_shut_down.  It runs from an atexit handler registered right after the
+// library's first HIP call (so it runs before the HIP runtime's own exit-time
+// teardown, which was registered earlier) and from the Python loader's atexit
+// hook.  After it, *_destroy of a handle it destroyed is a no-op and *_create
+// fails: a destructor that runs late at exit never touches a torn-down runtime.
+// ----------------------------------------------------------------------------
+enum { HK_MULTI = 1, HK_FILTER, HK_CONV, HK_MIMO, HK_MIMO_RT };
+static std::mutex g_live_mu;
+static std::vector<std::pair<void *, int>> g_live;
+static bool g_shut = false;
+
+static bool live_add(void *p, int kind)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    if (g_shut) return false;
+    g_live.push_back({p, kind});
+    return true;
+}
+
+// True if p was live (it is no longer); false for a handle already destroyed
+// (by pc_shutdown or an earlier *_destroy) or never returned by *_create.
+static bool live_take(void *p)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    for (size_t i = 0; i < g_live.size(); i++)
+        if (g_live[i].first == p) {
+            g_live.erase(g_live.begin() + i);
+            return true;
+        }
+    return false;
+}
+
+static bool lib_shut()
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    return g_shut;
+}
+
+PC_API void pc_shutdown(void);
+static void shutdown_at_exit() { pc_shutdown(); }
+
+// Called after a HIP call has succeeded: the HIP runtime is initialised, so an
+// atexit handler registered now runs before its teardown.
+static void hip_live()
+{
+    static std::once_flag once;
+    std::call_once(once, [] { atexit(shutdown_at_exit); });
 }
 
 // ----------------------------------------------------------------------------
@@ -702,7 +760,6 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     if (!is_pow2(N0) || N0 < PC_MIN_BLOCK || N0 > 2048)
         return fail(PC_ERR_ARG, "buffer_size must be a power of two in [16, 2048]");
     if (nIn < 1 || nOut < 1) return fail(PC_ERR_ARG, "need at least one input and output channel");
-    HIPCHK(hipGetDevice(&E.dev));
     E.FS = FS;
     E.N0 = N0;
     E.nIn = nIn;
@@ -744,6 +801,9 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
         I.R = (int)(1ll << ilog2(need + N0));
         E.inst.push_back(I);
     }
+    // every partition checked (errors as the reference's, without a device): now the GPU
+    HIPCHK(hipGetDevice(&E.dev));
+    hip_live();
     // global level set (src/partconvMulti.cpp:139-244)
     const int temp_levels = ilog2(max_block / N0) + 1;
     E.gsize.clear();
@@ -1845,6 +1905,23 @@ PC_API int pc_device_name(char *buf, int len)
     return PC_OK;
 }
 
+PC_API int pc_read_ir(const char *filename, float *out, long long cap, long long *frames, int *sample_rate,
+                      int verbosity)
+{
+    if (!filename || cap < 0 || (cap > 0 && !out)) return fail(PC_ERR_ARG, "bad argument");
+    std::vector<float> v;
+    std::string err;
+    pcio::WavInfo wi;
+    if (!pcio::read_wav(filename, v, &wi, err)) return fail(PC_ERR_IO, err);
+    if (frames) *frames = (long long)v.size();
+    if (sample_rate) *sample_rate = wi.sample_rate;
+    if (out) memcpy(out, v.data(), (size_t)std::min<long long>(cap, (long long)v.size()) * sizeof(float));
+    if (verbosity > 0)  // as src/sndtools.cpp:36-44
+        printf("\nImpulse response file: %s\nchannels: %d (only used one)\nframes: %zu\nformat: 0x%x (%d-bit)\n"
+               "sample rate: %d\n\n", filename, wi.channels, v.size(), wi.format_tag, wi.bits, wi.sample_rate);
+    return PC_OK;
+}
+
 PC_API int pc_greedy_partition(const int *block_sizes, int levels, int filter_length, int *N, int *P,
                                int *M_out)
 {
@@ -1882,6 +1959,11 @@ PC_API pc_multi *pcm_create(int verbosity, int relaxed)
     pc_multi *h = new pc_multi();
     h->E.verbosity = verbosity;
     h->E.relaxed = relaxed ? 1 : 0;
+    if (!live_add(h, HK_MULTI)) {
+        delete h;
+        fail(PC_ERR_STATE, "library shut down");
+        return nullptr;
+    }
     return h;
 }
 
@@ -1928,8 +2010,8 @@ PC_API int pcm_setup(pc_multi *h, const char *config_file, int max_threads_per_l
         x.scale = L.scale;
         x.N = L.N;
         x.P = L.P;
-        int sr = 0;
-        if (!pcio::read_wav(L.ir_file, x.ir, &sr, err)) return fail(PC_ERR_IO, err);
+        pcio::WavInfo wi;
+        if (!pcio::read_wav(L.ir_file, x.ir, &wi, err)) return fail(PC_ERR_IO, err);
         d.push_back(std::move(x));
     }
     return multi_setup(h, cfg.FS, cfg.buffer_size, cfg.nIn, cfg.nOut, d);
@@ -2140,11 +2222,16 @@ PC_API int pcm_cleanup(pc_multi *h)
     return PC_OK;
 }
 
-PC_API void pcm_destroy(pc_multi *h)
+static void multi_destroy(pc_multi *h)
 {
-    if (!h) return;
     engine_free(h->E);
     delete h;
+}
+
+PC_API void pcm_destroy(pc_multi *h)
+{
+    if (!h || !live_take(h)) return;
+    multi_destroy(h);
 }
 
 PC_API int pcm_get_info(const pc_multi *h, pc_multi_info *info)
@@ -2380,10 +2467,15 @@ PC_API pc_filter *pcf_create(int verbosity, int sample_rate)
     pc_filter *h = new pc_filter();
     h->E.verbosity = verbosity;
     h->FS = sample_rate;
+    if (!live_add(h, HK_FILTER)) {
+        delete h;
+        fail(PC_ERR_STATE, "library shut down");
+        return nullptr;
+    }
     return h;
 }
 
-PC_API int pcf_setup(pc_filter *h, int M, const int *N, int *num_parts, const float *impulse,
+PC_API int pcf_setup(pc_filter *h, int M, const int *N, const int *num_parts, const float *impulse,
                      int impulse_len, int fftwOptLevel, const char *wisdom)
 {
     (void)fftwOptLevel;
@@ -2413,10 +2505,7 @@ PC_API int pcf_setup(pc_filter *h, int M, const int *N, int *num_parts, const fl
     h->M = M;
     h->N.assign(N, N + M);
     h->P.resize(M);
-    for (int L = 0; L < M; L++) {
-        h->P[L] = I.lv[L].P;
-        num_parts[L] = I.lv[L].P;
-    }
+    for (int L = 0; L < M; L++) h->P[L] = I.lv[L].P;  // recounted; the caller's array is not written
     h->num_N0.resize(M);
     h->ind_N0.resize(M);
     h->rb.resize(M);
@@ -2607,11 +2696,16 @@ PC_API int pcf_cleanup(pc_filter *h)
     return PC_OK;
 }
 
-PC_API void pcf_destroy(pc_filter *h)
+static void filter_destroy(pc_filter *h)
 {
-    if (!h) return;
     engine_free(h->E);
     delete h;
+}
+
+PC_API void pcf_destroy(pc_filter *h)
+{
+    if (!h || !live_take(h)) return;
+    filter_destroy(h);
 }
 
 PC_API int pcf_get_levels(const pc_filter *h, int *M, int *N, int *num_parts, int cap)
@@ -2840,8 +2934,12 @@ static int conv_fill_zero(float *p, long long n, hipStream_t s)
     return PC_OK;
 }
 
-PC_API pc_conv *pcc_create(int M, const int *N, const int *P, int taps)
+static pc_conv *conv_create(int M, const int *N, const int *P, int taps)
 {
+    if (lib_shut()) {
+        fail(PC_ERR_STATE, "library shut down");
+        return nullptr;
+    }
     if (M < 1 || !N || !P || taps < 1) {
         fail(PC_ERR_ARG, "bad argument");
         return nullptr;
@@ -2860,6 +2958,7 @@ PC_API pc_conv *pcc_create(int M, const int *N, const int *P, int taps)
         fail(PC_ERR_HIP, msg);
         return nullptr;
     }
+    hip_live();
     c->taps = taps;
     for (int L = 0; L < M; L++) {
         ConvLevel cl;
@@ -2878,6 +2977,19 @@ PC_API pc_conv *pcc_create(int M, const int *N, const int *P, int taps)
     }
     c->keep.assign(M, nullptr);
     c->keep_cap.assign(M, 0);
+    return c;
+}
+
+static void conv_destroy(pc_conv *c);
+
+PC_API pc_conv *pcc_create(int M, const int *N, const int *P, int taps)
+{
+    pc_conv *c = conv_create(M, N, P, taps);
+    if (c && !live_add(c, HK_CONV)) {
+        conv_destroy(c);
+        fail(PC_ERR_STATE, "library shut down");
+        return nullptr;
+    }
     return c;
 }
 
@@ -3184,9 +3296,8 @@ PC_API int pcc_probe_read(pc_conv *c, double *avg_us, long long *launches, doubl
     return PC_OK;
 }
 
-PC_API void pcc_destroy(pc_conv *c)
+static void conv_destroy(pc_conv *c)
 {
-    if (!c) return;
     hipDeviceSynchronize();
     conv_probe_fold(c);
     for (ConvLevel &L : c->lv) hipFree(L.H);
@@ -3196,6 +3307,12 @@ PC_API void pcc_destroy(pc_conv *c)
     hipFree(c->dpart);
     hipFree(c->dsum);
     delete c;
+}
+
+PC_API void pcc_destroy(pc_conv *c)
+{
+    if (!c || !live_take(c)) return;
+    conv_destroy(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -3215,7 +3332,7 @@ PC_API pc_mimo *pcx_create(int M, const int *N, const int *P, int taps, int n_in
         fail(PC_ERR_ARG, "MIMO sizes must be 1..32");
         return nullptr;
     }
-    pc_conv *c = pcc_create(M, N, P, taps);
+    pc_conv *c = conv_create(M, N, P, taps);
     if (!c) return nullptr;
     pc_mimo *x = new pc_mimo();
     x->c = c;
@@ -3226,11 +3343,18 @@ PC_API pc_mimo *pcx_create(int M, const int *N, const int *P, int taps, int n_in
         if (hipMalloc(&h, (size_t)L.N * L.P * n_in * 32 * sizeof(cf)) != hipSuccess) {
             fail(PC_ERR_HIP, "hipMalloc MIMO spectra");
             for (cf *q : x->Ht) hipFree(q);
-            pcc_destroy(c);
+            conv_destroy(c);
             delete x;
             return nullptr;
         }
         x->Ht.push_back(h);
+    }
+    if (!live_add(x, HK_MIMO)) {
+        for (cf *q : x->Ht) hipFree(q);
+        conv_destroy(c);
+        delete x;
+        fail(PC_ERR_STATE, "library shut down");
+        return nullptr;
     }
     return x;
 }
@@ -3316,14 +3440,19 @@ PC_API int pcx_forward(pc_mimo *x, const float *d_x, float *d_y, int T, void *st
     return PC_OK;
 }
 
-PC_API void pcx_destroy(pc_mimo *x)
+static void mimo_destroy(pc_mimo *x)
 {
-    if (!x) return;
     hipDeviceSynchronize();
     for (cf *q : x->Ht) hipFree(q);
     hipFree(x->tmp);
-    pcc_destroy(x->c);
+    conv_destroy(x->c);
     delete x;
+}
+
+PC_API void pcx_destroy(pc_mimo *x)
+{
+    if (!x || !live_take(x)) return;
+    mimo_destroy(x);
 }
 
 // ---------------------------------------------------------------------------
@@ -3489,6 +3618,7 @@ static int mxr_alloc(pc_mimo_rt *r, void **p, size_t bytes)
 static int mxr_setup(pc_mimo_rt *r)
 {
     HIPCHK(hipGetDevice(&r->dev));
+    hip_live();
     // level blocks of 4096 transform in 64 KiB of LDS (two float2 buffers)
     const int lds12 = 2 * 4096 * (int)sizeof(cf);
     HIPCHK(hipFuncSetAttribute((const void *)k_mxr_spec<12>, hipFuncAttributeMaxDynamicSharedMemorySize, lds12));
@@ -3596,7 +3726,9 @@ PC_API pc_mimo_rt *pcxr_create(int FS, int buffer_size, int M, const int *N, con
     int rc = check_partition(r->N, r->P, taps, r->delay, r->T);
     for (int l = 0; !rc && l < M; l++)
         if (r->N[l] > MXR_MAX_N) rc = fail(PC_ERR_PARTITION, "real-time MIMO: block sizes up to 4096");
+    if (!rc && lib_shut()) rc = fail(PC_ERR_STATE, "library shut down");
     if (!rc) rc = mxr_setup(r);
+    if (!rc && !live_add(r, HK_MIMO_RT)) rc = fail(PC_ERR_STATE, "library shut down");
     if (rc) {
         mxr_free(r);
         delete r;
@@ -3931,9 +4063,60 @@ PC_API int pcxr_partition(const pc_mimo_rt *r, int *N, int *P, int cap)
     return r->M;
 }
 
-PC_API void pcxr_destroy(pc_mimo_rt *r)
+static void mimo_rt_destroy(pc_mimo_rt *r)
 {
-    if (!r) return;
     mxr_free(r);
     delete r;
+}
+
+PC_API void pcxr_destroy(pc_mimo_rt *r)
+{
+    if (!r || !live_take(r)) return;
+    mimo_rt_destroy(r);
+}
+
+// Library shutdown (see live_add): destroy every live handle, return the BAR
+// blocks and twiddle tables, and pair each hsa_init with hsa_shut_down.
+// Idempotent; afterwards handle creation fails.
+PC_API void pc_shutdown(void)
+{
+    std::vector<std::pair<void *, int>> v;
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        if (g_shut) return;
+        g_shut = true;
+        v.swap(g_live);
+    }
+    // newest first: a handle never outlives one created before it
+    for (size_t i = v.size(); i-- > 0;) {
+        void *p = v[i].first;
+        switch (v[i].second) {
+        case HK_MULTI: multi_destroy(static_cast<pc_multi *>(p)); break;
+        case HK_FILTER: filter_destroy(static_cast<pc_filter *>(p)); break;
+        case HK_CONV: conv_destroy(static_cast<pc_conv *>(p)); break;
+        case HK_MIMO: mimo_destroy(static_cast<pc_mimo *>(p)); break;
+        case HK_MIMO_RT: mimo_rt_destroy(static_cast<pc_mimo_rt *>(p)); break;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_bar_mu);
+    for (int d = 0; d < 64; d++) {
+        BarDevice &B = g_bar[d];
+        if (!B.ok) continue;
+        for (auto &blk : B.free_blocks) hsa_amd_memory_pool_free(blk.second);
+        B.free_blocks.clear();
+        if (B.rel_base) hsa_amd_memory_pool_free(B.rel_base);
+        B.rel_base = nullptr;
+        B.free_slots.clear();
+        B.ok = false;
+    }
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (int d = 0; d < 64; d++)
+        if (g_tw[d]) {
+            (void)hipSetDevice(d);
+            hipFree(g_tw[d]);
+            g_tw[d] = nullptr;
+        }
+    if (cur >= 0) (void)hipSetDevice(cur);
+    for (; g_hsa_inits > 0; g_hsa_inits--) hsa_shut_down();
 }

@@ -33,8 +33,10 @@ static uint16_t le16(const unsigned char *p) { return (uint16_t)(p[0] | (p[1] <<
 
 // PCM integer samples scale to [-1, 1) by 2^(bits-1), the convention libsndfile
 // applies when reading integer files as float; 8-bit WAVE data is unsigned.
-bool read_wav(const std::string &path, std::vector<float> &out, int *sample_rate,
-              std::string &err)
+// The header is validated before any sample is read: the sample width must be
+// a whole number of bytes the format supports, and one frame (block_align)
+// must hold every channel's sample, so each read stays inside the data chunk.
+bool read_wav(const std::string &path, std::vector<float> &out, WavInfo *info, std::string &err)
 {
     std::vector<unsigned char> b;
     if (!read_file(path, b)) {
@@ -45,7 +47,8 @@ bool read_wav(const std::string &path, std::vector<float> &out, int *sample_rate
         err = "not a RIFF/WAVE file: " + path;
         return false;
     }
-    int fmt_tag = 0, channels = 0, bits = 0, block_align = 0;
+    int fmt_tag = 0, channels = 0, bits = 0, block_align = 0, rate = 0;
+    bool have_fmt = false;
     const unsigned char *data = nullptr;
     size_t data_len = 0;
     size_t pos = 12;
@@ -55,24 +58,47 @@ bool read_wav(const std::string &path, std::vector<float> &out, int *sample_rate
         size_t body = pos + 8;
         size_t avail = b.size() - body;
         if (memcmp(ck, "fmt ", 4) == 0 && len >= 16 && avail >= 16) {
+            have_fmt = true;
             fmt_tag = le16(ck + 8);
             channels = le16(ck + 10);
-            if (sample_rate) *sample_rate = (int)le32(ck + 12);
+            rate = (int)le32(ck + 12);
             block_align = le16(ck + 20);
             bits = le16(ck + 22);
-            if (fmt_tag == 0xFFFE && len >= 40 && avail >= 40) fmt_tag = le16(ck + 32);
+            if (fmt_tag == 0xFFFE) {  // WAVE_FORMAT_EXTENSIBLE: the subformat GUID's first two bytes
+                if (len < 40 || avail < 40) {
+                    err = "truncated WAVE_FORMAT_EXTENSIBLE header in " + path;
+                    return false;
+                }
+                fmt_tag = le16(ck + 32);
+            }
         } else if (memcmp(ck, "data", 4) == 0) {
             data = ck + 8;
             data_len = (len <= avail) ? len : avail;
         }
-        pos = body + len + (len & 1);
+        pos = body + (size_t)len + (len & 1);
     }
-    if (!data || channels <= 0 || block_align <= 0) {
+    if (!have_fmt || !data || channels <= 0) {
         err = "WAVE file without fmt/data chunk: " + path;
         return false;
     }
+    const bool pcm_ok = fmt_tag == 1 && (bits == 8 || bits == 16 || bits == 24 || bits == 32);
+    const bool flt_ok = fmt_tag == 3 && (bits == 32 || bits == 64);
+    if (!pcm_ok && !flt_ok) {
+        err = (fmt_tag == 1 || fmt_tag == 3) ? "unsupported bit depth " + std::to_string(bits) + " in " + path
+                                              : "unsupported WAVE format tag in " + path;
+        return false;
+    }
+    if ((long long)block_align < (long long)channels * (bits / 8)) {
+        err = "WAVE block_align smaller than one frame of samples in " + path;
+        return false;
+    }
+    if (info) {
+        info->format_tag = fmt_tag;
+        info->channels = channels;
+        info->bits = bits;
+        info->sample_rate = rate;
+    }
     const size_t frames = data_len / (size_t)block_align;
-    const int bytes = bits / 8;
     out.resize(frames);
     for (size_t i = 0; i < frames; i++) {
         const unsigned char *s = data + i * (size_t)block_align;  // first channel
@@ -83,29 +109,16 @@ bool read_wav(const std::string &path, std::vector<float> &out, int *sample_rate
             else if (bits == 24) {
                 int32_t x = (int32_t)((uint32_t)s[0] << 8 | (uint32_t)s[1] << 16 | (uint32_t)s[2] << 24) >> 8;
                 v = x / 8388608.0;
-            } else if (bits == 32) v = (int32_t)le32(s) / 2147483648.0;
-            else {
-                err = "unsupported PCM bit depth in " + path;
-                return false;
-            }
-        } else if (fmt_tag == 3) {
-            if (bits == 32) {
-                float x;
-                memcpy(&x, s, 4);
-                v = x;
-            } else if (bits == 64) {
-                double x;
-                memcpy(&x, s, 8);
-                v = x;
-            } else {
-                err = "unsupported float bit depth in " + path;
-                return false;
-            }
+            } else v = (int32_t)le32(s) / 2147483648.0;
+        } else if (bits == 32) {
+            float x;
+            memcpy(&x, s, 4);
+            v = x;
         } else {
-            err = "unsupported WAVE format tag in " + path;
-            return false;
+            double x;
+            memcpy(&x, s, 8);
+            v = x;
         }
-        (void)bytes;
         out[i] = (float)v;
     }
     return true;

@@ -18,6 +18,12 @@ struct Config {
 };
 
 bool parse_config(const std::string &path, Config &cfg, std::string &err);
-bool read_wav(const std::string &path, std::vector<float> &out, int *sample_rate, std::string &err);
+struct WavInfo {
+    int format_tag = 0;  // 1 integer PCM, 3 IEEE float (WAVE_FORMAT_EXTENSIBLE resolved to its subformat)
+    int channels = 0, bits = 0, sample_rate = 0;
+};
+
+// First channel of a RIFF/WAVE file as float; false with a message in err.
+bool read_wav(const std::string &path, std::vector<float> &out, WavInfo *info, std::string &err);
 
 }  // namespace pcio

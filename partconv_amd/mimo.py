@@ -25,6 +25,7 @@ import ctypes
 
 import torch
 
+from . import _lib
 from ._lib import last_error, lib
 
 
@@ -60,12 +61,8 @@ class MimoConv:
         return y
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcx_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcx_destroy", h)
 
 
 class MimoRealtime:
@@ -148,12 +145,8 @@ class MimoRealtime:
         return list(N)[:M], list(P)[:M]
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcxr_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcxr_destroy", h)
 
 
 def _throughput_demo():

@@ -77,6 +77,23 @@ def _as_int_list(x, n=None):
     return x if n is None else x[:n]
 
 
+def readImpulseResponse(filename, verbosity=0):
+    """readImpulseResponse (src/sndtools.cpp:3-47): the first channel of a
+    RIFF/WAVE file as float32 (integer PCM scaled by 2^(bits-1)); returns
+    (samples, sample_rate).  Raises OSError for a missing, unsupported or
+    malformed file."""
+    n = ctypes.c_longlong()
+    sr = ctypes.c_int()
+    path = str(filename).encode()
+    if lib().pc_read_ir(path, None, 0, ctypes.byref(n), ctypes.byref(sr), int(verbosity)) != 0:
+        raise OSError(last_error())
+    out = np.zeros(n.value, dtype=np.float32)
+    if n.value and lib().pc_read_ir(path, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n.value,
+                                    ctypes.byref(n), None, 0) != 0:
+        raise OSError(last_error())
+    return out, sr.value
+
+
 def greedy_partition(block_sizes, filter_length):
     """PartConvMax's partition rule (src/partconv.cpp:712-728): returns (N, P)
     with the last count recounted for filter_length taps."""
@@ -138,12 +155,15 @@ class PartConvFilter:
         if rc != 0:
             _report(rc, "PartConvFilter ERROR")
             return 1
+        # the recounted partition lives in the filter's members; the caller's
+        # arrays are left as they were (src/partconv.cpp:83-84, 169)
+        Mo = ctypes.c_int()
+        No = (ctypes.c_int * M)()
+        Po = (ctypes.c_int * M)()
+        lib().pcf_get_levels(self._h, ctypes.byref(Mo), No, Po, M)
         self._M = M
-        self._N = list(N)
-        self._P = list(P)
-        if isinstance(num_parts_in, intArray):
-            for i in range(M):
-                num_parts_in[i] = P[i]
+        self._N = list(No)
+        self._P = list(Po)
         if self.verbosity > 0:
             print("partitioning: " + ", ".join(f"{a}x{b}" for a, b in zip(self._N, self._P)))
         return 0
@@ -216,12 +236,8 @@ class PartConvFilter:
         return y
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcf_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcf_destroy", h)
 
 
 class PartConvMulti:
@@ -396,12 +412,8 @@ class PartConvMulti:
     frameNum = property(lambda self: self.info().frameNum)
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcm_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcm_destroy", h)
 
 
 class PartConvMultiRelaxed(PartConvMulti):
@@ -497,9 +509,5 @@ class PartConvMax:
     frameNum = property(lambda self: self.info().frameNum)
 
     def __del__(self):
-        try:
-            if self._h:
-                lib().pcm_destroy(self._h)
-                self._h = None
-        except Exception:
-            pass
+        h, self._h = getattr(self, "_h", None), None
+        _lib.destroy("pcm_destroy", h)

@@ -154,8 +154,8 @@ int main()
 }
 """)
     exe = tmp_path / "dropin"
-    subprocess.run(["g++", "-std=c++17", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
-                    "-L", str(_lib.LIB_PATH.parent), "-lpartconv_amd",
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-Wno-unused-variable", "-I", str(ROOT / "include"),
+                    str(src), "-o", str(exe), "-L", str(_lib.LIB_PATH.parent), "-lpartconv_amd",
                     f"-Wl,-rpath,{_lib.LIB_PATH.parent}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
@@ -183,3 +183,73 @@ def test_prof_summary_splits_calls_by_phase(tmp_path, capsys):
             means[int(line[8])] = float(line.split()[-3])
     assert means == {0: 1.0, 1: 2.0, 2: 3.0}
     assert "k_other(int)" in out
+
+
+def test_cxx_reference_call_shapes(tmp_path):
+    """The reference's own call shapes compile and behave without a GPU:
+    PartConvFilter::setup(M, N, P, Vector*, fftwOptLevel, wisdom) with a
+    Vector, with NULL (src/tune.py:31 calls setup(M, N, P, None, 2,
+    'wisdom.wis')) and with defaults; readImpulseResponse(Vector&, file,
+    verbosity) (src/sndtools.h:16) and Vector's members; a partition error
+    returns 1 and leaves the caller's arrays unchanged (the reference copies
+    them, src/partconv.cpp:83-84); PartConvMulti::run counts frameNum only for
+    ticks that ran."""
+    import shutil
+    import struct
+    import subprocess
+    import numpy as np
+    from partconv_amd import _lib
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    pcm = (np.arange(300) - 150).astype("<i2")
+    fmt = struct.pack("<HHIIHH", 1, 1, 44100, 88200, 2, 16)
+    body = b"WAVE" + b"fmt " + struct.pack("<I", 16) + fmt + b"data" + struct.pack("<I", pcm.nbytes) + pcm.tobytes()
+    wav = tmp_path / "ir.wav"
+    wav.write_bytes(b"RIFF" + struct.pack("<I", len(body)) + body)
+    src = tmp_path / "shapes.cpp"
+    src.write_text(r"""
+#include "partconvMulti.h"
+#include "sndtools.h"
+int main(int argc, char **argv)
+{
+    Vector ir;
+    if (readImpulseResponse(ir, argv[1], 0) != 0) return 10;
+    if (ir.getSize() != 300 || ir[0] != -150.f / 32768.f || *ir(299) != 149.f / 32768.f) return 11;
+    Vector copy = ir;
+    if (copy.zeroPad(400, 10) != 0 || copy.getSize() != 400 || copy[10] != ir[0] || copy[0] != 0.f) return 12;
+    if (copy.zeroPad(100) != 1) return 13;
+    Vector missing;
+    if (readImpulseResponse(missing, "/nonexistent.wav") == 0) return 14;
+
+    PartConvFilter pc;
+    int M = 2;
+    int blocksize[2] = {64, 256}, numblocks[2] = {6, 7};
+    int bad_blocks[2] = {64, 100}, bad_counts[2] = {6, 7};
+    if (pc.setup(M, bad_blocks, bad_counts, NULL, 2, "wisdom.wis") != 1) return 20;   /* not a power of 2 */
+    if (bad_counts[1] != 7 || bad_blocks[1] != 100) return 21;                         /* caller's arrays kept */
+    int short_blocks[2] = {64, 256}, short_counts[2] = {1, 7};
+    if (pc.setup(M, short_blocks, short_counts, &ir) != 1) return 22;                  /* delay rule */
+    if (short_counts[1] != 7) return 23;
+    /* the valid shapes: without a GPU they fail at the device, after validation */
+    int r1 = pc.setup(M, blocksize, numblocks, NULL, 2, "wisdom.wis");
+    int r2 = pc.setup(M, blocksize, numblocks, &ir, 2, "wisdom.wis");
+    int r3 = pc.setup(M, blocksize, numblocks);
+    if (numblocks[1] != 7) return 24;
+    PartConvMulti multi;
+    float in[64] = {0}, out[64] = {0};
+    int rr = multi.run(out, in, 64);                                                   /* before setup: error */
+    if (rr == 0 || multi.frameNum != 0) return 25;
+    Waiter *w = multi.doneWaiter;
+    (void)w;
+    printf("%d %d %d\n", r1 == 0 || r1 == 1, r2 == 0 || r2 == 1, r3 == 0 || r3 == 1);
+    return 0;
+}
+""")
+    exe = tmp_path / "shapes"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-Wno-unused-variable", "-I", str(ROOT / "include"),
+                    str(src), "-o", str(exe), "-L", str(_lib.LIB_PATH.parent), "-lpartconv_amd", f"-Wl,-rpath,{_lib.LIB_PATH.parent}"],
+                   check=True)
+    out = subprocess.run([str(exe), str(wav)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, (out.returncode, out.stderr)
+    assert out.stdout.split() == ["1", "1", "1"]
+    assert "power of 2" in out.stderr and "sufficient delay" in out.stderr
