@@ -64,6 +64,8 @@ typedef struct pc_stats {
     double p50_us, p99_us, max_us, mean_us;
     double period_us;                   /* buffer_size / FS */
     unsigned long long persistent_ticks; /* ticks served by the persistent tick kernel */
+    unsigned long long stalls;           /* persistent runs a late level block interrupted (the
+                                            engine then waited for the block and went on) */
 } pc_stats;
 
 /* One instance of an in-memory setup: the fields of one config line
@@ -259,6 +261,16 @@ long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
 int pcm_set_persistent_workers(pc_multi *h, int mode);
 int pcm_workers_info(pc_multi *h, int *self_mode);
 
+/* Deadline-handling test hook (no reference counterpart): the next block of
+ * global level `global_level` (>= 1) launched from the host waits on its
+ * stream behind a host function that sleeps `seconds` (0 < seconds <= 5), so
+ * the tick that first reads the block is late.  A persistent run then ends
+ * its session after the block is PC_LEVEL_LATE (0.2 s) past due and serves the
+ * remaining ticks with per-tick launches that wait for the block on the
+ * device: the output is unchanged, the late tick counts as a deadline miss
+ * and pc_stats.stalls counts the interruption.  One shot. */
+int pcm_stall_level(pc_multi *h, int global_level, double seconds);
+
 /* Where the persistent kernel's release counter lives (no reference
  * counterpart): 1 = device memory written by the host through the PCIe BAR
  * (large-BAR systems; the kernel polls locally), 0 = pinned host memory (the
@@ -409,6 +421,12 @@ int pcxr_run(pc_mimo_rt *r, float *out, const float *in, int frameCount);
 int pcxr_reset(pc_mimo_rt *r);
 int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st);
 int pcxr_clear_stats(pc_mimo_rt *r);
+/* Deadline-handling test hook, as pcm_stall_level: the next block of level
+ * `level` (>= 1) waits behind a host function sleeping `seconds` (0, 5].  The
+ * engine stops its persistent kernel once the block is 0.2 s late, waits for
+ * the block, and serves the remaining ticks with a new persistent kernel
+ * (pc_stats.stalls counts it); output unchanged, the late tick a miss. */
+int pcxr_stall_level(pc_mimo_rt *r, int level, double seconds);
 /* partition after the last count is recounted; returns the level count */
 int pcxr_partition(const pc_mimo_rt *r, int *N, int *P, int cap);
 /* Time the delay-line-sum kernel of level `level` (>= 1; -1 disables) with HIP

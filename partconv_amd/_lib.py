@@ -31,7 +31,7 @@ class PcStats(ctypes.Structure):
     _fields_ = [("ticks", ctypes.c_ulonglong), ("deadline_misses", ctypes.c_ulonglong),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double),
                 ("mean_us", ctypes.c_double), ("period_us", ctypes.c_double),
-                ("persistent_ticks", ctypes.c_ulonglong)]
+                ("persistent_ticks", ctypes.c_ulonglong), ("stalls", ctypes.c_ulonglong)]
 
 
 class PcInstance(ctypes.Structure):
@@ -78,6 +78,7 @@ SIGNATURES = {
     "pcm_run_channels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
     "pcm_workers_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "pcm_release_info": (ctypes.c_int, [_vp]),
+    "pcm_stall_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),
@@ -124,6 +125,7 @@ SIGNATURES = {
     "pcxr_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(PcStats)]),
     "pcxr_clear_stats": (ctypes.c_int, [_vp]),
     "pcxr_partition": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int]),
+    "pcxr_stall_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pcxr_destroy": (None, [_vp]),
     "pcxr_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcxr_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),

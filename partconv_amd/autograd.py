@@ -120,8 +120,11 @@ class PartitionedConv:
         return PartConvFunction.apply(x, h, self)
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcc_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcc_destroy", h)
+        except Exception:
+            pass
 
 
 def _prep(t, shape_msg):

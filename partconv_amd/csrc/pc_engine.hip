@@ -338,6 +338,87 @@ static void hip_live()
 }
 
 // ----------------------------------------------------------------------------
+// Persistent-kernel launch.
+//
+// Cooperative launch first: the runtime then guarantees that every workgroup
+// is resident at once.  The device grants cooperative launches to one process
+// at a time (while a process that made one is alive, another process's
+// cooperative launch fails), and these kernels use no grid barrier, so on
+// refusal the kernel is launched plainly if it fits the device's occupancy.
+// Its workgroups then stamp a pinned word each on entry (stamp_present), and
+// the host releases the first tick only after seeing every stamp within
+// PC_RESIDENT_S; otherwise it stops the kernel (every wait in it ends at the
+// stop code) and falls back like any other failed launch.
+// ----------------------------------------------------------------------------
+#define PC_RESIDENT_S 0.1
+
+struct Residency {
+    unsigned *h = nullptr;  // pinned words, host view
+    unsigned *d = nullptr;  // device view
+    int cap = 0;
+    unsigned tag = 0;
+};
+
+static void resident_free(Residency &R)
+{
+    if (R.h) hipHostFree(R.h);
+    R = Residency();
+}
+
+static int resident_alloc(Residency &R, int n)
+{
+    if (R.cap >= n) return PC_OK;
+    resident_free(R);
+    HIPCHK(hipHostMalloc(&R.h, (size_t)n * sizeof(unsigned), hipHostMallocCoherent | hipHostMallocMapped));
+    memset(R.h, 0, (size_t)n * sizeof(unsigned));
+    HIPCHK(hipHostGetDevicePointer((void **)&R.d, R.h, 0));
+    R.cap = n;
+    return PC_OK;
+}
+
+// Launch fn over `grid` workgroups of nt threads; present / tag are the
+// kernel argument's stamp fields (args points at that argument).  *plain: the
+// launch was not cooperative and the caller must resident_wait before the
+// first release.
+static hipError_t launch_persistent(const void *fn, int grid, int nt, void **args, size_t lds, hipStream_t s,
+                                    Residency &R, int ncu, unsigned **present, unsigned *tag, bool *plain)
+{
+    *present = nullptr;
+    *plain = false;
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(nt), args, (unsigned)lds, s);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nt, lds) != hipSuccess || (long long)nb * ncu < grid ||
+        resident_alloc(R, grid) != PC_OK) {
+        (void)hipGetLastError();
+        return e;
+    }
+    if (++R.tag == 0) R.tag = 1;
+    *present = R.d;
+    *tag = R.tag;
+    const hipError_t e2 = hipLaunchKernel(fn, dim3(grid), dim3(nt), args, lds, s);
+    if (e2 != hipSuccess) {
+        (void)hipGetLastError();
+        *present = nullptr;
+        return e;
+    }
+    *plain = true;
+    return hipSuccess;
+}
+
+// Every workgroup of a plain launch stamped the current tag in time.
+static bool resident_wait(const Residency &R, int grid)
+{
+    const double t0 = now_s();
+    const volatile unsigned *h = R.h;
+    for (int w = 0; w < grid; w++)
+        while (h[w] != R.tag)
+            if (now_s() - t0 > PC_RESIDENT_S) return false;
+    return true;
+}
+
+// ----------------------------------------------------------------------------
 // Partition rules (PartConvFilter::setup validation, src/partconv.cpp:86-182)
 // ----------------------------------------------------------------------------
 static int check_partition(std::vector<int> &N, std::vector<int> &P, long long ir_len,
@@ -537,11 +618,18 @@ struct Engine {
                                       // 2: without tagged words
     int sess_host = 0;                // the active session serves host buffers
     int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
+    double pend_t0 = -1;              // submit time of a tick a stalled session left unserved (-1: none)
+    int ncu = 0;                      // compute units of the device
+    Residency resid;                  // residency stamps of plain persistent launches
+    // held level (pcm_stall_level): the next block of global level stall_g
+    // waits behind a host function that sleeps stall_us on the level's stream
+    std::atomic<int> stall_g{-1};
+    std::atomic<long long> stall_us{0};
     // stats
     std::vector<double> lat;
     size_t lat_pos = 0;
     bool lat_wrap = false;
-    unsigned long long ticks = 0, misses = 0;
+    unsigned long long ticks = 0, misses = 0, stalls = 0;
     double lat_sum = 0, lat_max = 0;
 };
 
@@ -631,6 +719,7 @@ static void engine_free(Engine &E)
     E.probe_g = -1;
     if (E.ev_tick) hipEventDestroy(E.ev_tick);
     E.ev_tick = nullptr;
+    resident_free(E.resid);
     for (hipStream_t s : E.s_lvl) hipStreamDestroy(s);
     E.s_lvl.clear();
     if (E.s_main) hipStreamDestroy(E.s_main);
@@ -1062,6 +1151,7 @@ static int engine_setup(Engine &E, int FS, int N0, int nIn, int nOut, std::vecto
     const int nls = std::min(PC_LEVEL_STREAMS, std::max(0, G - 1));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, E.dev));
+    E.ncu = prop.multiProcessorCount;
     std::vector<uint32_t> cu_mask((prop.multiProcessorCount + 31) / 32, 0u);
     for (int c = 0; c < prop.multiProcessorCount; c++) cu_mask[c / 32] |= 1u << (c % 32);
     for (int q = 0; q < nls; q++) {
@@ -1273,9 +1363,14 @@ static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
 }
 
 // Launch level g's blocks that start at tick f on its stream and record its event.
+static void host_sleep_fn(void *us) { std::this_thread::sleep_for(std::chrono::microseconds((uintptr_t)us)); }
+
 static int launch_level(Engine &E, int g, uint64_t f)
 {
     hipStream_t s = level_stream(E, g);
+    int want = g;
+    if (E.stall_g.load() == g && E.stall_g.compare_exchange_strong(want, -1))
+        HIPCHK(hipLaunchHostFunc(s, host_sleep_fn, (void *)(uintptr_t)E.stall_us.load()));
     const uint64_t m = f >> E.gshift[g];
     for (Group &Gr : E.groups)
         for (Batch &b : Gr.batches) {
@@ -1400,7 +1495,10 @@ static bool level_due(const Engine &E, int g, uint64_t f)
 static int engine_tick(Engine &E, const float *in, long long in_stride, long long fbase, float *out,
                        long long out_stride, bool wait)
 {
-    const double t0 = now_s();
+    // a tick that a stalled persistent session did not serve counts from its
+    // first submission
+    const double t0 = E.pend_t0 >= 0 ? E.pend_t0 : now_s();
+    E.pend_t0 = -1;
     const uint64_t f = E.frame;
     const int G = (int)E.gsize.size();
     // deadlines: wait for the blocks this tick reads for the first time
@@ -1490,28 +1588,27 @@ static int engine_tick(Engine &E, const float *in, long long in_stride, long lon
 // Grid: one tick workgroup per instance, then one workgroup per level worker.
 // Dynamic LDS: two float2 buffers of max(N0, largest worker block).
 template <int LOGN>
-static hipError_t launch_persist_t(const PersistArgs &p, const cf *tw, hipStream_t s, int nblocks, size_t lds)
+static hipError_t launch_persist_t(Engine &E, PersistArgs &pa, hipStream_t s, int nblocks, size_t lds, bool *plain)
 {
-    PersistArgs pa = p;
-    const cf *t = tw;
+    const cf *t = E.tw;
     void *args[] = {(void *)&pa, (void *)&t};
-    return hipLaunchCooperativeKernel((const void *)k_persist<LOGN>, dim3(nblocks), dim3(TickCfg<LOGN>::NT),
-                                      args, (unsigned)lds, s);
+    return launch_persistent((const void *)k_persist<LOGN>, nblocks, TickCfg<LOGN>::NT, args, lds, s, E.resid, E.ncu,
+                             &pa.present, &pa.present_tag, plain);
 }
 
-static hipError_t launch_persist(int N0, const PersistArgs &p, const cf *tw, hipStream_t s, int lds_n)
+static hipError_t launch_persist(Engine &E, PersistArgs &p, hipStream_t s, int lds_n, bool *plain)
 {
     const int nblocks = p.A.njobs + p.nworkers;
     const size_t lds = (size_t)2 * lds_n * sizeof(cf);
-    switch (ilog2(N0)) {
-    case 4: return launch_persist_t<4>(p, tw, s, nblocks, lds);
-    case 5: return launch_persist_t<5>(p, tw, s, nblocks, lds);
-    case 6: return launch_persist_t<6>(p, tw, s, nblocks, lds);
-    case 7: return launch_persist_t<7>(p, tw, s, nblocks, lds);
-    case 8: return launch_persist_t<8>(p, tw, s, nblocks, lds);
-    case 9: return launch_persist_t<9>(p, tw, s, nblocks, lds);
-    case 10: return launch_persist_t<10>(p, tw, s, nblocks, lds);
-    case 11: return launch_persist_t<11>(p, tw, s, nblocks, lds);
+    switch (ilog2(E.N0)) {
+    case 4: return launch_persist_t<4>(E, p, s, nblocks, lds, plain);
+    case 5: return launch_persist_t<5>(E, p, s, nblocks, lds, plain);
+    case 6: return launch_persist_t<6>(E, p, s, nblocks, lds, plain);
+    case 7: return launch_persist_t<7>(E, p, s, nblocks, lds, plain);
+    case 8: return launch_persist_t<8>(E, p, s, nblocks, lds, plain);
+    case 9: return launch_persist_t<9>(E, p, s, nblocks, lds, plain);
+    case 10: return launch_persist_t<10>(E, p, s, nblocks, lds, plain);
+    case 11: return launch_persist_t<11>(E, p, s, nblocks, lds, plain);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1736,6 +1833,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         if (E.lvl_pending[g]) HIPCHK(hipStreamWaitEvent(E.s_main, E.ev_lvl[g], 0));
     store_go(E, (unsigned long long)f0);
     hipError_t le = hipErrorInvalidValue;
+    bool plain = false;
     if (E.nwjobs > 0 && E.persist_workers && (E.nworkers > 0 || E.self_levels)) {
         // worker-level progress counters for a history served up to tick f0 - 1
         const int n = std::max(E.nwjobs, (int)E.inst.size());
@@ -1749,7 +1847,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         P.dev_tick = E.d_dev_tick;
         P.lvl_done = E.d_lvl_done;
         set_masks(true, E.nworkers > 0);
-        le = launch_persist(E.N0, P, E.tw, E.s_main, std::max(E.N0, E.lds_w));
+        le = launch_persist(E, P, E.s_main, std::max(E.N0, E.lds_w), &plain);
         if (le != hipSuccess) (void)hipGetLastError();  // too many workgroups to co-reside: no workers
         else E.workers_active = true;
     }
@@ -1760,7 +1858,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         P.self_levels = 0;
         P.dev_tick = P.lvl_done = nullptr;
         set_masks(false, false);
-        le = launch_persist(E.N0, P, E.tw, E.s_main, E.N0);
+        le = launch_persist(E, P, E.s_main, E.N0, &plain);
     }
     if (le != hipSuccess) {
         (void)hipGetLastError();
@@ -1772,11 +1870,21 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     launcher_start(E);
     E.sess_end = f0 + (uint64_t)nticks;
     E.sess_last = now_s();
+    if (plain && !resident_wait(E.resid, P.A.njobs + P.nworkers)) {
+        // not every workgroup found room: stop it, serve with per-tick launches
+        const int rc = persist_end(E, true);
+        return rc ? rc : 1;
+    }
     return PC_OK;
 }
 
 // Serve tick E.frame: 0 served, 1 not served (stalled: end the session and
 // fall back), other values are errors.
+// A level block that is not complete PC_LEVEL_LATE_S after its deadline tick
+// was submitted ends the session (the caller falls back to per-tick launches,
+// which wait for the block on the device); the unserved tick's submit time is
+// kept in pend_t0 so its latency counts from its first submission.
+#define PC_LEVEL_LATE_S 0.2
 static int persist_tick(Engine &E)
 {
     const uint64_t f = E.frame;
@@ -1788,14 +1896,20 @@ static int persist_tick(Engine &E)
         if (E.lvl_async[g]) {
             // the helper thread must have issued the block (and recorded the event)
             while (E.lau->done[g].load(std::memory_order_acquire) < E.lvl_start[g] + 1)
-                if (now_s() - t0 > 0.2) return 1;
+                if (now_s() - t0 > PC_LEVEL_LATE_S) {
+                    E.pend_t0 = t0;
+                    return 1;
+                }
             E.lvl_async[g] = false;
         }
         for (;;) {
             hipError_t q = hipEventQuery(E.ev_lvl[g]);
             if (q == hipSuccess) break;
             if (q != hipErrorNotReady) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
-            if (now_s() - t0 > 0.2) return 1;
+            if (now_s() - t0 > PC_LEVEL_LATE_S) {
+                E.pend_t0 = t0;
+                return 1;
+            }
         }
         E.lvl_pending[g] = false;
     }
@@ -1803,6 +1917,7 @@ static int persist_tick(Engine &E)
     else store_go(E, f + 1);
     E.sess_last = now_s();
     int wr = wait_flag(E, f + 1, true);
+    if (wr == 1) E.pend_t0 = t0;
     if (wr) return wr;
     record_latency(E, 1e6 * (now_s() - t0));
     E.frame++;
@@ -1831,6 +1946,8 @@ static int persist_end(Engine &E, bool stop)
         if (E.frame < E.sess_end && tick_done(E, E.frame + 1)) {
             // the kernel served the released tick after all (slow, not stalled)
             const uint64_t f = E.frame;
+            if (E.pend_t0 >= 0) record_latency(E, 1e6 * (now_s() - E.pend_t0));
+            E.pend_t0 = -1;
             E.frame++;
             E.sess_served_late = true;
             int rc = start_levels(E, f, true);
@@ -1862,6 +1979,7 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
         rc = persist_tick(E);
         if (rc == 1) {
             stalled = true;
+            E.stalls++;
             break;
         }
         if (rc) return rc;
@@ -2071,6 +2189,7 @@ static int host_tick(pc_multi *h)
                 return PC_OK;
             }
             if (rc != 1) return rc;
+            E.stalls++;
             E.sess_served_late = false;
             rc = session_stop(E);
             if (rc) return rc;
@@ -2260,6 +2379,7 @@ PC_API int pcm_get_stats(const pc_multi *h, pc_stats *st)
     st->period_us = E.FS > 0 ? 1e6 * E.N0 / E.FS : 0.0;
     st->max_us = E.lat_max;
     st->persistent_ticks = (unsigned long long)h->persistent_ticks;
+    st->stalls = E.stalls;
     st->mean_us = E.ticks ? E.lat_sum / (double)E.ticks : 0.0;
     const size_t n = E.lat_wrap ? E.lat.size() : E.lat_pos;
     if (n) {
@@ -2281,7 +2401,7 @@ PC_API int pcm_clear_stats(pc_multi *h)
     Engine &E = h->E;
     E.lat_pos = 0;
     E.lat_wrap = false;
-    E.ticks = E.misses = 0;
+    E.ticks = E.misses = E.stalls = 0;
     E.lat_sum = E.lat_max = 0;
     h->persistent_ticks = 0;
     return PC_OK;
@@ -2415,6 +2535,18 @@ PC_API int pcm_workers_info(pc_multi *h, int *self_mode)
     if (plan_workers(h->E)) return -1;
     if (self_mode) *self_mode = h->E.self_levels ? 1 : 0;
     return h->E.nworkers;
+}
+
+PC_API int pcm_stall_level(pc_multi *h, int global_level, double seconds)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    Engine &E = h->E;
+    if (!E.ready) return fail(PC_ERR_STATE, "stall before setup");
+    if (global_level < 1 || global_level >= (int)E.gsize.size()) return fail(PC_ERR_ARG, "level out of range");
+    if (!(seconds > 0.0 && seconds <= 5.0)) return fail(PC_ERR_ARG, "stall must be in (0, 5] seconds");
+    E.stall_us.store((long long)(seconds * 1e6));
+    E.stall_g.store(global_level);
+    return PC_OK;
 }
 
 PC_API int pcm_release_info(const pc_multi *h)
@@ -3504,6 +3636,13 @@ struct pc_mimo_rt {
     std::vector<uint64_t> lstart;
     int lerr = 0;
     std::string lmsg;
+    // held level (pcxr_stall_level) and the persistent kernel's restarts
+    // after a late level block
+    std::atomic<int> stall_l{-1};
+    std::atomic<long long> stall_us{0};
+    unsigned long long stalls = 0;
+    int ncu = 0;
+    Residency resid;  // residency stamps of plain persistent launches
 };
 
 static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f);
@@ -3605,6 +3744,7 @@ static void mxr_free(pc_mimo_rt *r)
     r->h_in = r->h_out = r->d_in_map = r->d_out_map = nullptr;
     release_slot_put(r->dev, r->go_slot);
     r->go_slot = -1;
+    resident_free(r->resid);
 }
 
 static int mxr_alloc(pc_mimo_rt *r, void **p, size_t bytes)
@@ -3667,6 +3807,7 @@ static int mxr_setup(pc_mimo_rt *r)
     HIPCHK(hipStreamCreateWithFlags(&r->s_main, hipStreamNonBlocking));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, r->dev));
+    r->ncu = prop.multiProcessorCount;
     std::vector<uint32_t> cu_mask((prop.multiProcessorCount + 31) / 32, 0u);
     for (int c = 0; c < prop.multiProcessorCount; c++) cu_mask[c / 32] |= 1u << (c % 32);
     r->ev_lvl.assign(M, nullptr);
@@ -3790,6 +3931,9 @@ static void mxr_level_t(pc_mimo_rt *r, int l, uint64_t m)
 static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f)
 {
     const uint64_t m = f >> r->A.lv[l].kshift;
+    int want = l;
+    if (r->stall_l.load() == l && r->stall_l.compare_exchange_strong(want, -1))
+        HIPCHK(hipLaunchHostFunc(r->s_lvl[l], host_sleep_fn, (void *)(uintptr_t)r->stall_us.load()));
     switch (ilog2(r->A.lv[l].N)) {
     case 7: mxr_level_t<7>(r, l, m); break;
     case 8: mxr_level_t<8>(r, l, m); break;
@@ -3805,7 +3949,7 @@ static int mxr_start_level(pc_mimo_rt *r, int l, uint64_t f)
 }
 
 template <int LOGN>
-static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
+static hipError_t mxr_launch_t(pc_mimo_rt *r, int G, bool *plain)
 {
     MxrArgs a = r->A;
     const cf *t = r->tw;
@@ -3821,8 +3965,8 @@ static hipError_t mxr_launch_t(pc_mimo_rt *r, int G)
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    return hipLaunchCooperativeKernel((const void *)k_mxr_persist<LOGN>, dim3(G), dim3(MXR_NTT), args,
-                                      (unsigned)lds, r->s_main);
+    return launch_persistent((const void *)k_mxr_persist<LOGN>, G, MXR_NTT, args, lds, r->s_main, r->resid, r->ncu,
+                             &a.present, &a.present_tag, plain);
 }
 
 static void mxr_store_go(pc_mimo_rt *r, unsigned long long v)
@@ -3836,8 +3980,62 @@ static void mxr_store_go(pc_mimo_rt *r, unsigned long long v)
     }
 }
 
+// Launch the persistent level-0 kernel for nticks ticks from r->frame (frames
+// of this call start at tick A.fbase).  The kernel waits for each release; its
+// counters restart from zero.
+static int mxr_persist_launch(pc_mimo_rt *r, long long nticks)
+{
+    MxrArgs &A = r->A;
+    A.f0 = r->frame;
+    A.nticks = nticks;
+    // the grid: input and output workgroups, then one delay-line-sum workgroup
+    // per bin (at most 256)
+    const int G = r->n_in + r->n_out + std::min(r->N0, 256);
+    HIPCHK(hipMemsetAsync(r->d_bar, 0, 640, r->s_main));
+    mxr_store_go(r, r->frame);
+    hipError_t le = hipErrorInvalidValue;
+    bool plain = false;
+    switch (ilog2(r->N0)) {
+    case 6: le = mxr_launch_t<6>(r, G, &plain); break;
+    case 7: le = mxr_launch_t<7>(r, G, &plain); break;
+    case 8: le = mxr_launch_t<8>(r, G, &plain); break;
+    case 9: le = mxr_launch_t<9>(r, G, &plain); break;
+    case 10: le = mxr_launch_t<10>(r, G, &plain); break;
+    }
+    if (le != hipSuccess) return fail(PC_ERR_HIP, std::string("real-time MIMO launch: ") + hipGetErrorString(le));
+    if (plain && !resident_wait(r->resid, G)) {
+        mxr_store_go(r, PC_GO_STOP);  // every wait of the kernel ends at the stop code
+        HIPCHK(hipStreamSynchronize(r->s_main));
+        return fail(PC_ERR_HIP, "real-time MIMO: the persistent kernel's workgroups could not all be resident");
+    }
+    return PC_OK;
+}
+
+// Wait until level l's pending block has been issued by the helper and has
+// completed, for at most `limit` seconds from t0: 0 done, 1 still late, else
+// an error.
+static int mxr_level_wait(pc_mimo_rt *r, int l, double t0, double limit)
+{
+    while (r->ldone[l].load(std::memory_order_acquire) < r->lstart[l] + 1)
+        if (now_s() - t0 > limit) return 1;
+    for (;;) {
+        const hipError_t q = hipEventQuery(r->ev_lvl[l]);
+        if (q == hipSuccess) return PC_OK;
+        if (q != hipErrorNotReady) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
+        if (now_s() - t0 > limit) return 1;
+    }
+}
+
 // d_in / d_out: nframes frames of buffer_size x n_in / n_out interleaved
 // samples on the device.  Every tick is released and waited for (real time).
+// The strict run of the reference waits for late levels
+// (src/partconvMulti.cpp:886-903); here a level block that is not complete
+// PC_LEVEL_LATE_S after its deadline tick was submitted ends the persistent
+// kernel (it exits at its next release wait, with the state of every served
+// tick complete), the host waits for the block, and a new persistent kernel
+// serves the remaining ticks: a level kernel that cannot run beside the
+// resident one then runs alone.  The output is unchanged; the late tick's
+// latency counts from its first submission (a deadline miss).
 PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long long nframes)
 {
     if (!r || !d_out || !d_in || nframes < 0) return fail(PC_ERR_ARG, "bad argument");
@@ -3850,22 +4048,10 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     A.out = d_out;
     A.out_stride = (long long)N0 * r->n_out;
     A.fbase = (long long)r->frame;
-    A.f0 = r->frame;
-    A.nticks = nframes;
-    // the grid: input and output workgroups, then one delay-line-sum workgroup
-    // per bin (at most 256)
-    const int G = r->n_in + r->n_out + std::min(N0, 256);
-    HIPCHK(hipMemsetAsync(r->d_bar, 0, 640, r->s_main));
-    mxr_store_go(r, r->frame);
-    hipError_t le = hipErrorInvalidValue;
-    switch (ilog2(N0)) {
-    case 6: le = mxr_launch_t<6>(r, G); break;
-    case 7: le = mxr_launch_t<7>(r, G); break;
-    case 8: le = mxr_launch_t<8>(r, G); break;
-    case 9: le = mxr_launch_t<9>(r, G); break;
-    case 10: le = mxr_launch_t<10>(r, G); break;
+    {
+        const int rc = mxr_persist_launch(r, nframes);
+        if (rc) return rc;
     }
-    if (le != hipSuccess) return fail(PC_ERR_HIP, std::string("real-time MIMO launch: ") + hipGetErrorString(le));
     if (r->M > 1) mxr_launcher_start(r);
     const double period = 1e6 * N0 / r->FS;
     int err = PC_OK;
@@ -3873,32 +4059,32 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
         const uint64_t f = r->frame;
         const double t0 = now_s();
         // blocks first read at this tick (strict: (f + 1) % k == 0) must be complete
+        bool late = false;
         for (int l = 1; l < r->M && !err; l++) {
             const uint64_t k = 1ull << A.lv[l].kshift;
             if ((f + 1) % k != 0 || !r->pending[l]) continue;
-            // the helper thread has issued the block's launches and its event
-            while (r->ldone[l].load(std::memory_order_acquire) < r->lstart[l] + 1) {
-                if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
-                    err = fail(PC_ERR_TIMEOUT, "level block was not launched");
-                    break;
-                }
-            }
-            if (err) break;
-            for (;;) {
-                const hipError_t q = hipEventQuery(r->ev_lvl[l]);
-                if (q == hipSuccess) break;
-                if (q != hipErrorNotReady) {
-                    err = fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
-                    break;
-                }
-                if (now_s() - t0 > PC_TICK_TIMEOUT_S) {
-                    err = fail(PC_ERR_TIMEOUT, "level block did not complete");
-                    break;
-                }
-            }
-            r->pending[l] = false;
+            const int w = mxr_level_wait(r, l, t0, PC_LEVEL_LATE_S);
+            if (w == 1) late = true;
+            else if (w) err = w;
+            else r->pending[l] = false;
         }
         if (err) break;
+        if (late) {
+            // tick f is not released: the kernel ends at its release wait
+            mxr_store_go(r, PC_GO_STOP);
+            HIPCHK(hipStreamSynchronize(r->s_main));
+            for (int l = 1; l < r->M && !err; l++) {
+                if ((f + 1) % (1ull << A.lv[l].kshift) != 0 || !r->pending[l]) continue;
+                const int w = mxr_level_wait(r, l, t0, PC_TICK_TIMEOUT_S);
+                if (w == 1) err = fail(PC_ERR_TIMEOUT, "level block did not complete");
+                else if (w) err = w;
+                else r->pending[l] = false;
+            }
+            if (err) break;
+            r->stalls++;
+            err = mxr_persist_launch(r, nframes - t);
+            if (err) break;
+        }
         mxr_store_go(r, f + 1);
         for (int o = 0; o < r->n_out && !err; o++) {
             const volatile unsigned long long *fl = r->h_flags + (size_t)o * 8;
@@ -3986,6 +4172,7 @@ PC_API int pcxr_get_stats(const pc_mimo_rt *r, pc_stats *st)
     st->deadline_misses = r->misses;
     st->period_us = 1e6 * r->N0 / r->FS;
     st->persistent_ticks = r->ticks;
+    st->stalls = r->stalls;
     if (!r->lat.empty()) {
         std::vector<double> v = r->lat;
         std::sort(v.begin(), v.end());
@@ -4001,7 +4188,7 @@ PC_API int pcxr_clear_stats(pc_mimo_rt *r)
 {
     if (!r) return fail(PC_ERR_ARG, "null handle");
     r->lat.clear();
-    r->ticks = r->misses = 0;
+    r->ticks = r->misses = r->stalls = 0;
     r->lat_sum = r->lat_max = 0;
     return PC_OK;
 }
@@ -4049,6 +4236,16 @@ PC_API int pcxr_probe_read(pc_mimo_rt *r, double *avg_us, long long *launches, d
         }
         *bytes_per_launch = b;
     }
+    return PC_OK;
+}
+
+PC_API int pcxr_stall_level(pc_mimo_rt *r, int level, double seconds)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    if (level < 1 || level >= r->M) return fail(PC_ERR_ARG, "level out of range");
+    if (!(seconds > 0.0 && seconds <= 5.0)) return fail(PC_ERR_ARG, "stall must be in (0, 5] seconds");
+    r->stall_us.store((long long)(seconds * 1e6));
+    r->stall_l.store(level);
     return PC_OK;
 }
 

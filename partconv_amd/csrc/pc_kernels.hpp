@@ -1011,7 +1011,18 @@ struct PersistArgs {
     int self_levels;               // no workers: each tick workgroup runs its own worker-level blocks
     unsigned long long *dev_tick;  // per tick workgroup: frames published to the workers
     unsigned long long *lvl_done;  // per worker slot: 2m + 1 (block m out), 2m + 2 (S of m + 1 too)
+    unsigned *present;             // plain (non-cooperative) launch: workgroup w stores present_tag
+    unsigned present_tag;          // into present[w] (host-visible) on entry; null otherwise
 };
+
+// Residency stamp of a persistent kernel launched without the cooperative
+// launch's guarantee: the host releases the first tick only once every
+// workgroup has stamped (each one polls the release from then on).
+__device__ __forceinline__ void stamp_present(unsigned *present, unsigned tag, int tid)
+{
+    if (present && tid == 0)
+        __hip_atomic_store(present + blockIdx.x, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 
 // S[k] = sum_{p=1}^{P-1} FDL[(b - p) % P][k] H_p[k] for one job, by one workgroup
@@ -1315,6 +1326,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     __shared__ int s_go;
     __shared__ TickJob sTJ;
     const int tid = threadIdx.x;
+    stamp_present(P.present, P.present_tag, tid);
     if ((int)blockIdx.x >= P.A.njobs) {
         const int2 range = P.wrange[blockIdx.x - P.A.njobs];
         const int lw = P.wjobs[range.x].logN;

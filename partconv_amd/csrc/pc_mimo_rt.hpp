@@ -67,6 +67,8 @@ struct MxrArgs {
                                 // before each launch)
     unsigned long long spin_limit;
     int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
+    unsigned *present;      // plain launch: residency stamps (see stamp_present), else null
+    unsigned present_tag;
 };
 
 // Bin k of level L, for every output: the sum over partitions p in [p_lo, p_hi)
@@ -227,6 +229,7 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
     const int n_io = A.n_in + A.n_out, G4 = (int)gridDim.x - n_io;
     const MxrLevel &L0 = A.lv[0];
     unsigned long long *cX = A.bar, *cS = A.bar + 8;
+    stamp_present(A.present, A.present_tag, tid);
     // input and output workgroups: the transforms' twiddles (entries below
     // 2 N0 + N0 / 2 + 1) in LDS, so the transforms make no global loads;
     // output workgroups also keep their partition-0 filter spectra,

@@ -61,8 +61,11 @@ class MimoConv:
         return y
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcx_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcx_destroy", h)
+        except Exception:
+            pass
 
 
 class MimoRealtime:
@@ -138,6 +141,12 @@ class MimoRealtime:
             raise RuntimeError(last_error())
         return a.value, n.value, b.value
 
+    def stall_level(self, level, seconds):
+        """Deadline-handling test hook (pcxr_stall_level): the next block of
+        level `level` waits `seconds` behind a host function on its stream."""
+        if lib().pcxr_stall_level(self._h, int(level), float(seconds)) != 0:
+            raise RuntimeError(last_error())
+
     def partition(self):
         N = (ctypes.c_int * 16)()
         P = (ctypes.c_int * 16)()
@@ -145,8 +154,11 @@ class MimoRealtime:
         return list(N)[:M], list(P)[:M]
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcxr_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcxr_destroy", h)
+        except Exception:
+            pass
 
 
 def _throughput_demo():

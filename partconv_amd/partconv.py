@@ -236,8 +236,11 @@ class PartConvFilter:
         return y
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcf_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcf_destroy", h)
+        except Exception:
+            pass
 
 
 class PartConvMulti:
@@ -403,6 +406,11 @@ class PartConvMulti:
         M = lib().pcm_instance_partition(self._h, int(i), N, P, 16)
         return list(N)[:M], list(P)[:M]
 
+    def stall_level(self, global_level, seconds):
+        """Deadline-handling test hook (pcm_stall_level): the next block of a
+        global level waits `seconds` behind a host function on its stream."""
+        return _report(lib().pcm_stall_level(self._h, int(global_level), float(seconds)), "stall_level")
+
     # public fields (src/partconvMulti.h:64-79)
     FS = property(lambda self: self.info().FS)
     buffer_size = property(lambda self: self.info().buffer_size)
@@ -412,8 +420,11 @@ class PartConvMulti:
     frameNum = property(lambda self: self.info().frameNum)
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcm_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcm_destroy", h)
+        except Exception:
+            pass
 
 
 class PartConvMultiRelaxed(PartConvMulti):
@@ -509,5 +520,8 @@ class PartConvMax:
     frameNum = property(lambda self: self.info().frameNum)
 
     def __del__(self):
-        h, self._h = getattr(self, "_h", None), None
-        _lib.destroy("pcm_destroy", h)
+        try:  # module globals may already be gone at interpreter exit
+            h, self._h = self.__dict__.get("_h"), None
+            _lib.destroy("pcm_destroy", h)
+        except Exception:
+            pass
