@@ -350,6 +350,19 @@ int pcc_levels(const pc_conv *c, int *N, int *P, int cap);
 int pcc_set_filter(pc_conv *c, const float *d_h, void *stream);
 /* y[C][T] = partitioned convolution of x[C][T] with the filter. */
 int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream);
+/* pcc_forward without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same result as
+ * pcc_forward (the same kernels in the same order, bitwise), but every level's
+ * block spectra, products and transform staging live in one file-scope device
+ * array of 64 MiB per device (held until the process ends) instead of the
+ * handle's grown work space; nothing is allocated, freed or synchronised.
+ * Limit: 3 * C * ceil(T / N_L) * N_L * 8 bytes <= 64 MiB for every level L
+ * (e.g. 16 clips x 131072 samples), else PC_ERR_ARG before any GPU work.
+ * No spectra are kept for pcc_backward (it recomputes them from x).  The
+ * filter must be set (pcc_set_filter).  Calls that overlap in time on one
+ * device share the array and race. */
+int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream);
 /* Adjoint for upstream gradient g[C][T]: dx[C][T] = correlation of g with h,
  * dh[taps] = sum over clips of the correlation of g with x (either output may
  * be null). */

@@ -2998,18 +2998,24 @@ static void off_irfft_t(const cf *Y, const InvDst &D, int ntrans, const cf *tw, 
     hipLaunchKernelGGL(k_off_irfft_small<LOGN>, dim3(ntrans), dim3(SmallCfg<LOGN>::NT), 0, s, Y, D, tw);
 }
 
-// batched r2c of `rows * W.nb` windows into X (N float2 each)
-static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, cf *X, hipStream_t s)
+// batched r2c of `rows * W.nb` windows into X (N float2 each).  Big levels
+// stage through `scr` (rows * W.nb * N float2) if given, else the handle's
+// grown work space.
+static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, cf *X, hipStream_t s,
+                    cf *scr = nullptr)
 {
     const int nt = rows * W.nb;
     if (nt == 0) return PC_OK;
     if (L.big) {
-        int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)nt * L.N);
-        if (rc) return rc;
+        if (!scr) {
+            int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)nt * L.N);
+            if (rc) return rc;
+            scr = c->scr;
+        }
         const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
         hipLaunchKernelGGL(k_off_big_fwd1, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, W, L.sh,
-                           c->scr, c->tw);
-        hipLaunchKernelGGL(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, c->scr,
+                           scr, c->tw);
+        hipLaunchKernelGGL(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, scr,
                            X, W.scale, c->tw);
     } else {
         switch (ilog2(L.N)) {
@@ -3028,18 +3034,24 @@ static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, c
     return PC_OK;
 }
 
-// batched c2r of `rows * D.nb` spectra (all blocks, or one parity class)
-static int off_irfft(pc_conv *c, const ConvLevel &L, const cf *Y, const InvDst &D, int rows, hipStream_t s)
+// batched c2r of `rows * D.nb` spectra (all blocks, or one parity class);
+// big levels stage through `scr` (rows * D.nb * N float2) if given, else the
+// handle's grown work space.
+static int off_irfft(pc_conv *c, const ConvLevel &L, const cf *Y, const InvDst &D, int rows, hipStream_t s,
+                     cf *scr = nullptr)
 {
     const int nt = (D.parity >= 0) ? rows * ((D.nb + 1 - D.parity) / 2) : rows * D.nb;
     if (nt == 0) return PC_OK;
     if (L.big) {
-        int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)rows * D.nb * L.N);
-        if (rc) return rc;
+        if (!scr) {
+            int rc = conv_grow(&c->scr, &c->scr_cap, (size_t)rows * D.nb * L.N);
+            if (rc) return rc;
+            scr = c->scr;
+        }
         const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
-        hipLaunchKernelGGL(k_off_big_inv1, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, Y, c->scr,
+        hipLaunchKernelGGL(k_off_big_inv1, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, Y, scr,
                            c->tw, D.parity, D.nb);
-        hipLaunchKernelGGL(k_off_big_inv2, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, L.sh, c->scr,
+        hipLaunchKernelGGL(k_off_big_inv2, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, L.sh, scr,
                            D, c->tw);
     } else {
         switch (ilog2(L.N)) {
@@ -3183,6 +3195,26 @@ static WinSrc conv_xwin(const ConvLevel &L, const float *x, int T)
     return W;
 }
 
+// The level's inverse-transform target for the forward: level 0's blocks tile
+// the clip, so they write every sample once and the later levels add to it
+// (no clearing pass).
+static InvDst conv_ydst(const ConvLevel &L, float *y, int T, int M, bool first)
+{
+    InvDst D{};
+    D.dst = y;
+    D.row_stride = T;
+    D.len = T;
+    D.base0 = 0;
+    D.base_step = L.N;
+    D.nb = M;
+    D.ulo = 0;
+    D.uhi = L.N;
+    D.scale = 1.f;
+    D.accumulate = first ? 0 : 1;
+    D.parity = -1;
+    return D;
+}
+
 // y = sum over levels of the partitioned convolution of every clip (x, y: [C][T])
 PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
 {
@@ -3222,27 +3254,81 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
             off_mac(X, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
         }
         HIPCHK(hipGetLastError());
-        InvDst D{};
-        D.dst = d_y;
-        D.row_stride = T;
-        D.len = T;
-        D.base0 = 0;
-        D.base_step = L.N;
-        D.nb = M;
-        D.ulo = 0;
-        D.uhi = L.N;
-        D.scale = 1.f;
-        // level 0's blocks tile the clip: they write every sample once, the
-        // later levels add to it (no clearing pass)
-        D.accumulate = li > 0;
-        D.parity = -1;
-        if ((rc = off_irfft(c, L, c->work, D, C, s))) return rc;
+        if ((rc = off_irfft(c, L, c->work, conv_ydst(L, d_y, T, M, li == 0), C, s))) return rc;
     }
     if (keep) {
         c->keep_valid = true;
         c->keep_x = d_x;
         c->keep_C = C;
         c->keep_T = T;
+    }
+    return PC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pcc_forward without allocation: the same kernels in the same order, with
+// the block spectra, the products and the four-step staging of each level in
+// one file-scope device array instead of the handle's grown work space.  The
+// array's address on the current device comes from hipGetSymbolAddress (a
+// host-side use of its name would be a host address), taken once per device
+// and kept.  Levels run one after another on the caller's stream, so each
+// level reuses the array from its start: X (C M N), products (C M N), staging
+// (C M N), float2 each.  Every word a level reads was written earlier in the
+// same call, so nothing needs clearing.  All calls on one device share the
+// array: calls that overlap in time on one device race.
+// ---------------------------------------------------------------------------
+#define PC_NOALLOC_BYTES (64ull << 20)
+static __device__ __align__(256) cf scratch[PC_NOALLOC_BYTES / sizeof(cf)];
+static std::mutex g_scratch_mu;
+static cf *g_scratch_ptr[64];
+
+static int scratch_ptr(cf **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(PC_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<cf *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PC_OK;
+}
+
+PC_API int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
+{
+    if (!c || !d_x || !d_y || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "forward before set_filter");
+    for (const ConvLevel &L : c->lv) {
+        const unsigned long long need = 3ull * (unsigned long long)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
+        if (need > PC_NOALLOC_BYTES) {
+            char m[200];
+            snprintf(m, sizeof m,
+                     "pcc_forward_noalloc: %d clips x %d samples need %llu bytes of scratch at block size %d; "
+                     "the maximum is %llu (split the clips)",
+                     C, T, need, L.N, PC_NOALLOC_BYTES);
+            return fail(PC_ERR_ARG, m);
+        }
+    }
+    cf *base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    c->keep_valid = false;  // no spectra kept: a backward recomputes them from x
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        ConvLevel &L = c->lv[li];
+        WinSrc W = conv_xwin(L, d_x, T);
+        const int M = W.nb;
+        const size_t n = (size_t)C * M * L.N;
+        cf *X = base, *Y = base + n, *stage = base + 2 * n;
+        if ((rc = off_rfft(c, L, W, C, X, s, stage))) return rc;
+        off_mac(X, L.H, Y, C, M, L, 1, s, nullptr, nullptr);
+        HIPCHK(hipGetLastError());
+        if ((rc = off_irfft(c, L, Y, conv_ydst(L, d_y, T, M, li == 0), C, s, stage))) return rc;
     }
     return PC_OK;
 }
