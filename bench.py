@@ -93,13 +93,32 @@ def tick_bytes(parts, n0):
     return sum(block_bytes(n, p) * n0 / n for N, P in parts for n, p in zip(N, P))
 
 
-def hbm_roofline(bytes_per_tick, ticks, secs, ch_per_tick):
-    """Whole-workload HBM roofline: algorithmic bytes of every block served,
-    over the wall time, against MI355X's 8 TB/s."""
+def workload_roofline(bytes_per_tick, ticks, secs, ch_per_tick, what, limit):
+    """The timed region's roofline: the algorithmic bytes of every block served
+    (canonical 8(N+1)(2P+1)+12N per level block, or the engine's own bytes
+    where stated) over the wall time of the timed region, against 8 TB/s.
+    `limit` says what bounds the region when it is not HBM."""
     gbps = bytes_per_tick * ticks / secs / 1e9
-    return {"alg_bytes_per_ch_sample": bytes_per_tick / ch_per_tick, "achieved_GBps": gbps,
-            "peak_GBps": HBM_PEAK_GBPS, "frac": gbps / HBM_PEAK_GBPS,
-            "note": "whole workload: canonical block bytes 8(N+1)(2P+1)+12N of every block over wall time"}
+    return {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": None, "kernel": what, "alg_bytes_per_tick": bytes_per_tick,
+            "alg_bytes_per_ch_sample": bytes_per_tick / ch_per_tick, "limit": limit,
+            "traffic_note": "per-kernel HBM traffic is in kernel_rooflines (counter runs)"}
+
+
+def mimo_engine_tick_bytes(levels, n_in, n_out, n0):
+    """Bytes the real-time MIMO engine moves per tick (its shared-input
+    formulation, not the pair instances'): per level block of N bins and P
+    partitions, the filter spectra of every pair (level 0's p = 0 block stays in
+    the output workgroups' LDS for the whole run), P delay-line rows read and
+    one written per input, the delay-line sum written and read per output, the
+    2N-sample window read per input and N outputs written per output; a level
+    runs one block every N / n0 ticks."""
+    tot = 0.0
+    for l, (N, P) in enumerate(levels):
+        h = 8.0 * N * (P - (1 if l == 0 else 0)) * n_out * n_in
+        blk = h + 8.0 * N * (P + 1) * n_in + 16.0 * N * n_out + 8.0 * N * n_in + 4.0 * N * n_out
+        tot += blk * n0 / N
+    return tot
 
 
 def kernel_roofline(avg_us, launches, bytes_per, kernel):
@@ -113,52 +132,80 @@ def kernel_roofline(avg_us, launches, bytes_per, kernel):
 
 
 def with_traffic(roof, name):
-    """Attach the HBM bytes per launch measured by FETCH_SIZE / WRITE_SIZE
-    counters (copy-calibrated; profiles/<name>) to a kernel roofline."""
+    """Attach the HBM bytes per launch that FETCH_SIZE / WRITE_SIZE counters
+    measured for this kernel (copy-calibrated, profiles/<name>) to a kernel
+    roofline, labelled with that file: the counters come from a separate
+    profiled run of the same bench command (recorded in the file), not from
+    this run."""
     path = os.path.join(ROOT, "profiles", name)
     if roof and os.path.exists(path):
         with open(path) as fh:
             t = json.load(fh)
         roof["traffic"] = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
+        roof["traffic_source"] = (f"profiles/{name}: rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                                  f"copy-calibrated, separate profiled run"
+                                  + (f" at commit {t['commit']}" if t.get("commit") else ""))
     return roof
 
 
-def oracle_threads(units, budget_s, calib_ticks):
-    """Time OracleMulti units (each (engine, input columns)) on threads over the
-    host's cores, every unit processing the same number of ticks; the tick count
-    is calibrated for about budget_s.  Returns (ticks, seconds, cores)."""
-    import threading
-    cores = max(1, min(len(units), len(os.sched_getaffinity(0))))
+def cpu_reference_schedule(specs, in_ch, n_in, n0, budget_s, channels_per_tick, what, out_sum=None):
+    """CPU baseline with the reference's scheduling (src/partconvMulti.cpp:815-1013):
+    the fp32 CPU restatement of PartConvFilter per instance, level 0 on the
+    calling thread, one worker-thread group per global block size (started at
+    the level's start tick, waited for at the tick that first reads it), ticks
+    back to back over a cyclic seeded white-noise input, for about budget_s.
+    specs: (N, P, ir) per instance.  Reports channel-samples/s, per-tick p50 /
+    p99 against the period, and per level the reference's own load metric,
+    100 t_level / (N_L / FS) for one instance's block (src/partconv.cpp:553-559),
+    and the share of its worker threads' time."""
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    fs = [O.OracleFilter(N, P, h) for N, P, h in specs]
+    G = len({n for N, _, _ in specs for n in N[1:]})
+    nth = max(1, (cores - 1) // max(1, G))
+    chunk = 512
+    x = np.random.default_rng(123).standard_normal((chunk * n0, n_in)).astype(np.float32)
+    lats, lvl_sum, calls = [], None, 0
+    t0 = time.perf_counter()
+    while True:
+        _, lat, lv = O.rt_run(fs, x, nth, in_ch=in_ch)
+        lats.append(lat)
+        lvl_sum = [t for _, t in lv] if lvl_sum is None else [a + t for a, (_, t) in zip(lvl_sum, lv)]
+        calls += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    wall = time.perf_counter() - t0
+    for f in fs:
+        f.close()
+    lat = np.concatenate(lats) * 1e6
+    period = 1e6 * n0 / FS
+    ticks = lat.size
+    levels = []
+    for (n, _), tsum in zip(lv, lvl_sum):
+        t = tsum / calls
+        users = sum(1 for N, _, _ in specs if n in N)
+        share = users / (nth if n > n0 else 1)
+        levels.append({"block": n, "block_us": 1e6 * t, "pct_of_block_period": 100.0 * t / (n / FS),
+                       "thread_load_pct": 100.0 * t * share / (n / FS)})
+    return {"value": channels_per_tick * ticks * n0 / wall, "unit": "channel-samples/s",
+            "cores": 1 + nth * G, "kind": "port",
+            "sample": f"{ticks} ticks back to back ({ticks * n0 / FS:.1f} s of audio) of {what}, "
+                      f"{wall:.1f} s of CPU time",
+            "schedule": f"reference's: level 0 on the calling thread, {G} worker groups of {nth} threads "
+                        f"(one per global block size), a wait at each level's deadline tick",
+            "fft_backend": "project-written radix-2 complex FFT + real split, fp32 (FFTW is absent on this "
+                           "host; the reference uses FFTW 3.2.2)",
+            "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max()), "period": period, "misses": int(np.sum(lat > period))},
+            "levels": levels}
 
-    def run(nticks):
-        def work(w):
-            for u in range(w, len(units), cores):
-                eng, cols = units[u]
-                eng.process_frames(cols, nticks)
-        th = [threading.Thread(target=work, args=(w,)) for w in range(cores)]
-        t0 = time.perf_counter()
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
-        return time.perf_counter() - t0
 
-    t_cal = run(calib_ticks)
-    for eng, _ in units:
-        eng.reset()
-    max_ticks = min(cols.shape[0] for _, cols in units) // units[0][0].N0
-    ticks = max(calib_ticks, min(max_ticks, int(calib_ticks * budget_s / max(t_cal, 1e-6))))
-    t = run(ticks)
-    for eng, _ in units:
-        eng.close()
-    return ticks, t, cores
-
-
-def cpu_baseline(irs, x_host, budget_s):
-    """The fp32 oracle (CPU restatement of the reference algorithm) on the first
-    ticks of the same workload, one instance per channel, the channels spread
-    over the host's cores (threads; the oracle runs outside the GIL) as the
-    reference spreads its work over pthreads; ticks chosen for ~budget_s."""
+def cpu_free_running(irs, x_host, budget_s):
+    """The fp32 oracle on the same workload with no per-tick synchronisation:
+    one instance per channel, the channels spread over the host's cores
+    (threads; the oracle runs outside the GIL), every level inline at its start
+    tick; ticks chosen for ~budget_s.  An upper bound on CPU throughput, not a
+    real-time schedule."""
     import threading
     from oracle import oracle as O
     cores = max(1, min(CHANNELS, len(os.sched_getaffinity(0))))
@@ -188,9 +235,9 @@ def cpu_baseline(irs, x_host, budget_s):
     t = run(ticks)
     for m in ms:
         m.close()
-    return dict(value=CHANNELS * ticks * N0 / t, unit="channel-samples/s", cores=cores, kind="port",
-                sample=f"{ticks} ticks x {CHANNELS} channels ({ticks * N0 / FS:.2f} s of audio) of the same "
-                       f"workload through the fp32 CPU restatement, one instance per channel on {cores} threads")
+    return dict(value=CHANNELS * ticks * N0 / t, unit="channel-samples/s", cores=cores,
+                sample=f"{ticks} ticks x {CHANNELS} channels ({ticks * N0 / FS:.2f} s of audio), one instance per "
+                       f"channel on {cores} threads, no per-tick synchronisation")
 
 
 def bench_uneven(args):
@@ -258,12 +305,9 @@ def bench_uneven(args):
         # engines, the same IRs, partitions and input columns), over the host's cores
         from oracle import oracle as O
         pick = np.random.default_rng(33).choice(n, size=min(64, n), replace=False)
-        units = [(O.OracleMulti(FS, n0, 1, 1, [dict(N=insts[i]["N"], P=insts[i]["P"])], [irs[i]], prec="f32"),
-                  np.ascontiguousarray(x[:, i:i + 1])) for i in pick]
-        ticks, t, cores = oracle_threads(units, args.cpu_budget, 256)
-        cpu = {"value": len(units) * ticks * n0 / t, "unit": "channel-samples/s", "cores": cores, "kind": "port",
-               "sample": f"{ticks} ticks x {len(units)} of the {n} instances (seeded random pick) through the "
-                         f"fp32 CPU restatement, one mono engine per instance on {cores} threads"}
+        cpu = cpu_reference_schedule([(insts[i]["N"], insts[i]["P"], irs[i]) for i in pick], None, len(pick), n0,
+                                     args.cpu_budget, len(pick),
+                                     f"{len(pick)} of the {n} uneven instances (seeded random pick)")
     print(json.dumps({
         "metric": "channel-samples/s at p99 block latency < block period (uneven batched instances)",
         "value": value, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -277,12 +321,16 @@ def bench_uneven(args):
                        "persistent_ticks": int(st.persistent_ticks)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err,
-        "roofline": with_traffic(kernel_roofline(avg_us, launches, bytes_per,
-                                                 f"k_partial (largest level, "
-                                                 f"{int(parts and max(max(N) for N, _ in parts))}, "
-                                                 "delay-line sum of every instance that has it)"),
-                                 "k_partial_cfg3_512_traffic.json"),
-        "hbm_roofline": hbm_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0),
+        "roofline": workload_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0,
+                                      "whole timed region: k_persist<7> resident for every tick (level 0 and, in "
+                                      "self mode, the 256-1024 levels in each tick workgroup) + the longer "
+                                      "levels' kernels beside it",
+                                      "serial host round trip per tick and the tick workgroups' level-0 latency"),
+        "kernel_rooflines": {"k_partial": with_traffic(kernel_roofline(
+            avg_us, launches, bytes_per, f"k_partial (largest level, "
+                                         f"{int(parts and max(max(N) for N, _ in parts))}, "
+                                         "delay-line sum of every instance that has it)"),
+            "k_partial_cfg3_512_traffic.json")},
         "cpu_baseline": cpu}), flush=True)
 
 
@@ -413,14 +461,12 @@ def bench_mimo(args):
     if not args.no_cpu:
         # the fp32 oracle as the reference formulates MIMO (pair instances summed
         # per output): 16 of the 32 outputs, each an engine of its 32 pairs
-        from oracle import oracle as O
-        xh = np.ascontiguousarray(x.astype(np.float32))
-        units = [(O.OracleMulti(FS, n0, n, 1, [dict(N=[256, 4096], P=[30, 0], in_ch=i, out_ch=0) for i in range(n)],
-                                [irs[oo][i] for i in range(n)], prec="f32"), xh) for oo in range(16)]
-        ticks, t, cores = oracle_threads(units, args.cpu_budget, 16)
-        cpu = {"value": len(units) * ticks * n0 / t, "unit": "channel-samples/s", "cores": cores, "kind": "port",
-               "sample": f"{ticks} ticks x {len(units)} of the {n} outputs (32 pair instances each) through the "
-                         f"fp32 CPU restatement, one engine per output on {cores} threads"}
+        no = 8
+        specs = [([256, 4096], [30, 0], irs[oo][i]) for oo in range(no) for i in range(n)]
+        cpu = cpu_reference_schedule(specs, [i for oo in range(no) for i in range(n)], n, n0, args.cpu_budget, no,
+                                     f"{no} of the {n} outputs as the reference formulates MIMO: {no * n} pair "
+                                     f"instances (input i -> output o), outputs per instance (the per-output sum "
+                                     f"of {n} frames is not timed)")
     print(json.dumps({
         "metric": "channel-samples/s at p99 block latency < block period (MIMO 32x32)",
         "value": n * args.steps * tps * n0 / rsecs, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps,
@@ -435,10 +481,15 @@ def bench_mimo(args):
         "p99_below_period": bool(rst.p99_us < rst.period_us),
         "max_rel_err_vs_fp64": err_rt,
         "host_buffers_pcie_inclusive": host_rt,
-        "roofline": kernel_roofline(r_avg_us, r_launches, r_bytes,
-                                    "k_mxr_part (4096 level: the next block's delay-line sum, 32x32 pairs x 14 "
-                                    "partitions, one workgroup per bin)"),
-        "hbm_roofline": hbm_roofline(tick_bytes(parts, n0), args.steps * tps, rsecs, n * n0),
+        "roofline": workload_roofline(mimo_engine_tick_bytes([(256, 30), (4096, 15)], n, n, n0), args.steps * tps,
+                                      rsecs, n * n0,
+                                      "whole timed region of the real-time MIMO engine (its own bytes): "
+                                      "k_mxr_persist<8> resident for every tick + the 4096 level's kernels",
+                                      "level 0's per-tick critical path (host round trip, input transforms, "
+                                      "counter hand-offs) beside the filter-spectrum stream"),
+        "kernel_rooflines": {"k_mxr_part": kernel_roofline(
+            r_avg_us, r_launches, r_bytes, "k_mxr_part (4096 level: the next block's delay-line sum, 32x32 pairs x "
+                                           "14 partitions, one workgroup per bin), beside the persistent kernel")},
         "pair_instances": {"value": n * args.steps * tps * n0 / secs, "p50_us": st.p50_us, "p99_us": st.p99_us,
                            "misses": int(st.deadline_misses), "persistent_ticks": int(st.persistent_ticks),
                            "max_rel_err_vs_fp64": err_pairs,
@@ -700,12 +751,6 @@ def main():
     mb.probe(-1)
     mb.cleanup()
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_partial_cfg2_16ch_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            t = json.load(fh)
-        traffic = 1024.0 * (t["fetch_kib_per_dispatch_scaled"] + t["write_kib_per_dispatch_scaled"])
     out = {
         "metric": "channel-samples/s at p99 block latency < block period; batched vs plain, same GPU",
         "value": value,
@@ -730,14 +775,17 @@ def main():
                                     "misses": int(lst.deadline_misses)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err,
-        "roofline": {"bound": "hbm", "achieved": bytes_per / (avg_us * 1e-6) / 1e9 if avg_us else None,
-                     "peak": 8000.0, "unit": "GB/s",
-                     "frac": (bytes_per / (avg_us * 1e-6) / 1e9) / 8000.0 if avg_us else None,
-                     "traffic": traffic, "kernel": "k_partial (16384-level delay-line sum over 14 partitions, 16 jobs)",
-                     "avg_us": avg_us, "launches": launches, "alg_bytes_per_launch": bytes_per},
-        "tick_kernel": {"avg_us": tick_us, "alg_bytes_per_launch": tick_b,
-                        "achieved_GBps": tick_b / (tick_us * 1e-6) / 1e9 if tick_us else None},
-        "hbm_roofline": hbm_roofline(tick_bytes(parts, N0), ticks, secs, CHANNELS * N0),
+        "roofline": workload_roofline(tick_bytes(parts, N0), ticks, secs, CHANNELS * N0,
+                                      "whole timed region: k_persist<6> resident for every tick (level 0 and the "
+                                      "256 level inside it) + the 2048 and 16384 levels' kernels beside it",
+                                      "serial host round trip per tick (release -> completion flags): one tick "
+                                      "moves about 0.9 MB"),
+        "kernel_rooflines": {
+            "k_partial": with_traffic(kernel_roofline(avg_us, launches, bytes_per,
+                                                      "k_partial (16384-level delay-line sum over 14 partitions, "
+                                                      "16 jobs)"), "k_partial_cfg2_16ch_traffic.json"),
+            "tick_kernel": kernel_roofline(tick_us, 2, tick_b, "k_tick<6> (one launch per tick, level 0 of 16 "
+                                                               "instances; the persistent kernel's tick work)")},
     }
 
     # ---- host buffers (PCIe-inclusive: frames and output in pinned host memory),
@@ -773,7 +821,11 @@ def main():
         out["batched_over_plain"] = value / pval
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(irs, x_host, args.cpu_budget)
+        cpu = cpu_reference_schedule([(PART_N, PART_P, irs[c]) for c in range(CHANNELS)], None, CHANNELS, N0,
+                                     args.cpu_budget, CHANNELS, "the same 16-channel workload (same IRs and "
+                                                                "partition), strict run()")
+        cpu["free_running"] = cpu_free_running(irs, x_host, min(3.0, args.cpu_budget))
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -62,6 +62,8 @@ def lib(prec="f32"):
         L.pco_last_error.restype = ctypes.c_char_p
         L.pco_set_mutation.argtypes = [ctypes.c_int]
         L.pco_now.restype = ctypes.c_double
+        L.pco_rt_run.argtypes = [vp, ctypes.c_int, _c_int_p, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int,
+                                 vp, vp, vp]
         _libs[prec] = L
     return _libs[prec]
 
@@ -153,6 +155,33 @@ class OracleFilter:
             self.close()
         except Exception:
             pass
+
+
+def rt_run(filters, x, threads_per_level, in_ch=None):
+    """The reference's threaded schedule over OracleFilter instances with one
+    buffer size (pco_rt_run): instance i reads column in_ch[i] (default i) of
+    x [nticks*N0, nIn] and writes column i of the result.  Returns (y, per-tick
+    seconds, [(block size, mean seconds of one runLevel of one instance)] per
+    global level).  Timing leg of the CPU baseline; the output equals the
+    single-threaded restatement's."""
+    L = filters[0].L
+    n, N0 = len(filters), filters[0].N[0]
+    dt = _dt(filters[0].prec)
+    x = np.ascontiguousarray(x, dtype=dt)
+    if x.ndim == 1:
+        x = x[:, None]
+    nticks = x.shape[0] // N0
+    y = np.zeros((nticks * N0, n), dtype=dt)
+    lat = np.zeros(max(1, nticks))
+    gN = np.zeros(24, dtype=np.int32)
+    lvl = np.zeros(24)
+    hs = (ctypes.c_void_p * n)(*[f.h for f in filters])
+    ch = None if in_ch is None else _ints(in_ch)
+    G = L.pco_rt_run(ctypes.cast(hs, ctypes.c_void_p), n, ch, x.shape[1], _ptr(x), _ptr(y), nticks,
+                     int(threads_per_level), _ptr(lat), _ptr(gN), _ptr(lvl))
+    if G < 0:
+        raise RuntimeError("pco_rt_run: bad instance set (one buffer size, input columns in range)")
+    return y, lat[:nticks], [(int(gN[g]), float(lvl[g])) for g in range(G)]
 
 
 class OracleMulti:

@@ -778,3 +778,198 @@ PCO_EXPORT double pco_now(void)
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Threaded real-time runner: the reference's scheduling, for the CPU        */
+/* baseline of the benchmark (timing only; its output is the same as the     */
+/* single-threaded restatement's, which a test checks bitwise).              */
+/*   PartConvMulti::run (src/partconvMulti.cpp:815-928): per tick the frame  */
+/*   goes into every instance (sync_levels, input), the levels whose period  */
+/*   starts now are handed to their workers, level 0 of every instance runs  */
+/*   on the calling thread, the call waits for the levels whose block is     */
+/*   read for the first time now, then every instance's output is formed.    */
+/*   workerThreadEntry (src/partconvMulti.cpp:930-1013): one group of worker */
+/*   threads per global level (a block size used by any instance, mapped to  */
+/*   each instance's own level as level_map does, :139-244), each thread     */
+/*   running runLevel for its share of the instances when the level starts,  */
+/*   then signalling completion.                                             */
+/* ------------------------------------------------------------------------ */
+#include <pthread.h>
+
+#define RT_MAX_G 24
+
+typedef struct rt_group {
+    int k;               /* period in ticks (block size / N0) */
+    int nth;
+    int *inst, *lvl;     /* the instances with this block size and their local level */
+    int nj;
+    pthread_t *th;
+    pthread_mutex_t mu;
+    pthread_cond_t go_cv, done_cv;
+    unsigned long gen;   /* starts issued */
+    int pending;         /* threads still running the current start */
+    int stop;
+    double busy_s;       /* summed runLevel time of all its threads */
+    long long blocks;    /* runLevel calls */
+} rt_group;
+
+typedef struct rt_arg {
+    pco_filter **f;
+    rt_group *g;
+    int t;
+} rt_arg;
+
+static void *rt_worker(void *p)
+{
+    rt_arg *a = (rt_arg *)p;
+    rt_group *g = a->g;
+    unsigned long seen = 0;
+    for (;;) {
+        pthread_mutex_lock(&g->mu);
+        while (g->gen == seen && !g->stop) pthread_cond_wait(&g->go_cv, &g->mu);
+        if (g->stop) {
+            pthread_mutex_unlock(&g->mu);
+            return NULL;
+        }
+        seen = g->gen;
+        pthread_mutex_unlock(&g->mu);
+        const double t0 = pco_now();
+        long long nb = 0;
+        for (int j = a->t; j < g->nj; j += g->nth) {
+            pco_filter_run_level(a->f[g->inst[j]], g->lvl[j]);
+            nb++;
+        }
+        const double dt = pco_now() - t0;
+        pthread_mutex_lock(&g->mu);
+        g->busy_s += dt;
+        g->blocks += nb;
+        if (--g->pending == 0) pthread_cond_broadcast(&g->done_cv);
+        pthread_mutex_unlock(&g->mu);
+    }
+}
+
+/* Run nticks ticks of n strict instances (one buffer size N0).  Instance i
+ * reads column in_ch[i] of x [nticks * N0][nIn] (in_ch may be NULL: column i)
+ * and writes column i of y [nticks * N0][n].  Every global level above the
+ * buffer size gets threads_per_level worker threads.  lat_s[t]: seconds from
+ * the start of tick t to its output being formed (ticks back to back).
+ * gN[g], level_s[g] (room for RT_MAX_G): block size of global level g and the
+ * mean seconds of one of its runLevel calls (one instance's block); returns
+ * the number of global levels, or -1 on error. */
+PCO_EXPORT int pco_rt_run(pco_filter **f, int n, const int *in_ch, int nIn, const real *x, real *y, int nticks,
+                          int threads_per_level, double *lat_s, int *gN, double *level_s)
+{
+    if (n < 1 || nticks < 0 || threads_per_level < 1 || nIn < 1) return -1;
+    const int N0 = f[0]->N[0];
+    int sizes[RT_MAX_G], G = 0;
+    for (int i = 0; i < n; i++) {
+        if (f[i]->N[0] != N0 || (in_ch && (in_ch[i] < 0 || in_ch[i] >= nIn)) || (!in_ch && i >= nIn)) return -1;
+        for (int L = 1; L < f[i]->M; L++) {
+            int q = 0;
+            while (q < G && sizes[q] != f[i]->N[L]) q++;
+            if (q == G) {
+                if (G == RT_MAX_G) return -1;
+                sizes[G++] = f[i]->N[L];
+            }
+        }
+    }
+    for (int a = 1; a < G; a++)  /* ascending block size */
+        for (int b = a; b > 0 && sizes[b] < sizes[b - 1]; b--) {
+            const int t = sizes[b];
+            sizes[b] = sizes[b - 1];
+            sizes[b - 1] = t;
+        }
+    rt_group *grp = (rt_group *)calloc((size_t)(G > 0 ? G : 1), sizeof(rt_group));
+    rt_arg *args = (rt_arg *)calloc((size_t)(G > 0 ? G : 1) * threads_per_level, sizeof(rt_arg));
+    for (int g = 0; g < G; g++) {
+        rt_group *q = &grp[g];
+        q->k = sizes[g] / N0;
+        q->inst = (int *)malloc(sizeof(int) * n);
+        q->lvl = (int *)malloc(sizeof(int) * n);
+        for (int i = 0; i < n; i++)
+            for (int L = 1; L < f[i]->M; L++)
+                if (f[i]->N[L] == sizes[g]) {
+                    q->inst[q->nj] = i;
+                    q->lvl[q->nj] = L;
+                    q->nj++;
+                }
+        q->nth = threads_per_level < q->nj ? threads_per_level : q->nj;
+        q->th = (pthread_t *)calloc((size_t)q->nth, sizeof(pthread_t));
+        pthread_mutex_init(&q->mu, NULL);
+        pthread_cond_init(&q->go_cv, NULL);
+        pthread_cond_init(&q->done_cv, NULL);
+        for (int t = 0; t < q->nth; t++) {
+            rt_arg *a = &args[(size_t)g * threads_per_level + t];
+            a->f = f;
+            a->g = q;
+            a->t = t;
+            pthread_create(&q->th[t], NULL, rt_worker, a);
+        }
+    }
+    real *fr = (real *)malloc(sizeof(real) * N0);
+    double busy0 = 0.0;
+    for (int t = 0; t < nticks; t++) {
+        const double t0 = pco_now();
+        const real *xt = x + (size_t)t * N0 * nIn;
+        real *yt = y + (size_t)t * N0 * n;
+        for (int i = 0; i < n; i++) {
+            const int c = in_ch ? in_ch[i] : i;
+            pco_filter_sync_levels(f[i]);
+            for (int j = 0; j < N0; j++) fr[j] = xt[(size_t)j * nIn + c];
+            pco_filter_input(f[i], fr);
+        }
+        /* each instance's counters advance one tick at a time from the same
+         * reset state, so level starts and ends follow from the tick number:
+         * a level of period k starts at tick f = 0 mod k and its block is
+         * first read at tick f + 1 = 0 mod k */
+        const long long tick = (long long)f[0]->bcurr[0];
+        for (int g = 0; g < G; g++) {
+            rt_group *q = &grp[g];
+            if (tick % q->k != 0) continue;
+            pthread_mutex_lock(&q->mu);
+            q->pending = q->nth;
+            q->gen++;
+            pthread_cond_broadcast(&q->go_cv);
+            pthread_mutex_unlock(&q->mu);
+        }
+        const double tb = pco_now();
+        for (int i = 0; i < n; i++) pco_filter_run_level(f[i], 0);
+        busy0 += pco_now() - tb;
+        for (int g = 0; g < G; g++) {
+            rt_group *q = &grp[g];
+            if ((tick + 1) % q->k != 0) continue;
+            pthread_mutex_lock(&q->mu);
+            while (q->pending > 0) pthread_cond_wait(&q->done_cv, &q->mu);
+            pthread_mutex_unlock(&q->mu);
+        }
+        for (int i = 0; i < n; i++) {
+            pco_filter_output(f[i], fr);
+            for (int j = 0; j < N0; j++) yt[(size_t)j * n + i] = fr[j];
+        }
+        if (lat_s) lat_s[t] = pco_now() - t0;
+    }
+    /* every started block finished before the workers stop */
+    for (int g = 0; g < G; g++) {
+        rt_group *q = &grp[g];
+        pthread_mutex_lock(&q->mu);
+        while (q->pending > 0) pthread_cond_wait(&q->done_cv, &q->mu);
+        q->stop = 1;
+        pthread_cond_broadcast(&q->go_cv);
+        pthread_mutex_unlock(&q->mu);
+        for (int t = 0; t < q->nth; t++) pthread_join(q->th[t], NULL);
+        if (gN) gN[g + 1] = sizes[g];
+        if (level_s) level_s[g + 1] = q->blocks ? q->busy_s / (double)q->blocks : 0.0;
+        pthread_mutex_destroy(&q->mu);
+        pthread_cond_destroy(&q->go_cv);
+        pthread_cond_destroy(&q->done_cv);
+        free(q->th);
+        free(q->inst);
+        free(q->lvl);
+    }
+    if (gN) gN[0] = N0;
+    if (level_s) level_s[0] = nticks ? busy0 / ((double)nticks * n) : 0.0;
+    free(fr);
+    free(args);
+    free(grp);
+    return G + 1;
+}

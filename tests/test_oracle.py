@@ -272,3 +272,27 @@ def test_planted_mistakes_are_caught(mutation):
     finally:
         O.set_mutation(0, "f64")
     assert _pins_pass()
+
+
+def test_threaded_reference_schedule_matches_restatement():
+    """The CPU baseline's threaded runner (the reference's run/worker
+    scheduling, src/partconvMulti.cpp:815-1013: level 0 inline, one worker
+    group per global block size, a wait at each level's deadline tick) gives
+    bitwise the single-threaded restatement's output, for instances with
+    different partitions and an input-column map, over several calls."""
+    from oracle import oracle as O
+    from partconv_amd.workloads import decaying_noise_ir
+    insts = [dict(N=[64, 256, 2048], P=[6, 14, 0], in_ch=1), dict(N=[64, 512], P=[14, 0], in_ch=0),
+             dict(N=[64, 256, 1024, 4096], P=[6, 6, 6, 0], in_ch=1)]
+    irs = [decaying_noise_ir(t, 0.3, 48000, 40 + i) for i, t in enumerate((9000, 4000, 20000))]
+    x = np.random.default_rng(8).standard_normal((400 * 64, 2)).astype(np.float32)
+    ref = O.OracleMulti(48000, 64, 2, 3, [dict(out_ch=i, **d) for i, d in enumerate(insts)], irs).process(x)
+    fs = [O.OracleFilter(d["N"], d["P"], h) for d, h in zip(insts, irs)]
+    chans = [d["in_ch"] for d in insts]
+    ys = []
+    for a, b in ((0, 150), (150, 400)):        # the stream continues across calls
+        y, lat, lv = O.rt_run(fs, x[a * 64:b * 64], 2, in_ch=chans)
+        ys.append(y)
+        assert lat.shape == (b - a,) and np.all(lat > 0)
+    assert [n for n, _ in lv] == [64, 256, 512, 1024, 2048, 4096]
+    assert np.array_equal(np.concatenate(ys), ref)
