@@ -259,6 +259,13 @@ long long pcm_trace_read(pc_multi *h, unsigned long long *out, long long cap);
  * Output is identical in all modes.  pcm_workers_info returns the number of
  * worker workgroups and sets *self_mode (may be null). */
 int pcm_set_persistent_workers(pc_multi *h, int mode);
+
+/* Width of the level delay-line-sum kernel (tuning switch, no reference
+ * counterpart): 2, 4 or 8 frequency bins per thread (default 2).  Output is
+ * identical for every width.  pcm_get_tuning reads the current workers mode
+ * and width (the autotuner's knobs, partconv_amd/autotune.py). */
+int pcm_set_partial_width(pc_multi *h, int bins_per_thread);
+int pcm_get_tuning(const pc_multi *h, int *workers_mode, int *bins_per_thread);
 int pcm_workers_info(pc_multi *h, int *self_mode);
 
 /* Deadline-handling test hook (no reference counterpart): the next block of
@@ -378,9 +385,14 @@ long long pcc_recomputed_levels(const pc_conv *c);
  * and of the dL/dh partial sums by timing candidates with HIP events on the
  * caller's clips d_x [C][T].  Setup-time: allocates and synchronises.  The
  * forward output does not depend on the choice; dL/dh sums are grouped by it
- * (deterministic for a fixed choice).  pcc_level_tuning reads the choice. */
+ * (deterministic for a fixed choice).  Candidates within 5% of the fastest
+ * count as equal, and then the default (32, 64) wins, else the first in
+ * candidate order.  pcc_level_tuning reads the choice. */
 int pcc_tune(pc_conv *c, const float *d_x, int C, int T, void *stream);
 int pcc_level_tuning(const pc_conv *c, int level, int *mac_mc, int *dh_mc);
+/* Apply a chunk-size choice without timing (a cached pcc_tune result):
+ * powers of two in [1, 1024]. */
+int pcc_set_level_tuning(pc_conv *c, int level, int mac_mc, int dh_mc);
 /* Measurement hook, as pcm_probe (no reference counterpart): time with HIP
  * events stamped by the launch itself the forward spectral product (k_off_mac)
  * of level `level` each time it launches (-1 disables); read back the mean

@@ -79,6 +79,8 @@ SIGNATURES = {
     "pcm_workers_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "pcm_release_info": (ctypes.c_int, [_vp]),
     "pcm_stall_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
+    "pcm_set_partial_width": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),
     "pcf_create": (_vp, [ctypes.c_int, ctypes.c_int]),
@@ -109,6 +111,7 @@ SIGNATURES = {
     "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),
     "pcc_tune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_level_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "pcc_set_level_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pcc_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcc_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_double)]),

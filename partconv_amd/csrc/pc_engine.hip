@@ -50,6 +50,7 @@
 #define PC_WG_FLAGS_MAX 1024        // direct mix: per-workgroup host flags up to this many instances
                                     // (the host polls them; beyond, device tickets raise one flag)
 #define PC_DH_PT 8                  // dL/dh: partitions per thread (register window)
+#define PC_TUNE_TIE 0.05            // tuners: candidates within 5% of the fastest are equal
 
 static thread_local std::string g_err;
 
@@ -620,6 +621,7 @@ struct Engine {
     int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
     double pend_t0 = -1;              // submit time of a tick a stalled session left unserved (-1: none)
     int ncu = 0;                      // compute units of the device
+    int part_q = 1;                   // k_partial float4 groups per thread (pcm_set_partial_width)
     Residency resid;                  // residency stamps of plain persistent launches
     // held level (pcm_stall_level): the next block of global level stall_g
     // waits behind a host function that sleeps stall_us on the level's stream
@@ -779,13 +781,18 @@ static void launch_big(BigShape sh, const LevelJob *jobs, int njobs, int N0, uin
                            ypar, tw);
 }
 
-// Partial sum S for block b of every job of a batch (k_partial).
+// Partial sum S for block b of every job of a batch (k_partial<Q>, Q float4
+// groups per thread; the result does not depend on Q).
 static void launch_partial(int N, const LevelJob *jobs, int njobs, uint64_t b, hipStream_t s,
-                           const LaunchExtra &x = LaunchExtra())
+                           const LaunchExtra &x = LaunchExtra(), int Q = 1)
 {
-    const int wpj = (N / 2 + PART_NT - 1) / PART_NT;
-    hipExtLaunchKernelGGL(k_partial, dim3(wpj * njobs), dim3(PART_NT), 0, s, x.ev0, x.ev1, 0, jobs, N, wpj,
-                          b);
+    const int wpj = (N / 2 + PART_NT * Q - 1) / (PART_NT * Q);
+    if (Q == 4)
+        hipExtLaunchKernelGGL(k_partial<4>, dim3(wpj * njobs), dim3(PART_NT), 0, s, x.ev0, x.ev1, 0, jobs, N, wpj, b);
+    else if (Q == 2)
+        hipExtLaunchKernelGGL(k_partial<2>, dim3(wpj * njobs), dim3(PART_NT), 0, s, x.ev0, x.ev1, 0, jobs, N, wpj, b);
+    else
+        hipExtLaunchKernelGGL(k_partial<1>, dim3(wpj * njobs), dim3(PART_NT), 0, s, x.ev0, x.ev1, 0, jobs, N, wpj, b);
 }
 
 template <int LOGN>
@@ -1358,7 +1365,7 @@ static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
         if (rc) return rc;
     }
     // the next block's delay-line sum, off its critical path
-    launch_partial(b.N, b.d_jobs, b.njobs, m + 1, s, x);
+    launch_partial(b.N, b.d_jobs, b.njobs, m + 1, s, x, E.part_q);
     return PC_OK;
 }
 
@@ -2517,6 +2524,29 @@ PC_API int pcm_set_persistent_workers(pc_multi *h, int mode)
     return PC_OK;
 }
 
+PC_API int pcm_set_partial_width(pc_multi *h, int bins_per_thread)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    if (bins_per_thread != 2 && bins_per_thread != 4 && bins_per_thread != 8)
+        return fail(PC_ERR_ARG, "bins per thread must be 2, 4 or 8");
+    int rc = session_stop(h->E);
+    if (rc) return rc;
+    if (h->E.ready) {  // level blocks in flight use the width they were launched with
+        rc = engine_drain(h->E);
+        if (rc) return rc;
+    }
+    h->E.part_q = bins_per_thread / 2;
+    return PC_OK;
+}
+
+PC_API int pcm_get_tuning(const pc_multi *h, int *workers_mode, int *bins_per_thread)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    if (workers_mode) *workers_mode = h->E.persist_workers;
+    if (bins_per_thread) *bins_per_thread = 2 * h->E.part_q;
+    return PC_OK;
+}
+
 PC_API int pcm_set_host_session(pc_multi *h, int on)
 {
     if (!h) return fail(PC_ERR_ARG, "null handle");
@@ -3435,7 +3465,8 @@ PC_API int pcc_tune(pc_conv *c, const float *d_x, int C, int T, void *stream)
         // products (n) or the level's dL/dh spectra (P N)
         if ((rc = conv_grow(&c->work, &c->work_cap, std::max(n, (size_t)L.P * L.N)))) break;
         if ((rc = off_rfft(c, L, W, C, c->dsum, s))) break;
-        auto time_it = [&](auto launch, double *best_ms, int *best, int cand) -> int {
+        // 3 timed launches per candidate after a warm-up launch
+        auto time_it = [&](auto launch, double *ms_out) -> int {
             float tot = 0.f;
             for (int r = 0; r < 4; r++) {
                 HIPCHK(hipEventRecord(e0, s));
@@ -3445,41 +3476,60 @@ PC_API int pcc_tune(pc_conv *c, const float *d_x, int C, int T, void *stream)
                 HIPCHK(hipEventSynchronize(e1));
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (r > 0) tot += ms;  // the first launch warms up
+                if (r > 0) tot += ms;
             }
-            if (tot < *best_ms) {
-                *best_ms = tot;
-                *best = cand;
-            }
+            *ms_out = tot;
             return PC_OK;
         };
+        // Deterministic choice: candidates within PC_TUNE_TIE of the fastest
+        // count as equal; among them the default wins, else the first in the
+        // candidate order.  Timing noise then moves the choice (and dL/dh's
+        // grouping) only when a candidate is clearly faster.
+        auto pick = [&](const int *cand, const double *ms, int n, int def) {
+            double best = 1e30;
+            for (int q = 0; q < n; q++) best = std::min(best, ms[q]);
+            for (int q = 0; q < n; q++)
+                if (cand[q] == def && ms[q] <= best * (1.0 + PC_TUNE_TIE)) return def;
+            for (int q = 0; q < n; q++)
+                if (ms[q] <= best * (1.0 + PC_TUNE_TIE)) return cand[q];
+            return def;
+        };
         if (L.P <= 16) {
-            double best_ms = 1e30;
-            int best = L.mac_mc;
-            for (int mc : mac_c) {
-                L.mac_mc = mc;
-                if ((rc = time_it([&]() { off_mac(c->dsum, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
-                                          return PC_OK; }, &best_ms, &best, mc)))
-                    break;
+            double ms[5];
+            const int def = 32;
+            for (int q = 0; q < 5 && !rc; q++) {
+                L.mac_mc = mac_c[q];
+                rc = time_it([&]() { off_mac(c->dsum, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
+                                     return PC_OK; }, &ms[q]);
             }
-            L.mac_mc = best;
+            L.mac_mc = rc ? def : pick(mac_c, ms, 5, def);
             if (rc) break;
         }
-        double best_ms = 1e30;
-        int best = L.dh_mc;
-        // the block spectra stand in for the gradient spectra (same shape)
-        for (int mc : dh_c) {
-            L.dh_mc = mc;
-            if ((rc = time_it([&]() { return off_dh(c, L, c->dsum, c->dsum, c->work, C, M, s); }, &best_ms, &best,
-                              mc)))
-                break;
+        {
+            double ms[5];
+            const int def = 64;
+            // the block spectra stand in for the gradient spectra (same shape)
+            for (int q = 0; q < 5 && !rc; q++) {
+                L.dh_mc = dh_c[q];
+                rc = time_it([&]() { return off_dh(c, L, c->dsum, c->dsum, c->work, C, M, s); }, &ms[q]);
+            }
+            L.dh_mc = rc ? def : pick(dh_c, ms, 5, def);
         }
-        L.dh_mc = best;
     }
     c->keep_valid = false;  // the work space held tuning data
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return rc;
+}
+
+PC_API int pcc_set_level_tuning(pc_conv *c, int level, int mac_mc, int dh_mc)
+{
+    if (!c || level < 0 || level >= (int)c->lv.size()) return fail(PC_ERR_ARG, "bad argument");
+    auto ok = [](int v) { return v >= 1 && v <= 1024 && (v & (v - 1)) == 0; };
+    if (!ok(mac_mc) || !ok(dh_mc)) return fail(PC_ERR_ARG, "chunk sizes must be powers of two in [1, 1024]");
+    c->lv[level].mac_mc = mac_mc;
+    c->lv[level].dh_mc = dh_mc;
+    return PC_OK;
 }
 
 PC_API int pcc_level_tuning(const pc_conv *c, int level, int *mac_mc, int *dh_mc)

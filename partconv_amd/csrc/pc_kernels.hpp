@@ -228,6 +228,28 @@ __device__ __forceinline__ cf pmul(cf a, cf b, bool bin0)
     return bin0 ? make_float2(a.x * b.x, a.y * b.y) : cmul(a, b);
 }
 
+// Delay-line sum step for two bins held as one float4 (bins 2k, 2k+1):
+// acc += a * h, with bin 0 of a row packing the real DC and Nyquist values
+// (bin0: componentwise products for the first pair).  Written as a fixed
+// sequence of fused multiply-adds -- re: fma(a.x, h.x, acc) then
+// fma(-a.y, h.y, .), im: fma(a.x, h.y, acc) then fma(a.y, h.x, .) -- so every
+// kernel that forms a delay-line sum (k_partial of any width, the persistent
+// kernel's level workers) rounds identically, whatever contraction the
+// compiler would choose for the plain expression in each of them.
+__device__ __forceinline__ float4 dl_mac2(float4 acc, float4 a, float4 h, bool bin0)
+{
+    if (bin0) {
+        acc.x = fmaf(a.x, h.x, acc.x);
+        acc.y = fmaf(a.y, h.y, acc.y);
+    } else {
+        acc.x = fmaf(-a.y, h.y, fmaf(a.x, h.x, acc.x));
+        acc.y = fmaf(a.y, h.x, fmaf(a.x, h.y, acc.y));
+    }
+    acc.z = fmaf(-a.w, h.w, fmaf(a.z, h.z, acc.z));
+    acc.w = fmaf(a.w, h.z, fmaf(a.z, h.w, acc.w));
+    return acc;
+}
+
 // Block output spectrum Y[k] = X[k] H_0[k] + S[k], where S (written by
 // k_partial after the previous block) holds sum_{p=1}^{P-1} X_{b-p}[k] H_p[k].
 // Stores X into FDL row b % P for later blocks.  X and Y alias.
@@ -1043,15 +1065,7 @@ __device__ __forceinline__ void wg_partial(const LevelJob &J, int N, uint64_t b,
             s = (s == 0) ? P - 1 : s - 1;
             const float4 a = fdl[(size_t)s * row + k2];
             const float4 h = H[(size_t)p * row + k2];
-            if (k2 == 0) {
-                acc.x += a.x * h.x;
-                acc.y += a.y * h.y;
-            } else {
-                acc.x += a.x * h.x - a.y * h.y;
-                acc.y += a.x * h.y + a.y * h.x;
-            }
-            acc.z += a.z * h.z - a.w * h.w;
-            acc.w += a.z * h.w + a.w * h.z;
+            acc = dl_mac2(acc, a, h, k2 == 0);
         }
         reinterpret_cast<float4 *>(J.S)[k2] = acc;
     }
@@ -1453,15 +1467,22 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
 // ---------------------------------------------------------------------------
 #define PART_NT 256
 
+// Q float4 groups (2 Q bins) per thread, Q in {1, 2, 4}: every bin is summed
+// over p = 1 .. P-1 in the same order whatever Q is, so the result does not
+// depend on it; a larger Q keeps more loads in flight per thread on fewer
+// workgroups (a tuning choice).
+template <int Q>
 __global__ void __launch_bounds__(PART_NT)
 k_partial(const LevelJob *__restrict__ jobs, int N, int wg_per_job, uint64_t b)
 {
     const int jb = blockIdx.x / wg_per_job, w = blockIdx.x - jb * wg_per_job;
     const LevelJob J = jobs[jb];
     const int P = J.P;
-    const int k2 = w * PART_NT + threadIdx.x;  // float4 index = bins 2*k2, 2*k2+1
-    if (2 * k2 >= N) return;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int base = w * PART_NT * Q + threadIdx.x;  // float4 index of group 0; group q at + q * PART_NT
+    if (2 * base >= N) return;
+    float4 acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     int s = (int)(b % (uint64_t)P);
     const float4 *fdl = reinterpret_cast<const float4 *>(J.fdl);
     const float4 *H = reinterpret_cast<const float4 *>(J.H);
@@ -1469,19 +1490,27 @@ k_partial(const LevelJob *__restrict__ jobs, int N, int wg_per_job, uint64_t b)
 #pragma unroll 4
     for (int p = 1; p < P; p++) {
         s = (s == 0) ? P - 1 : s - 1;
-        const float4 a = fdl[(size_t)s * row + k2];
-        const float4 h = H[(size_t)p * row + k2];
-        if (k2 == 0) {  // bin 0 packs the real DC and Nyquist bins
-            acc.x += a.x * h.x;
-            acc.y += a.y * h.y;
-        } else {
-            acc.x += a.x * h.x - a.y * h.y;
-            acc.y += a.x * h.y + a.y * h.x;
+        float4 a[Q], h[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int k2 = base + q * PART_NT;
+            if (2 * k2 < N) {
+                a[q] = fdl[(size_t)s * row + k2];
+                h[q] = H[(size_t)p * row + k2];
+            }
         }
-        acc.z += a.z * h.z - a.w * h.w;
-        acc.w += a.z * h.w + a.w * h.z;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int k2 = base + q * PART_NT;
+            if (2 * k2 >= N) continue;
+            acc[q] = dl_mac2(acc[q], a[q], h[q], k2 == 0);
+        }
     }
-    reinterpret_cast<float4 *>(J.S)[k2] = acc;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int k2 = base + q * PART_NT;
+        if (2 * k2 < N) reinterpret_cast<float4 *>(J.S)[k2] = acc[q];
+    }
 }
 
 // Plain-mode mix: out[c cs + j ss] = sum over unmuted instances feeding c, in
