@@ -299,6 +299,31 @@ def bench_uneven(args):
         r = irfft(rfft(x[:, c].astype(np.float64), L) * rfft(irs[c].astype(np.float64), L), L)[:x.shape[0]]
         lo = args.warmup * tps * n0
         err = max(err, float(np.max(np.abs(y[lo:, c] - r[lo:])) / np.sqrt(np.mean(r[lo:] ** 2))))
+    # the same ticks with the level blocks launched as per-level HIP graphs, and
+    # with one tick-kernel launch per tick (with and without the graphs)
+    variants = {}
+    vsteps = max(1, args.steps // 4)
+    for name, persistent, graphs in (("persistent_level_graphs", True, True),
+                                     ("launch_per_tick", False, False),
+                                     ("launch_per_tick_level_graphs", False, True)):
+        m.reset()
+        m.set_level_graphs(graphs)
+        if args.warmup and m.run_device(yd.data_ptr(), xd.data_ptr(), args.warmup * tps, True,
+                                        persistent=persistent) != 0:
+            raise RuntimeError(last_error())
+        m.clear_stats()
+        torch.cuda.synchronize()
+        tv = time.perf_counter()
+        if m.run_device(yd.data_ptr() + 4 * off, xd.data_ptr() + 4 * off, vsteps * tps, True,
+                        persistent=persistent) != 0:
+            raise RuntimeError(last_error())
+        torch.cuda.synchronize()
+        vsecs = time.perf_counter() - tv
+        vst = m.stats()
+        variants[name] = {"value": n * vsteps * tps * n0 / vsecs, "p50_us": vst.p50_us, "p99_us": vst.p99_us,
+                          "misses": int(vst.deadline_misses), "ticks": int(vst.ticks),
+                          "persistent_ticks": int(vst.persistent_ticks)}
+    m.set_level_graphs(False)
     m.cleanup()
     value = n * args.steps * tps * n0 / secs
     cpu = None
@@ -324,6 +349,9 @@ def bench_uneven(args):
                        "persistent_ticks": int(st.persistent_ticks)},
         "p99_below_period": bool(st.p99_us < st.period_us),
         "max_rel_err_vs_fp64": err,
+        "frame_variants": dict(variants, note="level blocks as per-level HIP graphs (arguments updated in "
+                                              "place, one graph launch per level start) and one tick-kernel "
+                                              "launch per tick, same engine and frames"),
         "roofline": workload_roofline(tick_bytes(parts, n0), args.steps * tps, secs, n * n0,
                                       "whole timed region: k_persist<7> resident for every tick (level 0 and, in "
                                       "self mode, the 256-1024 levels in each tick workgroup) + the longer "

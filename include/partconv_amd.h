@@ -266,6 +266,16 @@ int pcm_set_persistent_workers(pc_multi *h, int mode);
  * and width (the autotuner's knobs, partconv_amd/autotune.py). */
 int pcm_set_partial_width(pc_multi *h, int bins_per_thread);
 int pcm_get_tuning(const pc_multi *h, int *workers_mode, int *bins_per_thread);
+
+/* Level blocks launched as HIP graphs (no reference counterpart; on != 0):
+ * each global level's block kernels (every batch, plus the next block's
+ * delay-line sum) form one executable graph, built at first use; per block
+ * start the tick-dependent arguments are updated in place
+ * (hipGraphExecKernelNodeSetParams) and one hipGraphLaunch on the level's
+ * stream replaces the 2-4 kernel launches per batch.  Applies to per-tick and
+ * persistent runs (the levels launched from the host).  Output identical.
+ * A probed level (pcm_probe) is not timed while graphs are on. */
+int pcm_set_level_graphs(pc_multi *h, int on);
 int pcm_workers_info(pc_multi *h, int *self_mode);
 
 /* Deadline-handling test hook (no reference counterpart): the next block of

@@ -80,6 +80,7 @@ SIGNATURES = {
     "pcm_release_info": (ctypes.c_int, [_vp]),
     "pcm_stall_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pcm_set_partial_width": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_set_level_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pcm_trace_read": (ctypes.c_longlong, [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong]),
     "pcm_probe_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong), _dp]),

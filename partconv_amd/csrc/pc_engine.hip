@@ -541,6 +541,31 @@ struct LevelLauncher {
     std::string msg;
 };
 
+// A level's block launches as one executable HIP graph (pcm_set_level_graphs):
+// the kernel nodes of every batch of the level, chained in launch order; per
+// block start only the tick-dependent arguments (block index, FDL slot,
+// output half, next block's delay-line sum) change, through
+// hipGraphExecKernelNodeSetParams, and one hipGraphLaunch on the level's stream
+// replaces the 2-4 launches per batch.
+struct GArgs {  // argument storage of one kernel node (kernelParams point into it)
+    const LevelJob *jobs = nullptr;
+    BigShape sh{0, 0};
+    int N0 = 0, N = 0, wpj = 0, ypar = 0, mode = 0;
+    uint64_t m = 0, bcur = 0, b = 0;
+    const cf *tw = nullptr;
+    void *ptrs[8] = {};
+};
+struct GNode {
+    hipGraphNode_t node = nullptr;
+    hipKernelNodeParams p{};
+    std::unique_ptr<GArgs> a;
+};
+struct LevelGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    std::vector<GNode> nodes;
+};
+
 struct Engine {
     int dev = 0, verbosity = 0, relaxed = 0, mode = PC_BATCHED;
     int FS = 0, N0 = 0, nIn = 0, nOut = 0;
@@ -622,6 +647,8 @@ struct Engine {
     double pend_t0 = -1;              // submit time of a tick a stalled session left unserved (-1: none)
     int ncu = 0;                      // compute units of the device
     int part_q = 1;                   // k_partial float4 groups per thread (pcm_set_partial_width)
+    bool level_graphs = false;        // level blocks launched as graphs (pcm_set_level_graphs)
+    std::vector<LevelGraph> lgraph;   // per global level, built at first use
     Residency resid;                  // residency stamps of plain persistent launches
     // held level (pcm_stall_level): the next block of global level stall_g
     // waits behind a host function that sleeps stall_us on the level's stream
@@ -655,6 +682,15 @@ static void set_io_layout(const Engine &E, TickArgs &a, bool in_staging, bool ou
 static size_t wg_flag_words(const Engine &E)
 {
     return (size_t)std::max(E.nwg, 1) * (PC_WG_FLAG_STRIDE + 1);
+}
+
+static void level_graphs_free(Engine &E)
+{
+    for (LevelGraph &G : E.lgraph) {
+        if (G.x) hipGraphExecDestroy(G.x);
+        if (G.g) hipGraphDestroy(G.g);
+    }
+    E.lgraph.clear();
 }
 
 static void engine_free(Engine &E)
@@ -722,6 +758,7 @@ static void engine_free(Engine &E)
     if (E.ev_tick) hipEventDestroy(E.ev_tick);
     E.ev_tick = nullptr;
     resident_free(E.resid);
+    level_graphs_free(E);
     for (hipStream_t s : E.s_lvl) hipStreamDestroy(s);
     E.s_lvl.clear();
     if (E.s_main) hipStreamDestroy(E.s_main);
@@ -1372,12 +1409,147 @@ static int launch_batch(Engine &E, const Batch &b, uint64_t m, hipStream_t s)
 // Launch level g's blocks that start at tick f on its stream and record its event.
 static void host_sleep_fn(void *us) { std::this_thread::sleep_for(std::chrono::microseconds((uintptr_t)us)); }
 
+static const void *small_fn(int logn)
+{
+    switch (logn) {
+    case 4: return (const void *)k_level_small<4>;
+    case 5: return (const void *)k_level_small<5>;
+    case 6: return (const void *)k_level_small<6>;
+    case 7: return (const void *)k_level_small<7>;
+    case 8: return (const void *)k_level_small<8>;
+    case 9: return (const void *)k_level_small<9>;
+    case 10: return (const void *)k_level_small<10>;
+    case 11: return (const void *)k_level_small<11>;
+    default: return nullptr;
+    }
+}
+
+static int small_nt(int N) { return (N / 4 < 64) ? 64 : ((N / 4 > 512) ? 512 : N / 4); }  // SmallCfg::NT
+
+// Append one kernel node after `dep` (or none); its arguments live in *a.
+static int graph_node(LevelGraph &G, const void *fn, dim3 grid, int nt, size_t lds, std::unique_ptr<GArgs> a,
+                      int nargs, hipGraphNode_t *dep)
+{
+    GNode n;
+    n.a = std::move(a);
+    n.p.func = const_cast<void *>(fn);
+    n.p.gridDim = grid;
+    n.p.blockDim = dim3(nt);
+    n.p.sharedMemBytes = (unsigned)lds;
+    n.p.kernelParams = n.a->ptrs;
+    n.p.extra = nullptr;
+    (void)nargs;
+    HIPCHK(hipGraphAddKernelNode(&n.node, G.g, *dep ? dep : nullptr, *dep ? 1 : 0, &n.p));
+    *dep = n.node;
+    G.nodes.push_back(std::move(n));
+    return PC_OK;
+}
+
+// Build level g's graph: every batch's block kernels (run mode) and its
+// delay-line sum for the next block, chained in the order launch_batch issues
+// them on one stream.  Tick-dependent arguments start at 0 (set per launch).
+static int level_graph_build(Engine &E, int g)
+{
+    if ((int)E.lgraph.size() != (int)E.gsize.size()) E.lgraph.resize(E.gsize.size());
+    LevelGraph &G = E.lgraph[g];
+    HIPCHK(hipGraphCreate(&G.g, 0));
+    hipGraphNode_t dep = nullptr;
+    int rc = PC_OK;
+    for (Group &Gr : E.groups)
+        for (Batch &b : Gr.batches) {
+            if (b.g != g || rc) continue;
+            auto mk = [&]() {
+                std::unique_ptr<GArgs> a(new GArgs());
+                a->jobs = b.d_jobs;
+                a->sh = b.sh;
+                a->N0 = E.N0;
+                a->N = b.N;
+                a->tw = E.tw;
+                a->mode = PC_MODE_RUN;
+                return a;
+            };
+            if (b.big) {
+                const int A = 1 << b.sh.logA, B = 1 << b.sh.logB;
+                auto a1 = mk();
+                GArgs *r1 = a1.get();
+                void *p1[] = {&r1->jobs, &r1->sh, &r1->N0, &r1->m, &r1->tw, &r1->mode};
+                memcpy(r1->ptrs, p1, sizeof p1);
+                if ((rc = graph_node(G, (const void *)k_big1, dim3((A / BIG_G) * b.njobs), BIG_NT, big1_lds(b.sh),
+                                     std::move(a1), 6, &dep)))
+                    break;
+                auto a2 = mk();
+                GArgs *r2 = a2.get();
+                void *p2[] = {&r2->jobs, &r2->sh, &r2->bcur, &r2->tw, &r2->mode};
+                memcpy(r2->ptrs, p2, sizeof p2);
+                if ((rc = graph_node(G, (const void *)k_big2, dim3((B / 2) * b.njobs), BIG_NT, big2_lds(b.sh),
+                                     std::move(a2), 5, &dep)))
+                    break;
+                auto a3 = mk();
+                GArgs *r3 = a3.get();
+                void *p3[] = {&r3->jobs, &r3->sh, &r3->ypar, &r3->tw};
+                memcpy(r3->ptrs, p3, sizeof p3);
+                if ((rc = graph_node(G, (const void *)k_big3, dim3((A / BIG_G) * b.njobs), BIG_NT, big1_lds(b.sh),
+                                     std::move(a3), 4, &dep)))
+                    break;
+            } else {
+                const void *fn = small_fn(ilog2(b.N));
+                if (!fn) {
+                    rc = fail(PC_ERR_ARG, "unsupported small block size");
+                    break;
+                }
+                auto a0 = mk();
+                GArgs *r0 = a0.get();
+                void *p0[] = {&r0->jobs, &r0->N0, &r0->m, &r0->bcur, &r0->ypar, &r0->tw, &r0->mode};
+                memcpy(r0->ptrs, p0, sizeof p0);
+                if ((rc = graph_node(G, fn, dim3(b.njobs), small_nt(b.N), 0, std::move(a0), 7, &dep))) break;
+            }
+            const int Q = E.part_q;
+            auto ap = mk();
+            GArgs *rp = ap.get();
+            rp->wpj = (b.N / 2 + PART_NT * Q - 1) / (PART_NT * Q);
+            void *pp[] = {&rp->jobs, &rp->N, &rp->wpj, &rp->b};
+            memcpy(rp->ptrs, pp, sizeof pp);
+            const void *pf = Q == 4 ? (const void *)k_partial<4> : Q == 2 ? (const void *)k_partial<2>
+                                                                          : (const void *)k_partial<1>;
+            if ((rc = graph_node(G, pf, dim3(rp->wpj * b.njobs), PART_NT, 0, std::move(ap), 4, &dep))) break;
+        }
+    if (rc) return rc;
+    HIPCHK(hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0));
+    return PC_OK;
+}
+
+// Level g's blocks that start at tick f through its graph, on stream s.
+static int level_graph_launch(Engine &E, int g, uint64_t f, hipStream_t s)
+{
+    if ((int)E.lgraph.size() <= g || !E.lgraph[g].x) {
+        const int rc = level_graph_build(E, g);
+        if (rc) return rc;
+    }
+    LevelGraph &G = E.lgraph[g];
+    const uint64_t m = f >> E.gshift[g];
+    for (GNode &n : G.nodes) {
+        n.a->m = m;
+        n.a->bcur = m;
+        n.a->ypar = (int)(m & 1);
+        n.a->b = m + 1;
+        HIPCHK(hipGraphExecKernelNodeSetParams(G.x, n.node, &n.p));
+    }
+    HIPCHK(hipGraphLaunch(G.x, s));
+    return PC_OK;
+}
+
 static int launch_level(Engine &E, int g, uint64_t f)
 {
     hipStream_t s = level_stream(E, g);
     int want = g;
     if (E.stall_g.load() == g && E.stall_g.compare_exchange_strong(want, -1))
         HIPCHK(hipLaunchHostFunc(s, host_sleep_fn, (void *)(uintptr_t)E.stall_us.load()));
+    if (E.level_graphs) {
+        int rc = level_graph_launch(E, g, f, s);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(E.ev_lvl[g], s));
+        return PC_OK;
+    }
     const uint64_t m = f >> E.gshift[g];
     for (Group &Gr : E.groups)
         for (Batch &b : Gr.batches) {
@@ -2536,6 +2708,17 @@ PC_API int pcm_set_partial_width(pc_multi *h, int bins_per_thread)
         if (rc) return rc;
     }
     h->E.part_q = bins_per_thread / 2;
+    level_graphs_free(h->E);  // rebuilt with the new width at the next use
+    return PC_OK;
+}
+
+PC_API int pcm_set_level_graphs(pc_multi *h, int on)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    int rc = session_stop(h->E);
+    if (rc) return rc;
+    if (h->E.ready && (rc = engine_drain(h->E))) return rc;
+    h->E.level_graphs = on != 0;
     return PC_OK;
 }
 

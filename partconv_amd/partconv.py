@@ -406,6 +406,10 @@ class PartConvMulti:
         M = lib().pcm_instance_partition(self._h, int(i), N, P, 16)
         return list(N)[:M], list(P)[:M]
 
+    def set_level_graphs(self, on):
+        """Launch level blocks as per-level HIP graphs (pcm_set_level_graphs)."""
+        return _report(lib().pcm_set_level_graphs(self._h, int(bool(on))), "set_level_graphs")
+
     def stall_level(self, global_level, seconds):
         """Deadline-handling test hook (pcm_stall_level): the next block of a
         global level waits `seconds` behind a host function on its stream."""
