@@ -518,9 +518,10 @@ def bench_mimo(args):
                                       "k_mxr_persist<8> resident for every tick + the 4096 level's kernels",
                                       "level 0's per-tick critical path (host round trip, input transforms, "
                                       "counter hand-offs) beside the filter-spectrum stream"),
-        "kernel_rooflines": {"k_mxr_part": kernel_roofline(
+        "kernel_rooflines": {"k_mxr_part": with_traffic(kernel_roofline(
             r_avg_us, r_launches, r_bytes, "k_mxr_part (4096 level: the next block's delay-line sum, 32x32 pairs x "
-                                           "14 partitions, one workgroup per bin), beside the persistent kernel")},
+                                           "14 partitions, one workgroup per bin), beside the persistent kernel"),
+            "k_mxr_part_cfg4_traffic.json")},
         "pair_instances": {"value": n * args.steps * tps * n0 / secs, "p50_us": st.p50_us, "p99_us": st.p99_us,
                            "misses": int(st.deadline_misses), "persistent_ticks": int(st.persistent_ticks),
                            "max_rel_err_vs_fp64": err_pairs,
