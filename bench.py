@@ -105,6 +105,30 @@ def workload_roofline(bytes_per_tick, ticks, secs, ch_per_tick, what, limit):
             "traffic_note": "per-kernel HBM traffic is in kernel_rooflines (counter runs)"}
 
 
+def diff_step_bytes(C, T, Ns, Ps, dh_chunks):
+    """Algorithmic HBM bytes of one forward + dL/dh step of the differentiable
+    form: every kernel of the step with each of its inputs read once and each
+    output written once (signal samples 4 B, spectra 8 B per bin).  Per level:
+    the filter spectra from h (h changes every step), the forward transform of
+    the input windows, the spectral product, the inverse transform into y (read
+    and written from the second level on), the gradient transform, the dL/dh
+    partial sums (G and X read, partials written), their reduction and the
+    inverse of the P filter-gradient spectra; plus the loss (y and the target
+    read, the gradient written).  Returns (total, {kernel: bytes})."""
+    sig = 4.0 * C * T
+    parts = {"loss": 3.0 * sig}
+    for li, (N, P) in enumerate(zip(Ns, Ps)):
+        M = -(-T // N)
+        spec, hsp = 8.0 * C * M * N, 8.0 * P * N
+        part = 8.0 * P * C * (-(-M // dh_chunks[li])) * N
+        for k, v in (("filter_transform", 4.0 * P * N + hsp), ("forward_transform", sig + spec),
+                     ("product", 2.0 * spec + hsp), ("inverse_transform", spec + (sig if li == 0 else 2.0 * sig)),
+                     ("gradient_transform", sig + spec), ("dh_partial", 2.0 * spec + part),
+                     ("dh_reduce", part + hsp), ("dh_inverse", hsp + 4.0 * P * N)):
+            parts[k] = parts.get(k, 0.0) + v
+    return sum(parts.values()), parts
+
+
 def mimo_engine_tick_bytes(levels, n_in, n_out, n0):
     """Bytes the real-time MIMO engine moves per tick (its shared-input
     formulation, not the pair instances'): per level block of N bins and P
@@ -568,14 +592,22 @@ def bench_diff(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    conv.probe(0)  # the forward spectral product of the 256 level, timed on its stream
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
-    p_avg_us, p_launches, p_bytes = conv.probe_read()
-    conv.probe(-1)
+    # each kernel of each level, timed by HIP events its own launch stamps,
+    # over a few more steps
+    probes = {}
+    for li, nl in enumerate(conv.N):
+        for kind in ("forward_transform", "product", "inverse_transform", "dh_partial"):
+            conv.probe(li, kind)
+            for _ in range(4):
+                step()
+            torch.cuda.synchronize()
+            probes[(kind, nl)] = conv.probe_read()
+            conv.probe(-1)
     # the same step captured once in a HIP graph and replayed (the MSE gradient
     # formed elementwise, 2 (y - tgt) / numel: no reduction inside the graph)
     scale = 2.0 / (C * T)
@@ -632,22 +664,40 @@ def bench_diff(args):
     with torch.no_grad():
         yc = conv(x, h0)[3].cpu().numpy()
     err = float(np.max(np.abs(yc - r)) / np.sqrt(np.mean(r ** 2)))
-    # per-kernel roofline: the forward spectral product of the 256 level
-    # (64 clips x 512 blocks, 14 partitions), its unique bytes over its
-    # HIP-event duration
-    roof = kernel_roofline(p_avg_us, p_launches, p_bytes,
-                           "k_off_mac (256 level, forward: Y_m = sum_p X_{m-p} H_p for 64 clips x 512 blocks, "
-                           "14 partitions)")
-    roof = with_traffic(roof, "k_off_mac_cfg5_traffic.json")
-    # whole step in the canonical per-block accounting (the partition spectra
-    # are shared by the 64 clips and served from cache, so this can exceed
-    # the HBM peak): forward, dL/dx and dL/dh each stream them once per block
+    # roofline of the timed region: the algorithmic bytes of every kernel of
+    # the step (each input read once, each output written once) over the
+    # measured step time; per-kernel figures from the probes beside it
+    step_b, step_parts = diff_step_bytes(C, T, conv.N, conv.P, [t[1] for t in tuning])
+    gbps = step_b * args.steps / secs / 1e9
+    roof = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBPS,
+            "traffic": None, "kernel": "whole timed step: forward (3 levels) + MSE loss + dL/dh, every kernel's "
+                                    "algorithmic bytes",
+            "alg_bytes_per_step": step_b, "alg_bytes_by_kernel": step_parts,
+            "traffic_note": "per-kernel HBM traffic is in kernel_rooflines (counter runs)"}
+    step_traffic = os.path.join(ROOT, "profiles", "kernels_cfg5_step_traffic.json")
+    tr = None
+    if os.path.exists(step_traffic):
+        with open(step_traffic) as fh:
+            tr = json.load(fh)
+        src = ("profiles/kernels_cfg5_step_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated, "
+               "separate profiled run" + (f" at commit {tr['commit']}" if tr.get("commit") else ""))
+        roof["traffic"] = tr["bytes_per_step"]
+        roof["traffic_source"] = src + " (every kernel of one step)"
+    kroofs = {}
+    names = {"forward_transform": "k_off_rfft_reg (input windows -> X)", "product": "k_off_mac_w (Y = sum X H)",
+             "inverse_transform": "k_off_irfft_reg (Y -> y)", "dh_partial": "k_off_dh_part (dL/dh partial sums)"}
+    for (kind, nl), (avg_us, launches, nbytes) in probes.items():
+        key = f"{kind}@{nl}"
+        kroofs[key] = kernel_roofline(avg_us, launches, nbytes, f"{names[kind]}, {nl} level, 64 clips x 131072")
+        if kroofs[key] and tr and key in tr.get("by_kernel", {}):
+            kroofs[key]["traffic"] = tr["by_kernel"][key]
+            kroofs[key]["traffic_source"] = src
+    # the reference's canonical per-block bytes (8(N+1)(2P+1)+12N per block) describe
+    # the real-time formulation, which streams H and the delay line per block;
+    # this form shares H across the clips and reads each spectrum once, so the
+    # canonical figure is not a traffic count here
     Ns, Ps = conv.N, conv.P
-    fwd_per_sample = sum(block_bytes(nl, pl) / nl for nl, pl in zip(Ns, Ps))
-    step_bytes = 3.0 * fwd_per_sample * C * T
-    step_canon = {"alg_bytes_per_step": step_bytes, "achieved_GBps": step_bytes * args.steps / secs / 1e9,
-                  "partition": [list(Ns), list(Ps)],
-                  "note": "canonical block bytes 8(N+1)(2P+1)+12N for every block of every clip, three passes"}
+    canon = sum(block_bytes(nl, pl) / nl for nl, pl in zip(Ns, Ps))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_diff(x[:8].cpu().numpy(), h0.detach().cpu().numpy(), tgt[:8].cpu().numpy(),
@@ -669,7 +719,11 @@ def bench_diff(args):
                              "grad_equals_eager": graph_ok,
                              "note": "forward + adjoint captured once (torch.cuda.graph) and replayed; loss "
                                      "gradient formed elementwise"},
-        "roofline": roof, "whole_step_canonical_bytes": step_canon, "cpu_baseline": cpu}), flush=True)
+        "roofline": roof, "kernel_rooflines": kroofs,
+        "canonical_bytes_per_clip_sample": {"value": canon, "use": "reference's per-block formula for one forward "
+                                            "pass; not a traffic count for this form (H shared by the clips, each "
+                                            "spectrum read once)"},
+        "cpu_baseline": cpu}), flush=True)
 
 
 def cpu_baseline_diff(x, h, tgt, budget_s):

@@ -410,6 +410,19 @@ int pcc_set_level_tuning(pc_conv *c, int level, int mac_mc, int dh_mc);
  * (every block spectrum of every clip read once, the level's filter spectra
  * once, the products written once). */
 int pcc_probe(pc_conv *c, int level);
+/* The same for another kernel of the level (no reference counterpart): kind
+ * PCC_PROBE_PRODUCT as pcc_probe; PCC_PROBE_FORWARD_TRANSFORM the r2c of the
+ * forward's input windows (every input sample read once, every spectrum
+ * written once); PCC_PROBE_INVERSE_TRANSFORM the forward's c2r into y (every
+ * spectrum read once, y written once, and read once for levels after the
+ * first); PCC_PROBE_DH_PARTIAL the dL/dh partial sums of the backward (G and X
+ * read once, the partial sums written once).  Four-step levels are timed
+ * from their first kernel's start to their second's end. */
+#define PCC_PROBE_PRODUCT 0
+#define PCC_PROBE_FORWARD_TRANSFORM 1
+#define PCC_PROBE_INVERSE_TRANSFORM 2
+#define PCC_PROBE_DH_PARTIAL 3
+int pcc_probe_kernel(pc_conv *c, int level, int kind);
 int pcc_probe_read(pc_conv *c, double *avg_us, long long *launches, double *bytes_per_launch);
 void pcc_destroy(pc_conv *c);
 

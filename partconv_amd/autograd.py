@@ -72,6 +72,10 @@ class PartitionedConv:
         self.set_filter(_prep(h, "h"))
         torch.cuda.synchronize()
         self._check(lib().pcc_tune(self._h, _p(x), x.shape[0], x.shape[1], _stream()))
+        return self.tuning()
+
+    def tuning(self):
+        """[(mac_chunk, dh_chunk)] per level: the chunk sizes in use."""
         out = []
         for l in range(len(self.N)):
             a, b = ctypes.c_int(), ctypes.c_int()
@@ -79,9 +83,15 @@ class PartitionedConv:
             out.append((a.value, b.value))
         return out
 
-    def probe(self, level):
-        """Time the forward spectral product of one level with HIP events (-1 disables)."""
-        self._check(lib().pcc_probe(self._h, int(level)))
+    PROBE_KINDS = {"product": 0, "forward_transform": 1, "inverse_transform": 2, "dh_partial": 3}
+
+    def probe(self, level, kind="product"):
+        """Time one kernel of one level with HIP events stamped by its launch
+        (-1 disables): the forward spectral product ("product"), the forward's
+        input transform ("forward_transform"), its inverse transform into y
+        ("inverse_transform"), or the backward's dL/dh partial sums
+        ("dh_partial")."""
+        self._check(lib().pcc_probe_kernel(self._h, int(level), self.PROBE_KINDS[kind]))
 
     def probe_read(self):
         """(avg_us, launches, algorithmic_bytes_per_launch) of the probed kernel."""

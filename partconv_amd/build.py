@@ -13,7 +13,8 @@ PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libpartconv_amd.so"
 SOURCES = [PKG / "csrc" / "pc_engine.hip", PKG / "csrc" / "pc_io.cpp"]
-HEADERS = [PKG / "csrc" / "pc_fft.hpp", PKG / "csrc" / "pc_kernels.hpp", PKG / "csrc" / "pc_io.hpp",
+HEADERS = [PKG / "csrc" / "pc_fft.hpp", PKG / "csrc" / "pc_fftreg.hpp", PKG / "csrc" / "pc_kernels.hpp",
+           PKG / "csrc" / "pc_offline.hpp", PKG / "csrc" / "pc_mimo_rt.hpp", PKG / "csrc" / "pc_io.hpp",
            ROOT / "include" / "partconv_amd.h"]
 ARCH = os.environ.get("PARTCONV_ARCH", "gfx950")
 

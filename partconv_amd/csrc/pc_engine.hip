@@ -3084,9 +3084,14 @@ __global__ void k_fill_zero(float *p, long long n)
     if (i < n) p[i] = 0.f;
 }
 
+// Offline transforms of block sizes up to this run in one workgroup with the
+// data in registers (pc_fftreg.hpp); larger ones (32768, 65536) use the
+// four-step pair through a staging buffer.
+#define PC_OFF_REG_MAX 16384
+
 struct ConvLevel {
     int N, P, T;
-    bool big;
+    bool big;  // four-step transforms
     BigShape sh;
     cf *H = nullptr;
     // blocks per chunk (one workgroup column each) of the spectral products
@@ -3114,7 +3119,7 @@ struct pc_conv {
     bool keep_valid = false;
     long long recomputed_levels = 0;
     // probe: HIP events around the forward spectral product of one level
-    int probe_level = -1;
+    int probe_level = -1, probe_kind = PCC_PROBE_PRODUCT;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
     double probe_ms = 0, probe_bytes = 0;
     long long probe_n = 0;
@@ -3131,6 +3136,18 @@ static void off_launch(K kern, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent
         hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, s, e0, e1, 0, args...);
     else
         hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, args...);
+}
+
+// With events (e0, e1), a launch stamps them (the probe); e0 alone or e1
+// alone stamps the start or the end of a two-kernel sequence.
+template <typename K, typename... Args>
+static void off_launch_ev(K kern, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                          Args... args)
+{
+    if (e0 || e1)
+        hipExtLaunchKernelGGL(kern, grid, block, lds, s, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
 }
 
 static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLevel &L, int dir, hipStream_t s,
@@ -3155,7 +3172,8 @@ static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLev
 
 // dL/dh of one level into D (P spectra): partial sums per clip and chunk of
 // L.dh_mc blocks, PC_DH_PT partitions per thread, then the fixed-order reduce.
-static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s);
+static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s,
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // Fold the probe's pending events into its totals (waits for them).
 static int conv_probe_fold(pc_conv *c)
@@ -3173,6 +3191,27 @@ static int conv_probe_fold(pc_conv *c)
     return PC_OK;
 }
 
+// Events for the probed launch when (level, kind) is the one probed, else
+// nulls; probe_end files them with the launch's algorithmic bytes.
+static int probe_begin(pc_conv *c, size_t li, int kind, hipEvent_t *e0, hipEvent_t *e1)
+{
+    *e0 = *e1 = nullptr;
+    if (c->probe_level != (int)li || c->probe_kind != kind) return PC_OK;
+    if (c->probe_ev.size() >= 64) {
+        int rc = conv_probe_fold(c);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventCreate(e0));
+    HIPCHK(hipEventCreate(e1));
+    return PC_OK;
+}
+static void probe_end(pc_conv *c, hipEvent_t e0, hipEvent_t e1, double bytes)
+{
+    if (!e0) return;
+    c->probe_ev.push_back({e0, e1});
+    c->probe_bytes = bytes;
+}
+
 static int conv_grow(cf **p, size_t *cap, size_t need)
 {
     if (need <= *cap) return PC_OK;
@@ -3183,16 +3222,26 @@ static int conv_grow(cf **p, size_t *cap, size_t need)
     return PC_OK;
 }
 
-static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s)
+static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s,
+                  hipEvent_t e0, hipEvent_t e1)
 {
     const int nch = (M + L.dh_mc - 1) / L.dh_mc;
     int rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * nch * L.N);
     if (rc) return rc;
-    const long long items = (long long)C * (L.N / 2);
-    hipLaunchKernelGGL(k_off_dh_part<PC_DH_PT>,
-                       dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
-                            (unsigned)nch),
-                       dim3(256), 0, s, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+    if (L.P > PC_DH_PT) {
+        // one bin per thread, up to 16 partitions per group: G and X read once
+        // for P <= 16 (the bin-pair form would read them ceil(P / 8) times)
+        const long long items = (long long)C * L.N;
+        off_launch_ev(k_off_dh_part_b<16>,
+                      dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + 15) / 16), (unsigned)nch), dim3(256),
+                      0, s, e0, e1, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+    } else {
+        const long long items = (long long)C * (L.N / 2);
+        off_launch_ev(k_off_dh_part<PC_DH_PT>,
+                      dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
+                           (unsigned)nch),
+                      dim3(256), 0, s, e0, e1, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+    }
     const long long nd = (long long)L.P * L.N;
     hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, c->dpart, D, C * nch,
                        L.N, L.P);
@@ -3201,21 +3250,28 @@ static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *
 }
 
 template <int LOGN>
-static void off_rfft_t(const WinSrc &W, int ntrans, cf *X, const cf *tw, hipStream_t s)
+static void off_rfft_t(const WinSrc &W, int ntrans, cf *X, const cf *tw, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
-    hipLaunchKernelGGL(k_off_rfft_small<LOGN>, dim3(ntrans), dim3(SmallCfg<LOGN>::NT), 0, s, W, ntrans, X, tw);
+    using F = RegFft<LOGN>;
+    off_launch_ev(k_off_rfft_reg<LOGN>, dim3((unsigned)((ntrans + F::TPB - 1) / F::TPB)), dim3(F::NT), 0, s, e0, e1,
+                  W, ntrans, X, tw);
 }
 template <int LOGN>
-static void off_irfft_t(const cf *Y, const InvDst &D, int ntrans, const cf *tw, hipStream_t s)
+static void off_irfft_t(const cf *Y, const InvDst &D, int ntrans, const cf *tw, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1)
 {
-    hipLaunchKernelGGL(k_off_irfft_small<LOGN>, dim3(ntrans), dim3(SmallCfg<LOGN>::NT), 0, s, Y, D, tw);
+    using F = RegFft<LOGN>;
+    off_launch_ev(k_off_irfft_reg<LOGN>, dim3((unsigned)((ntrans + F::TPB - 1) / F::TPB)), dim3(F::NT), 0, s, e0,
+                  e1, Y, D, ntrans, tw);
 }
 
-// batched r2c of `rows * W.nb` windows into X (N float2 each).  Big levels
+// batched r2c of `rows * W.nb` windows into X (N float2 each, natural bin
+// order for block sizes up to PC_OFF_REG_MAX, four-step slot order above:
+// the products only need every spectrum of a level in one order).  Big levels
 // stage through `scr` (rows * W.nb * N float2) if given, else the handle's
 // grown work space.
 static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, cf *X, hipStream_t s,
-                    cf *scr = nullptr)
+                    cf *scr = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const int nt = rows * W.nb;
     if (nt == 0) return PC_OK;
@@ -3226,20 +3282,23 @@ static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, c
             scr = c->scr;
         }
         const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
-        hipLaunchKernelGGL(k_off_big_fwd1, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, W, L.sh,
-                           scr, c->tw);
-        hipLaunchKernelGGL(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, scr,
-                           X, W.scale, c->tw);
+        off_launch_ev(k_off_big_fwd1, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, e0, nullptr, W,
+                      L.sh, scr, c->tw);
+        off_launch_ev(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, nullptr, e1, L.sh, scr,
+                      X, W.scale, c->tw);
     } else {
         switch (ilog2(L.N)) {
-        case 4: off_rfft_t<4>(W, nt, X, c->tw, s); break;
-        case 5: off_rfft_t<5>(W, nt, X, c->tw, s); break;
-        case 6: off_rfft_t<6>(W, nt, X, c->tw, s); break;
-        case 7: off_rfft_t<7>(W, nt, X, c->tw, s); break;
-        case 8: off_rfft_t<8>(W, nt, X, c->tw, s); break;
-        case 9: off_rfft_t<9>(W, nt, X, c->tw, s); break;
-        case 10: off_rfft_t<10>(W, nt, X, c->tw, s); break;
-        case 11: off_rfft_t<11>(W, nt, X, c->tw, s); break;
+        case 4: off_rfft_t<4>(W, nt, X, c->tw, s, e0, e1); break;
+        case 5: off_rfft_t<5>(W, nt, X, c->tw, s, e0, e1); break;
+        case 6: off_rfft_t<6>(W, nt, X, c->tw, s, e0, e1); break;
+        case 7: off_rfft_t<7>(W, nt, X, c->tw, s, e0, e1); break;
+        case 8: off_rfft_t<8>(W, nt, X, c->tw, s, e0, e1); break;
+        case 9: off_rfft_t<9>(W, nt, X, c->tw, s, e0, e1); break;
+        case 10: off_rfft_t<10>(W, nt, X, c->tw, s, e0, e1); break;
+        case 11: off_rfft_t<11>(W, nt, X, c->tw, s, e0, e1); break;
+        case 12: off_rfft_t<12>(W, nt, X, c->tw, s, e0, e1); break;
+        case 13: off_rfft_t<13>(W, nt, X, c->tw, s, e0, e1); break;
+        case 14: off_rfft_t<14>(W, nt, X, c->tw, s, e0, e1); break;
         default: return fail(PC_ERR_ARG, "block size");
         }
     }
@@ -3251,7 +3310,7 @@ static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, c
 // big levels stage through `scr` (rows * D.nb * N float2) if given, else the
 // handle's grown work space.
 static int off_irfft(pc_conv *c, const ConvLevel &L, const cf *Y, const InvDst &D, int rows, hipStream_t s,
-                     cf *scr = nullptr)
+                     cf *scr = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const int nt = (D.parity >= 0) ? rows * ((D.nb + 1 - D.parity) / 2) : rows * D.nb;
     if (nt == 0) return PC_OK;
@@ -3262,20 +3321,23 @@ static int off_irfft(pc_conv *c, const ConvLevel &L, const cf *Y, const InvDst &
             scr = c->scr;
         }
         const int A = 1 << L.sh.logA, B = 1 << L.sh.logB;
-        hipLaunchKernelGGL(k_off_big_inv1, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, L.sh, Y, scr,
-                           c->tw, D.parity, D.nb);
-        hipLaunchKernelGGL(k_off_big_inv2, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, L.sh, scr,
-                           D, c->tw);
+        off_launch_ev(k_off_big_inv1, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, e0, nullptr, L.sh, Y,
+                      scr, c->tw, D.parity, D.nb);
+        off_launch_ev(k_off_big_inv2, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, nullptr, e1, L.sh,
+                      scr, D, c->tw);
     } else {
         switch (ilog2(L.N)) {
-        case 4: off_irfft_t<4>(Y, D, nt, c->tw, s); break;
-        case 5: off_irfft_t<5>(Y, D, nt, c->tw, s); break;
-        case 6: off_irfft_t<6>(Y, D, nt, c->tw, s); break;
-        case 7: off_irfft_t<7>(Y, D, nt, c->tw, s); break;
-        case 8: off_irfft_t<8>(Y, D, nt, c->tw, s); break;
-        case 9: off_irfft_t<9>(Y, D, nt, c->tw, s); break;
-        case 10: off_irfft_t<10>(Y, D, nt, c->tw, s); break;
-        case 11: off_irfft_t<11>(Y, D, nt, c->tw, s); break;
+        case 4: off_irfft_t<4>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 5: off_irfft_t<5>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 6: off_irfft_t<6>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 7: off_irfft_t<7>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 8: off_irfft_t<8>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 9: off_irfft_t<9>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 10: off_irfft_t<10>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 11: off_irfft_t<11>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 12: off_irfft_t<12>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 13: off_irfft_t<13>(Y, D, nt, c->tw, s, e0, e1); break;
+        case 14: off_irfft_t<14>(Y, D, nt, c->tw, s, e0, e1); break;
         default: return fail(PC_ERR_ARG, "block size");
         }
     }
@@ -3322,7 +3384,7 @@ static pc_conv *conv_create(int M, const int *N, const int *P, int taps)
         cl.N = n[L];
         cl.P = p[L];
         cl.T = T[L];
-        cl.big = n[L] >= PC_BIG_MIN;
+        cl.big = n[L] > PC_OFF_REG_MAX;
         if (cl.big) cl.sh = big_shape(n[L]);
         if (hipMalloc(&cl.H, (size_t)cl.P * cl.N * sizeof(cf)) != hipSuccess) {
             fail(PC_ERR_HIP, "hipMalloc H");
@@ -3453,21 +3515,20 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
             X = c->dsum;
         }
         if ((rc = conv_grow(&c->work, &c->work_cap, n))) return rc;
-        if ((rc = off_rfft(c, L, W, C, X, s))) return rc;
-        if (c->probe_level == (int)li) {
-            if (c->probe_ev.size() >= 64 && (rc = conv_probe_fold(c))) return rc;
-            hipEvent_t e0, e1;
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            off_mac(X, L.H, c->work, C, M, L, 1, s, e0, e1);
-            c->probe_ev.push_back({e0, e1});
-            // the block spectra read once, the filter spectra once, the products written once
-            c->probe_bytes = 2.0 * 8.0 * (double)C * M * L.N + 8.0 * (double)L.P * L.N;
-        } else {
-            off_mac(X, L.H, c->work, C, M, L, 1, s, nullptr, nullptr);
-        }
+        // probed launches carry their algorithmic bytes: every signal sample
+        // and every spectrum read or written once
+        const double sig = 4.0 * (double)C * T, spec = 8.0 * (double)n;
+        hipEvent_t e0, e1;
+        if ((rc = probe_begin(c, li, PCC_PROBE_FORWARD_TRANSFORM, &e0, &e1))) return rc;
+        if ((rc = off_rfft(c, L, W, C, X, s, nullptr, e0, e1))) return rc;
+        probe_end(c, e0, e1, sig + spec);
+        if ((rc = probe_begin(c, li, PCC_PROBE_PRODUCT, &e0, &e1))) return rc;
+        off_mac(X, L.H, c->work, C, M, L, 1, s, e0, e1);
+        probe_end(c, e0, e1, 2.0 * spec + 8.0 * (double)L.P * L.N);
         HIPCHK(hipGetLastError());
-        if ((rc = off_irfft(c, L, c->work, conv_ydst(L, d_y, T, M, li == 0), C, s))) return rc;
+        if ((rc = probe_begin(c, li, PCC_PROBE_INVERSE_TRANSFORM, &e0, &e1))) return rc;
+        if ((rc = off_irfft(c, L, c->work, conv_ydst(L, d_y, T, M, li == 0), C, s, nullptr, e0, e1))) return rc;
+        probe_end(c, e0, e1, spec + (li == 0 ? sig : 2.0 * sig));
     }
     if (keep) {
         c->keep_valid = true;
@@ -3605,7 +3666,12 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
                 if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, s))) return rc;
                 c->recomputed_levels++;
             }
-            if ((rc = off_dh(c, L, G, X, Wsp, C, M, s))) return rc;
+            hipEvent_t e0, e1;
+            if ((rc = probe_begin(c, li, PCC_PROBE_DH_PARTIAL, &e0, &e1))) return rc;
+            if ((rc = off_dh(c, L, G, X, Wsp, C, M, s, e0, e1))) return rc;
+            // G and X read once, the partial sums written once
+            const int nch = (M + L.dh_mc - 1) / L.dh_mc;
+            probe_end(c, e0, e1, 16.0 * (double)n + 8.0 * (double)L.P * C * nch * L.N);
             InvDst D{};
             D.dst = d_dh;
             D.row_stride = 0;
@@ -3723,13 +3789,17 @@ PC_API int pcc_level_tuning(const pc_conv *c, int level, int *mac_mc, int *dh_mc
     return PC_OK;
 }
 
-PC_API int pcc_probe(pc_conv *c, int level)
+PC_API int pcc_probe(pc_conv *c, int level) { return pcc_probe_kernel(c, level, PCC_PROBE_PRODUCT); }
+
+PC_API int pcc_probe_kernel(pc_conv *c, int level, int kind)
 {
     if (!c) return fail(PC_ERR_ARG, "null handle");
     if (level < -1 || level >= (int)c->lv.size()) return fail(PC_ERR_ARG, "probe level out of range");
+    if (kind < PCC_PROBE_PRODUCT || kind > PCC_PROBE_DH_PARTIAL) return fail(PC_ERR_ARG, "unknown probe kind");
     int rc = conv_probe_fold(c);
     if (rc) return rc;
     c->probe_level = level;
+    c->probe_kind = kind;
     c->probe_ms = 0;
     c->probe_n = 0;
     c->probe_bytes = 0;

@@ -1,7 +1,8 @@
 // pc_offline.hpp -- batched offline partitioned convolution and its adjoint
 // (the differentiable form): whole clips [C, T] are resident, every block of
-// every level is computed at once, and the same FFT building blocks and
-// partition rules as the real-time engine are used.
+// every level is computed at once, with the partition rules of the real-time
+// engine.  The block transforms are the register-resident kernels of
+// pc_fftreg.hpp up to N = 16384 and a four-step pair above.
 //
 // Per level L (block N, P partitions, T_L taps before it), output block m
 // covers samples [mN, (m+1)N):
@@ -91,74 +92,11 @@ __device__ __forceinline__ void inv_store(const InvDst &D, int r, int m, int u, 
     else *p = v;
 }
 
-// ---- small N (<= 2048): one workgroup per transform -----------------------
-template <int LOGN>
-__global__ void __launch_bounds__(SmallCfg<LOGN>::NT)
-k_off_rfft_small(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict__ tw)
-{
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
-    __shared__ cf bufA[N];
-    __shared__ cf bufB[N];
-    const int t = blockIdx.x;
-    const int r = t / W.nb, m = t - r * W.nb;
-    const int tid = threadIdx.x;
-    int nz;
-    if (const float4 *w4 = win_fast(W, r, m, 2 * N, &nz)) {
-        for (int q = tid; q < N / 2; q += NT) {
-            const float4 v = (4 * q < nz) ? w4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-            bufA[2 * q] = make_float2(v.x, v.y);
-            bufA[2 * q + 1] = make_float2(v.z, v.w);
-        }
-    } else {
-        for (int n = tid; n < N; n += NT)
-            bufA[n] = make_float2(win_sample(W, r, m, 2 * n), win_sample(W, r, m, 2 * n + 1));
-    }
-    __syncthreads();
-    cf *Z = small_forward<LOGN, NT>(bufA, bufB, tw, tid);
-    cf *o = X + (size_t)t * N;
-    for (int k = tid; k < N; k += NT) o[k] = cscale(Z[k], W.scale);
-    (void)ntrans;
-}
+#include "pc_fftreg.hpp"
 
-template <int LOGN>
-__global__ void __launch_bounds__(SmallCfg<LOGN>::NT)
-k_off_irfft_small(const cf *__restrict__ Y, InvDst D, const cf *__restrict__ tw)
-{
-    constexpr int N = SmallCfg<LOGN>::N, NT = SmallCfg<LOGN>::NT;
-    __shared__ cf bufA[N];
-    __shared__ cf bufB[N];
-    int t = blockIdx.x;
-    if (D.parity >= 0) {
-        // blockIdx enumerates (row, m) with m of the requested parity
-        const int half = (D.nb + 1 - D.parity) / 2;
-        const int r = t / half, mi = t - r * half;
-        t = r * D.nb + 2 * mi + D.parity;
-    }
-    const int r = t / D.nb, m = t - r * D.nb;
-    const int tid = threadIdx.x;
-    const cf *y = Y + (size_t)t * N;
-    for (int k = tid; k < N; k += NT) bufA[k] = y[k];
-    __syncthreads();
-    cf *z = small_inverse<LOGN, NT>(bufA, bufB, tw, tid);
-    const float *zf = reinterpret_cast<const float *>(z);
-    if (float4 *d4 = inv_fast(D, r, m)) {
-        const int nq = (D.uhi - D.ulo) / 4;
-        for (int q = tid; q < nq; q += NT) {
-            const float *zq = zf + D.ulo + 4 * q;
-            float4 v = make_float4(zq[0] * D.scale, zq[1] * D.scale, zq[2] * D.scale, zq[3] * D.scale);
-            if (D.accumulate) {
-                const float4 o = d4[q];
-                v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
-            }
-            d4[q] = v;
-        }
-    } else {
-        for (int u = tid; u < 2 * N; u += NT) inv_store(D, r, m, u, zf[u]);
-    }
-}
-
-// ---- big N (four-step, as k_big1..3) ---------------------------------------
-// S (scratch) holds N float2 per transform.
+// ---- N > 16384 (four-step, as k_big1..3) -----------------------------------
+// S (scratch) holds N float2 per transform.  Block sizes up to 16384 use the
+// register-resident kernels of pc_fftreg.hpp.
 __global__ void __launch_bounds__(BIG_NT)
 k_off_big_fwd1(WinSrc W, BigShape sh, cf *__restrict__ S, const cf *__restrict__ tw)
 {
@@ -707,4 +645,56 @@ __global__ void k_h_transpose(const cf *__restrict__ src, cf *__restrict__ dst, 
     const int p = (int)(r % P);
     const int k = (int)(r / P);
     dst[i] = (o < n_out) ? src[(((size_t)o * n_in + ii) * P + p) * N + k] : make_float2(0.f, 0.f);
+}
+
+// ---- dL/dh partial sums with one bin per thread (float2) ----
+// As k_off_dh_part, with a thread owning one (clip, bin) instead of a bin
+// pair: half the registers per partition, so PT = 16 partitions fit one group
+// and G and X are read once for P <= 16 (the bin-pair form with PT = 8 reads
+// them twice for P = 14), with twice the waves in flight.  Packed bin 0 holds
+// the real DC and Nyquist bins.  Same partial-sum layout and summation order
+// as k_off_dh_part.  Grid: (ceil(C N / 256), ceil(P / PT), nch).
+template <int PT>
+__global__ void __launch_bounds__(256)
+k_off_dh_part_b(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict__ part, int C, int M, int N,
+                int P, int MC)
+{
+    const int nch = (M + MC - 1) / MC;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)C * N) return;
+    const int c = (int)(e / N), k = (int)(e - (long long)c * N);
+    const int p0 = blockIdx.y * PT, j = blockIdx.z;
+    const cf *Gr = G + (size_t)c * M * N + k;
+    const cf *Xr = X + (size_t)c * M * N + k;
+    const int m0 = j * MC, m1 = min(M, m0 + MC);
+    const bool bin0 = (k == 0);
+    cf xw[PT], acc[PT];
+#pragma unroll
+    for (int t = 0; t < PT; t++) {
+        xw[t] = make_float2(0.f, 0.f);
+        acc[t] = make_float2(0.f, 0.f);
+    }
+#pragma unroll 4
+    for (int m = max(p0, m0 - (PT - 1)); m < m1; m++) {
+#pragma unroll
+        for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
+        xw[0] = Xr[(size_t)(m - p0) * N];
+        if (m < m0) continue;
+        const cf g = Gr[(size_t)m * N];
+#pragma unroll
+        for (int t = 0; t < PT; t++) {
+            const cf x = xw[t];
+            if (bin0) {
+                acc[t].x += g.x * x.x;
+                acc[t].y += g.y * x.y;
+            } else {
+                acc[t].x += g.x * x.x + g.y * x.y;
+                acc[t].y += g.y * x.x - g.x * x.y;
+            }
+        }
+    }
+    cf *dst = part + ((size_t)c * nch + j) * N + k;
+#pragma unroll
+    for (int t = 0; t < PT; t++)
+        if (p0 + t < P) dst[(size_t)(p0 + t) * C * nch * N] = acc[t];
 }

@@ -168,6 +168,51 @@ def test_probe_times_the_forward_product():
         conv.probe(5)
 
 
+def test_probe_kernel_kinds():
+    """pcc_probe_kernel: the forward transform, the inverse transform and the
+    dL/dh partial sums of a level are timed once per call, with their
+    algorithmic bytes (signal samples and spectra read or written once); the
+    four-step level (32768) is timed across its two kernels; probing changes
+    no result."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    C, T, taps = 2, 70000, 70000
+    conv = PartitionedConv([256, 2048, 32768], [14, 30, 0], taps)
+    assert conv.N[2] == 32768
+    x = torch.randn((C, T), device="cuda")
+    h = torch.from_numpy(decaying_noise_ir(taps, 0.3, 48000, 10)).cuda().requires_grad_(True)
+    g = torch.randn((C, T), device="cuda")
+
+    def step():
+        h.grad = None
+        y = conv(x, h)
+        y.backward(g)
+        return y.detach().clone(), h.grad.clone()
+
+    y0, d0 = step()
+    for level in (0, 1, 2):
+        N, P = conv.N[level], conv.P[level]
+        M = -(-T // N)
+        sig, spec = 4 * C * T, 8 * C * M * N
+        for kind, want in (("forward_transform", sig + spec),
+                           ("inverse_transform", spec + (sig if level == 0 else 2 * sig)),
+                           ("product", 2 * spec + 8 * P * N),
+                           ("dh_partial", None)):
+            conv.probe(level, kind)
+            for _ in range(3):
+                y1, d1 = step()
+            avg_us, n, nbytes = conv.probe_read()
+            conv.probe(-1)
+            assert n == 3 and avg_us > 0, (level, kind, n)
+            if want is None:
+                dh_mc = conv.tuning()[level][1]
+                want = 2 * spec + 8 * P * C * (-(-M // dh_mc)) * N
+            assert nbytes == want, (level, kind, nbytes, want)
+            assert torch.equal(y0, y1) and torch.equal(d0, d1), (level, kind)
+    with pytest.raises(KeyError):
+        conv.probe(0, "no such kernel")
+
+
 def test_forward_and_adjoint_replay_in_a_cuda_graph():
     """A step (forward + adjoint, dL/dx and dL/dh) captured once in a HIP graph
     and replayed with new inputs, output gradients and filter values written
