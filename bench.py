@@ -585,21 +585,34 @@ def bench_diff(args):
 
     def step():
         h0.grad = None
-        loss = torch.mean((conv(x, h0) - tgt) ** 2)
+        loss = torch.nn.functional.mse_loss(conv(x, h0), tgt)
         loss.backward()
         return loss
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    if args.profile_step:
+        # one step between two marker copies (4 MiB device copies: a copy
+        # kernel of a grid no other launch of the step uses)
+        msrc = torch.ones(1 << 20, dtype=torch.float32, device="cuda")
+        mdst = torch.empty_like(msrc)
+        mdst.copy_(msrc)
+        step()
+        mdst.copy_(msrc)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_step": "cfg5", "marker_bytes": 4 << 20, "tuning": tuning}), flush=True)
+        return
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     # each kernel of each level, timed by HIP events its own launch stamps,
-    # over a few more steps
+    # over a few more steps, with the levels on one stream (each kernel alone
+    # on the device)
     probes = {}
+    conv.set_level_streams("off")
     for li, nl in enumerate(conv.N):
         for kind in ("forward_transform", "product", "inverse_transform", "dh_partial"):
             conv.probe(li, kind)
@@ -608,6 +621,15 @@ def bench_diff(args):
             torch.cuda.synchronize()
             probes[(kind, nl)] = conv.probe_read()
             conv.probe(-1)
+    # the eager step with every level on its own stream, for comparison
+    conv.set_level_streams("on")
+    step()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    secs_par = time.perf_counter() - t2
+    conv.set_level_streams("captured")
     # the same step captured once in a HIP graph and replayed (the MSE gradient
     # formed elementwise, 2 (y - tgt) / numel: no reduction inside the graph)
     scale = 2.0 / (C * T)
@@ -713,12 +735,17 @@ def bench_diff(args):
         "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
                               "ms_per_step": 1e3 * secs2 / args.steps,
                               "recomputed_levels": int(conv2.recomputed_levels)},
+        "eager_level_streams": {"ms_per_step": 1e3 * secs_par / args.steps,
+                                "note": "the same eager step with each level on its own stream (value: every "
+                                        "level on the caller's stream, the default for eager calls; the graph "
+                                        "replay below runs the levels as parallel branches); bitwise the same "
+                                        "results"},
         "tuned_chunks": {"levels": list(conv.N), "mac_dh_blocks_per_chunk": tuning,
                          "from_cache": tinfo["from_cache"]},
         "hip_graph_replay": {"ms_per_step": 1e3 * gsecs / args.steps, "value": C * T * args.steps / gsecs,
                              "grad_equals_eager": graph_ok,
-                             "note": "forward + adjoint captured once (torch.cuda.graph) and replayed; loss "
-                                     "gradient formed elementwise"},
+                             "note": "forward + adjoint captured once (torch.cuda.graph) and replayed, the "
+                                     "levels as parallel branches; loss gradient formed elementwise"},
         "roofline": roof, "kernel_rooflines": kroofs,
         "canonical_bytes_per_clip_sample": {"value": canon, "use": "reference's per-block formula for one forward "
                                             "pass; not a traffic count for this form (H shared by the clips, each "
@@ -775,6 +802,9 @@ def main():
     ap.add_argument("--instances", type=int, default=512, help="cfg3 instance count")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 64 MiB device copy first (HBM-counter calibration under a profiler)")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="cfg5: after set-up and warm-up run one step between two 4 MiB device copies (markers "
+                         "for per-step counter sums, prof_summary step) and stop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: each GPU runs its own share of channels independently "

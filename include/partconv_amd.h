@@ -385,6 +385,19 @@ int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, int T, 
  * be null). */
 int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
                  void *stream);
+/* Level-parallel calls (no reference counterpart): with mode
+ * PCC_LEVEL_STREAMS_ON pcc_forward and pcc_backward run each level on its own
+ * stream, forked from the caller's stream with events and joined back to it,
+ * so they behave as one stream for the caller; PCC_LEVEL_STREAMS_CAPTURED
+ * (the default) does so only while the caller's stream is being captured
+ * into a HIP graph (eager calls are faster on one stream, replayed graphs on
+ * several); PCC_LEVEL_STREAMS_OFF keeps every level on the caller's stream.
+ * The inverse transforms that add into y or dx run in level order, so results
+ * are bitwise the same in every mode.  Each level holds its own work space. */
+#define PCC_LEVEL_STREAMS_OFF 0
+#define PCC_LEVEL_STREAMS_ON 1
+#define PCC_LEVEL_STREAMS_CAPTURED 2
+int pcc_set_level_streams(pc_conv *c, int mode);
 /* Memory budget (bytes) for keeping every level's input spectra from forward
  * for backward; above it backward recomputes them from the stored input. */
 int pcc_set_budget(pc_conv *c, long long bytes);

@@ -74,6 +74,17 @@ class PartitionedConv:
         self._check(lib().pcc_tune(self._h, _p(x), x.shape[0], x.shape[1], _stream()))
         return self.tuning()
 
+    LEVEL_STREAMS = {"off": 0, "on": 1, "captured": 2}
+
+    def set_level_streams(self, mode):
+        """Run each level on its own stream: "on", "off" (every level on the
+        caller's stream) or "captured" (default: only while the caller's
+        stream is captured into a graph).  True / False mean "on" / "off".
+        Results are bitwise the same in every mode."""
+        if isinstance(mode, bool):
+            mode = "on" if mode else "off"
+        self._check(lib().pcc_set_level_streams(self._h, self.LEVEL_STREAMS[mode]))
+
     def tuning(self):
         """[(mac_chunk, dh_chunk)] per level: the chunk sizes in use."""
         out = []

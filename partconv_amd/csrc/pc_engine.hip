@@ -3099,10 +3099,25 @@ struct ConvLevel {
     int mac_mc = 32, dh_mc = 64;
 };
 
+// One level's work space and stream for level-parallel calls: the levels of
+// a forward or backward run side by side on their own streams (forked from
+// and joined back to the caller's stream with events); only the inverse
+// transforms that add into y (forward) or dL/dx (backward) are chained in
+// level order, so every result is bitwise the same as with one stream.
+struct ConvLevelWork {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;  // chain / join point of this level
+    cf *Y = nullptr, *G = nullptr, *W = nullptr, *Xr = nullptr, *part = nullptr, *D = nullptr, *scr = nullptr;
+    size_t Yc = 0, Gc = 0, Wc = 0, Xrc = 0, partc = 0, Dc = 0, scrc = 0;
+};
+
 struct pc_conv {
     int dev = 0;
     int taps = 0;
     std::vector<ConvLevel> lv;
+    std::vector<ConvLevelWork> lw;
+    hipEvent_t fork_ev = nullptr;
+    int level_streams = PCC_LEVEL_STREAMS_CAPTURED;
     cf *tw = nullptr;
     // filter state
     const float *h_src = nullptr;
@@ -3173,7 +3188,7 @@ static void off_mac(const cf *X, const cf *H, cf *Y, int C, int M, const ConvLev
 // dL/dh of one level into D (P spectra): partial sums per clip and chunk of
 // L.dh_mc blocks, PC_DH_PT partitions per thread, then the fixed-order reduce.
 static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s,
-                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, cf **part = nullptr, size_t *part_cap = nullptr);
 
 // Fold the probe's pending events into its totals (waits for them).
 static int conv_probe_fold(pc_conv *c)
@@ -3223,27 +3238,32 @@ static int conv_grow(cf **p, size_t *cap, size_t need)
 }
 
 static int off_dh(pc_conv *c, const ConvLevel &L, const cf *G, const cf *X, cf *D, int C, int M, hipStream_t s,
-                  hipEvent_t e0, hipEvent_t e1)
+                  hipEvent_t e0, hipEvent_t e1, cf **part, size_t *part_cap)
 {
     const int nch = (M + L.dh_mc - 1) / L.dh_mc;
-    int rc = conv_grow(&c->dpart, &c->dpart_cap, (size_t)L.P * C * nch * L.N);
+    if (!part) {
+        part = &c->dpart;
+        part_cap = &c->dpart_cap;
+    }
+    int rc = conv_grow(part, part_cap, (size_t)L.P * C * nch * L.N);
     if (rc) return rc;
+    cf *dpart = *part;
     if (L.P > PC_DH_PT) {
         // one bin per thread, up to 16 partitions per group: G and X read once
         // for P <= 16 (the bin-pair form would read them ceil(P / 8) times)
         const long long items = (long long)C * L.N;
         off_launch_ev(k_off_dh_part_b<16>,
                       dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + 15) / 16), (unsigned)nch), dim3(256),
-                      0, s, e0, e1, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+                      0, s, e0, e1, G, X, dpart, C, M, L.N, L.P, L.dh_mc);
     } else {
         const long long items = (long long)C * (L.N / 2);
         off_launch_ev(k_off_dh_part<PC_DH_PT>,
                       dim3((unsigned)((items + 255) / 256), (unsigned)((L.P + PC_DH_PT - 1) / PC_DH_PT),
                            (unsigned)nch),
-                      dim3(256), 0, s, e0, e1, G, X, c->dpart, C, M, L.N, L.P, L.dh_mc);
+                      dim3(256), 0, s, e0, e1, G, X, dpart, C, M, L.N, L.P, L.dh_mc);
     }
     const long long nd = (long long)L.P * L.N;
-    hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, c->dpart, D, C * nch,
+    hipLaunchKernelGGL(k_off_dh_reduce, dim3((unsigned)((nd + 15) / 16)), dim3(256), 0, s, dpart, D, C * nch,
                        L.N, L.P);
     HIPCHK(hipGetLastError());
     return PC_OK;
@@ -3396,6 +3416,7 @@ static pc_conv *conv_create(int M, const int *N, const int *P, int taps)
     }
     c->keep.assign(M, nullptr);
     c->keep_cap.assign(M, 0);
+    c->lw.resize(M);
     return c;
 }
 
@@ -3490,6 +3511,48 @@ static InvDst conv_ydst(const ConvLevel &L, float *y, int T, int M, bool first)
     return D;
 }
 
+// The stream each level of a call runs on: with level streams on (or, by
+// default, while the caller's stream is being captured into a graph) and
+// more than one level, the level's own stream, made to wait for the work
+// queued on the caller's stream s so far; else s.  Eager calls default to one
+// stream: the event records and waits cost more host time than the overlap
+// gains, while a replayed graph runs the level branches side by side.
+// Returns whether levels fork.
+static int conv_fork(pc_conv *c, hipStream_t s, std::vector<hipStream_t> &st, bool *par)
+{
+    const size_t G = c->lv.size();
+    st.assign(G, s);
+    bool on = c->level_streams == PCC_LEVEL_STREAMS_ON;
+    if (c->level_streams == PCC_LEVEL_STREAMS_CAPTURED) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(s, &cs));
+        on = cs == hipStreamCaptureStatusActive;
+    }
+    *par = on && G > 1;
+    if (!*par) return PC_OK;
+    if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->fork_ev, s));
+    for (size_t li = 0; li < G; li++) {
+        ConvLevelWork &w = c->lw[li];
+        if (!w.s) HIPCHK(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
+        if (!w.ev) HIPCHK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+        HIPCHK(hipStreamWaitEvent(w.s, c->fork_ev, 0));
+        st[li] = w.s;
+    }
+    return PC_OK;
+}
+
+// four-step staging of one level's transforms (null below the four-step sizes)
+static int conv_level_scr(const ConvLevel &L, ConvLevelWork &w, size_t n, cf **scr)
+{
+    *scr = nullptr;
+    if (!L.big) return PC_OK;
+    int rc = conv_grow(&w.scr, &w.scrc, n);
+    if (rc) return rc;
+    *scr = w.scr;
+    return PC_OK;
+}
+
 // y = sum over levels of the partitioned convolution of every clip (x, y: [C][T])
 PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
 {
@@ -3501,35 +3564,45 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
     for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
     const bool keep = (long long)need_keep <= c->budget;
     c->keep_valid = false;
+    std::vector<hipStream_t> st;
+    bool par;
+    if ((rc = conv_fork(c, s, st, &par))) return rc;
     for (size_t li = 0; li < c->lv.size(); li++) {
         ConvLevel &L = c->lv[li];
+        ConvLevelWork &w = c->lw[li];
+        const hipStream_t ls = st[li];
         WinSrc W = conv_xwin(L, d_x, T);
         const int M = W.nb;
         const size_t n = (size_t)C * M * L.N;
-        cf *X;
+        cf *X, *scr;
         if (keep) {
             if ((rc = conv_grow(&c->keep[li], &c->keep_cap[li], n))) return rc;
             X = c->keep[li];
         } else {
-            if ((rc = conv_grow(&c->dsum, &c->dsum_cap, n))) return rc;
-            X = c->dsum;
+            if ((rc = conv_grow(&w.Xr, &w.Xrc, n))) return rc;
+            X = w.Xr;
         }
-        if ((rc = conv_grow(&c->work, &c->work_cap, n))) return rc;
+        if ((rc = conv_grow(&w.Y, &w.Yc, n))) return rc;
+        if ((rc = conv_level_scr(L, w, n, &scr))) return rc;
         // probed launches carry their algorithmic bytes: every signal sample
         // and every spectrum read or written once
         const double sig = 4.0 * (double)C * T, spec = 8.0 * (double)n;
         hipEvent_t e0, e1;
         if ((rc = probe_begin(c, li, PCC_PROBE_FORWARD_TRANSFORM, &e0, &e1))) return rc;
-        if ((rc = off_rfft(c, L, W, C, X, s, nullptr, e0, e1))) return rc;
+        if ((rc = off_rfft(c, L, W, C, X, ls, scr, e0, e1))) return rc;
         probe_end(c, e0, e1, sig + spec);
         if ((rc = probe_begin(c, li, PCC_PROBE_PRODUCT, &e0, &e1))) return rc;
-        off_mac(X, L.H, c->work, C, M, L, 1, s, e0, e1);
+        off_mac(X, L.H, w.Y, C, M, L, 1, ls, e0, e1);
         probe_end(c, e0, e1, 2.0 * spec + 8.0 * (double)L.P * L.N);
         HIPCHK(hipGetLastError());
+        // level 0 writes y, later levels add to it: in level order
+        if (par && li > 0) HIPCHK(hipStreamWaitEvent(ls, c->lw[li - 1].ev, 0));
         if ((rc = probe_begin(c, li, PCC_PROBE_INVERSE_TRANSFORM, &e0, &e1))) return rc;
-        if ((rc = off_irfft(c, L, c->work, conv_ydst(L, d_y, T, M, li == 0), C, s, nullptr, e0, e1))) return rc;
+        if ((rc = off_irfft(c, L, w.Y, conv_ydst(L, d_y, T, M, li == 0), C, ls, scr, e0, e1))) return rc;
         probe_end(c, e0, e1, spec + (li == 0 ? sig : 2.0 * sig));
+        if (par) HIPCHK(hipEventRecord(w.ev, ls));
     }
+    if (par) HIPCHK(hipStreamWaitEvent(s, c->lw[c->lv.size() - 1].ev, 0));
     if (keep) {
         c->keep_valid = true;
         c->keep_x = d_x;
@@ -3619,13 +3692,19 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
     int rc;
     if (d_dx && (rc = conv_fill_zero(d_dx, (long long)C * T, s))) return rc;
     const bool kept = c->keep_valid && c->keep_x == d_x && c->keep_C == C && c->keep_T == T;
+    std::vector<hipStream_t> st;
+    bool par;
+    if ((rc = conv_fork(c, s, st, &par))) return rc;
     for (size_t li = 0; li < c->lv.size(); li++) {
         ConvLevel &L = c->lv[li];
+        ConvLevelWork &w = c->lw[li];
+        const hipStream_t ls = st[li];
         const int M = (T + L.N - 1) / L.N;
         const size_t n = (size_t)C * M * L.N;
-        // G = r2c([g block | 0_N]); second half of work: W (dx) or D_p (dh)
-        if ((rc = conv_grow(&c->work, &c->work_cap, n + std::max(n, (size_t)L.P * L.N)))) return rc;
-        cf *G = c->work, *Wsp = c->work + n;
+        cf *scr;
+        if ((rc = conv_level_scr(L, w, n, &scr))) return rc;
+        // G = r2c([g block | 0_N])
+        if ((rc = conv_grow(&w.G, &w.Gc, n))) return rc;
         WinSrc Wg{};
         Wg.src = d_g;
         Wg.row_stride = T;
@@ -3636,39 +3715,21 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
         Wg.ulo = 0;
         Wg.uhi = L.N;
         Wg.scale = 1.f;
-        if ((rc = off_rfft(c, L, Wg, C, G, s))) return rc;
-        if (d_dx) {
-            off_mac(G, L.H, Wsp, C, M, L, -1, s, nullptr, nullptr);
-            HIPCHK(hipGetLastError());
-            InvDst D{};
-            D.dst = d_dx;
-            D.row_stride = T;
-            D.len = T;
-            D.base0 = -(long long)L.N - L.T;
-            D.base_step = L.N;
-            D.nb = M;
-            D.ulo = 0;
-            D.uhi = 2 * L.N;
-            D.scale = 1.f;
-            D.accumulate = 1;
-            for (int par = 0; par < 2; par++) {
-                D.parity = par;
-                if ((rc = off_irfft(c, L, Wsp, D, C, s))) return rc;
-            }
-        }
+        if ((rc = off_rfft(c, L, Wg, C, w.G, ls, scr))) return rc;
         if (d_dh) {
             cf *X;
             if (kept) X = c->keep[li];
             else {
                 // recompute the delay-line spectra from the stored time-domain input
-                if ((rc = conv_grow(&c->dsum, &c->dsum_cap, n))) return rc;
-                X = c->dsum;
-                if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, s))) return rc;
+                if ((rc = conv_grow(&w.Xr, &w.Xrc, n))) return rc;
+                X = w.Xr;
+                if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, ls, scr))) return rc;
                 c->recomputed_levels++;
             }
+            if ((rc = conv_grow(&w.D, &w.Dc, (size_t)L.P * L.N))) return rc;
             hipEvent_t e0, e1;
             if ((rc = probe_begin(c, li, PCC_PROBE_DH_PARTIAL, &e0, &e1))) return rc;
-            if ((rc = off_dh(c, L, G, X, Wsp, C, M, s, e0, e1))) return rc;
+            if ((rc = off_dh(c, L, w.G, X, w.D, C, M, ls, e0, e1, &w.part, &w.partc))) return rc;
             // G and X read once, the partial sums written once
             const int nch = (M + L.dh_mc - 1) / L.dh_mc;
             probe_end(c, e0, e1, 16.0 * (double)n + 8.0 * (double)L.P * C * nch * L.N);
@@ -3684,9 +3745,43 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
             D.scale = 1.0f / (float)(2 * L.N);
             D.accumulate = 0;
             D.parity = -1;
-            if ((rc = off_irfft(c, L, Wsp, D, 1, s))) return rc;
+            if ((rc = off_irfft(c, L, w.D, D, 1, ls, scr))) return rc;
         }
+        if (d_dx) {
+            if ((rc = conv_grow(&w.W, &w.Wc, n))) return rc;
+            off_mac(w.G, L.H, w.W, C, M, L, -1, ls, nullptr, nullptr);
+            HIPCHK(hipGetLastError());
+            InvDst D{};
+            D.dst = d_dx;
+            D.row_stride = T;
+            D.len = T;
+            D.base0 = -(long long)L.N - L.T;
+            D.base_step = L.N;
+            D.nb = M;
+            D.ulo = 0;
+            D.uhi = 2 * L.N;
+            D.scale = 1.f;
+            D.accumulate = 1;
+            // every level adds to dL/dx: in level order
+            if (par && li > 0) HIPCHK(hipStreamWaitEvent(ls, c->lw[li - 1].ev, 0));
+            for (int par2 = 0; par2 < 2; par2++) {
+                D.parity = par2;
+                if ((rc = off_irfft(c, L, w.W, D, C, ls, scr))) return rc;
+            }
+        }
+        if (par) HIPCHK(hipEventRecord(w.ev, ls));
     }
+    if (par)
+        for (size_t li = 0; li < c->lv.size(); li++) HIPCHK(hipStreamWaitEvent(s, c->lw[li].ev, 0));
+    return PC_OK;
+}
+
+PC_API int pcc_set_level_streams(pc_conv *c, int mode)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    if (mode < PCC_LEVEL_STREAMS_OFF || mode > PCC_LEVEL_STREAMS_CAPTURED)
+        return fail(PC_ERR_ARG, "level stream mode must be 0 (off), 1 (on) or 2 (under graph capture)");
+    c->level_streams = mode;
     return PC_OK;
 }
 
@@ -3823,6 +3918,12 @@ static void conv_destroy(pc_conv *c)
     conv_probe_fold(c);
     for (ConvLevel &L : c->lv) hipFree(L.H);
     for (cf *p : c->keep) hipFree(p);
+    for (ConvLevelWork &w : c->lw) {
+        for (cf *p : {w.Y, w.G, w.W, w.Xr, w.part, w.D, w.scr}) hipFree(p);
+        if (w.ev) hipEventDestroy(w.ev);
+        if (w.s) hipStreamDestroy(w.s);
+    }
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
     hipFree(c->work);
     hipFree(c->scr);
     hipFree(c->dpart);

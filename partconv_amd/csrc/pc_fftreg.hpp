@@ -8,9 +8,12 @@
 // radix-4 form of fft_lds makes 6).  The forward's first pass loads its
 // inputs straight from the signal, and the inverse's last pass stores its
 // outputs straight to the signal.  A workgroup is one wave holding 64 / (N/16)
-// transforms for N <= 1024, else one transform (N/16 threads); N = 16384
-// takes 139 KB of LDS, one workgroup per CU, and replaces the four-step pair
-// with its scratch round trip.
+// transforms for N <= 1024, two transforms for N = 2048, else one transform
+// (N/16 threads); N = 16384 takes 139 KB of LDS, one workgroup per CU, and
+// replaces the four-step pair with its scratch round trip.  Every pass's
+// twiddles are requested at the start (w^j, w^2j, w^4j, w^8j per butterfly;
+// the other powers are products of these), so no global load waits behind a
+// barrier.
 //
 // Pass s (span NS = 16^s, radix R) is the decimation-in-time Stockham step
 // fft_lds uses: butterfly j reads elements j + q N/R (q < R), multiplies
@@ -30,7 +33,7 @@ struct RegFft {
     static constexpr int NP16 = LOGN / 4;                 // radix-16 passes
     static constexpr int RL = 1 << (LOGN % 4);            // radix of the last pass (1: none)
     static constexpr int NPASS = NP16 + (RL > 1 ? 1 : 0);
-    static constexpr int TPB = TT >= 64 ? 1 : 64 / TT;    // transforms per workgroup
+    static constexpr int TPB = TT >= 256 ? 1 : TT >= 128 ? 2 : 64 / TT;  // transforms per workgroup
     static constexpr int NT = TT * TPB;                   // threads per workgroup
     static constexpr int LDSN = N + N / 16;               // padded slots per transform
 };
@@ -140,19 +143,83 @@ __device__ __forceinline__ void dftr(cf *v)
     else dft2r<INV>(v);
 }
 
+// Radix, span and butterflies per thread of pass PASS; LB = log2 R twiddle
+// powers w^(jm), w^(2 jm), ... are loaded per butterfly.
+template <int LOGN, int PASS>
+struct PassShape {
+    static constexpr int R = (PASS < RegFft<LOGN>::NP16) ? 16 : RegFft<LOGN>::RL;
+    static constexpr int NS = 1 << (4 * PASS);
+    static constexpr int NB = 16 / R;
+    static constexpr int LB = (R == 16) ? 4 : (R == 8) ? 3 : (R == 4) ? 2 : 1;
+};
+#define REG_TWB 8  // base twiddle slots per pass (NB * LB <= 8)
+
+// Every pass's base twiddles, loaded once at the start of the kernel so that
+// their latency overlaps the first loads instead of following each barrier:
+// for pass s >= 1 and butterfly b, w^(2^k jm) with w = exp(-+2 pi i / (NS R)),
+// jm = (t + b TT) mod NS, k < LB.
+template <int LOGN, bool INV, int PASS>
+__device__ __forceinline__ void reg_twiddles(cf *twb, int t, const cf *__restrict__ tw)
+{
+    if constexpr (PASS < RegFft<LOGN>::NPASS) {
+        using S = PassShape<LOGN, PASS>;
+#pragma unroll
+        for (int b = 0; b < S::NB; b++) {
+            const int jm = (t + b * RegFft<LOGN>::TT) & (S::NS - 1);
+#pragma unroll
+            for (int k = 0; k < S::LB; k++) twb[PASS * REG_TWB + b * S::LB + k] = twiddle<INV>(tw, S::NS * S::R, jm << k);
+        }
+        reg_twiddles<LOGN, INV, PASS + 1>(twb, t, tw);
+    }
+}
+
+// w[q] = w^q for q = 1 .. R-1 from (w, w^2, w^4, w^8): at most three complex
+// products each.
+template <int R>
+__device__ __forceinline__ void tw_expand(const cf *base, cf *w)
+{
+    w[1] = base[0];
+    if constexpr (R > 2) {
+        w[2] = base[1];
+        w[3] = cmul(w[1], w[2]);
+    }
+    if constexpr (R > 4) {
+        w[4] = base[2];
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R > 8) {
+        w[8] = base[3];
+#pragma unroll
+        for (int q = 1; q < 8; q++) w[8 + q] = cmul(w[8], w[q]);
+    }
+}
+
+// exp(-i pi q / 16), q = 0 .. 15: the real-split twiddle of bin t + q N/16 is
+// exp(-i pi t / N) times this.
+__device__ __forceinline__ cf split_rot(int q)
+{
+    // cos, sin of pi q / 16 for q = 0 .. 8
+    constexpr float c[9] = {1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+                            0.70710678118654752f, 0.55557023301960218f, 0.38268343236508978f,
+                            0.19509032201612825f, 0.0f};
+    return q <= 8 ? make_float2(c[q], -c[8 - q]) : make_float2(-c[16 - q], -c[q - 8]);
+}
+
 // Pass PASS of a transform whose pass-PASS inputs are in v (PASS 0) or in L
-// (later passes).  Runs this pass and every later one; on return v holds the
+// (later passes), with the base twiddles of reg_twiddles in twb.  Runs this
+// pass and every later one; on return v holds the
 // last pass's outputs: butterfly b's output q is element
 // t + b TT + q N / RL_last (the last pass has span N / R).  Contains barriers:
 // every thread of the workgroup calls it.
 template <int LOGN, bool INV, int PASS>
-__device__ __forceinline__ void reg_passes(cf *v, cf *L, int t, const cf *__restrict__ tw)
+__device__ __forceinline__ void reg_passes(cf *v, cf *L, int t, const cf *twb)
 {
     using F = RegFft<LOGN>;
+    using S = PassShape<LOGN, PASS>;
     constexpr int N = F::N, TT = F::TT;
-    constexpr int R = (PASS < F::NP16) ? 16 : F::RL;
-    constexpr int NS = 1 << (4 * PASS);
-    constexpr int NB = 16 / R;
+    constexpr int R = S::R, NS = S::NS, NB = S::NB;
     if constexpr (PASS > 0) {
 #pragma unroll
         for (int b = 0; b < NB; b++)
@@ -160,9 +227,10 @@ __device__ __forceinline__ void reg_passes(cf *v, cf *L, int t, const cf *__rest
             for (int q = 0; q < R; q++) v[b * R + q] = L[rpad(t + b * TT + q * (N / R))];
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const int jm = (t + b * TT) & (NS - 1);
+            cf w[R];
+            tw_expand<R>(twb + PASS * REG_TWB + b * S::LB, w);
 #pragma unroll
-            for (int q = 1; q < R; q++) v[b * R + q] = cmul(v[b * R + q], twiddle<INV>(tw, NS * R, jm * q));
+            for (int q = 1; q < R; q++) v[b * R + q] = cmul(v[b * R + q], w[q]);
         }
     }
 #pragma unroll
@@ -176,7 +244,7 @@ __device__ __forceinline__ void reg_passes(cf *v, cf *L, int t, const cf *__rest
             for (int q = 0; q < R; q++) L[rpad((j - jm) * R + jm + q * NS)] = v[b * R + q];
         }
         __syncthreads();
-        reg_passes<LOGN, INV, PASS + 1>(v, L, t, tw);
+        reg_passes<LOGN, INV, PASS + 1>(v, L, t, twb);
     }
 }
 
@@ -232,9 +300,12 @@ k_off_rfft_reg(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict__ 
     const bool live = tr < ntrans;
     const int r = live ? tr / W.nb : 0, m = live ? tr - r * W.nb : 0;
     cf *L = lds + sub * F::LDSN;
+    cf twb[F::NPASS * REG_TWB];
+    reg_twiddles<LOGN, false, 1>(twb, t, tw);
+    const cf wsplit = tw[2 * N + t];  // exp(-i pi t / N)
     cf v[16];
     reg_load_window<LOGN>(W, r, m, t, live, v);
-    reg_passes<LOGN, false, 0>(v, L, t, tw);
+    reg_passes<LOGN, false, 0>(v, L, t, twb);
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < LP::NB; b++)
@@ -243,13 +314,16 @@ k_off_rfft_reg(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict__ 
     __syncthreads();
     if (!live) return;
     cf *o = X + (size_t)tr * N;
-    for (int k = t; k <= N / 2; k += TT) {
+#pragma unroll
+    for (int i = 0; i <= 8; i++) {
+        const int k = t + i * TT;
+        if (k > N / 2) break;
         if (k == 0) {
             const cf z = L[0];
             o[0] = cscale(make_float2(z.x + z.y, z.x - z.y), W.scale);
         } else {
             cf xk, xnk;
-            rsplit_fwd(L[rpad(k)], L[rpad(N - k)], tw[2 * N + k], &xk, &xnk);
+            rsplit_fwd(L[rpad(k)], L[rpad(N - k)], i ? cmul(wsplit, split_rot(i)) : wsplit, &xk, &xnk);
             o[k] = cscale(xk, W.scale);
             o[N - k] = cscale(xnk, W.scale);
         }
@@ -279,6 +353,9 @@ k_off_irfft_reg(const cf *__restrict__ Y, InvDst D, int ntrans, const cf *__rest
     }
     const int r = tr / D.nb, m = tr - r * D.nb;
     cf *L = lds + sub * F::LDSN;
+    cf twb[F::NPASS * REG_TWB];
+    reg_twiddles<LOGN, true, 1>(twb, t, tw);
+    const cf wsplit = tw[2 * N + t];  // exp(-i pi t / N)
     if (live) {
         const cf *y = Y + (size_t)tr * N;
 #pragma unroll
@@ -294,11 +371,11 @@ k_off_irfft_reg(const cf *__restrict__ Y, InvDst D, int ntrans, const cf *__rest
             v[q] = make_float2(y0.x + y0.y, y0.x - y0.y);
         } else {
             cf ck, cnk;
-            rsplit_inv(L[rpad(k)], L[rpad(N - k)], tw[2 * N + k], &ck, &cnk);
+            rsplit_inv(L[rpad(k)], L[rpad(N - k)], q ? cmul(wsplit, split_rot(q)) : wsplit, &ck, &cnk);
             v[q] = ck;
         }
     }
-    reg_passes<LOGN, true, 0>(v, L, t, tw);
+    reg_passes<LOGN, true, 0>(v, L, t, twb);
     if (!live) return;
     const long long i0 = D.base0 + (long long)m * D.base_step;
     float *row = D.dst + (long long)r * D.row_stride;

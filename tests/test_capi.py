@@ -185,6 +185,44 @@ def test_prof_summary_splits_calls_by_phase(tmp_path, capsys):
     assert "k_other(int)" in out
 
 
+def test_prof_summary_step_sums_between_markers(tmp_path):
+    """prof_summary step: only the dispatches between the last two 4 MiB
+    copies count (a copy of another size inside the step is a kernel of the
+    step); bytes are scaled by the calibration copy (65536 KiB); the
+    transforms are keyed by level and order (filter, forward, gradient,
+    inverse), the products and dL/dh kernels by their order (levels
+    ascending), other kernels as the loss."""
+    import json
+    from partconv_amd import prof_summary
+    seq = [("__amd_rocclr_copyBuffer", 131072, 2 * 65536),   # calibration: reads half -> scale 0.5
+           ("void k_off_rfft_reg<8>(WinSrc)", 1024, 999),      # before the markers: ignored
+           ("__amd_rocclr_copyBuffer", 131072, 8192),         # marker: 4 MiB after scaling
+           ("void k_off_rfft_reg<8>(WinSrc)", 64, 2),          # filter transform (smaller grid)
+           ("void k_off_rfft_reg<8>(WinSrc)", 1024, 10),       # forward transform
+           ("void k_off_mac_w<16, 1>(float2 const*)", 512, 20),
+           ("void k_off_irfft_reg<8>(float2 const*)", 1024, 30),
+           ("void at::native::reduce_kernel<512>(int)", 512, 4),
+           ("__amd_rocclr_copyBuffer", 131072, 16384),        # a copy inside the step (loss)
+           ("void k_off_rfft_reg<8>(WinSrc)", 1024, 40),       # gradient transform
+           ("void k_off_dh_part_b<16>(float2 const*)", 256, 50),
+           ("k_off_dh_reduce(float2 const*)", 256, 6),
+           ("void k_off_irfft_reg<8>(float2 const*)", 64, 8),  # dL/dh inverse
+           ("__amd_rocclr_copyBuffer", 131072, 8192)]         # marker
+    for cname in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = ["Dispatch_Id,Kernel_Name,Grid_Size,Counter_Name,Counter_Value"]
+        for i, (n, g, v) in enumerate(seq):
+            rows.append(f'{i},"{n}",{g},{cname},{v}')
+        (tmp_path / f"{cname}.csv").write_text("\n".join(rows) + "\n")
+    out = tmp_path / "step.json"
+    prof_summary.step(str(tmp_path / "FETCH_SIZE.csv"), str(tmp_path / "WRITE_SIZE.csv"), 131072, str(out), "abc")
+    d = json.loads(out.read_text())
+    kib = 1024 * 0.5 * 2  # both counters, scale 0.5
+    want = {"filter_transform@256": 2, "forward_transform@256": 10, "product@256": 20, "inverse_transform@256": 30,
+            "loss": 4 + 16384, "gradient_transform@256": 40, "dh_partial@256": 50, "dh_reduce@256": 6, "dh_inverse@256": 8}
+    assert d["by_kernel"] == {k: v * kib for k, v in want.items()}
+    assert d["bytes_per_step"] == sum(want.values()) * kib and d["commit"] == "abc"
+
+
 def test_cxx_reference_call_shapes(tmp_path):
     """The reference's own call shapes compile and behave without a GPU:
     PartConvFilter::setup(M, N, P, Vector*, fftwOptLevel, wisdom) with a

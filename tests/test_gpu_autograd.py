@@ -213,6 +213,44 @@ def test_probe_kernel_kinds():
         conv.probe(0, "no such kernel")
 
 
+@pytest.mark.parametrize("budget", [1 << 30, 0])
+def test_level_streams_bitwise_equal_to_one_stream(budget):
+    """pcc_set_level_streams: levels on their own streams give y, dL/dx and
+    dL/dh bitwise equal to every level on the caller's stream (and to the
+    default mode, which is one stream outside graph capture), with kept or
+    recomputed spectra and with a four-step level (32768)."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    C, T, taps = 3, 90000, 70000
+    conv = PartitionedConv([256, 2048, 32768], [14, 30, 0], taps, budget_bytes=budget)
+    x = torch.randn((C, T), device="cuda")
+    g = torch.randn((C, T), device="cuda")
+    h = torch.from_numpy(decaying_noise_ir(taps, 0.3, 48000, 11)).cuda()
+    out = {}
+    for on in ("off", "on", "off", "on", "captured"):
+        conv.set_level_streams(on)
+        xg = x.clone().requires_grad_(True)
+        hg = h.clone().requires_grad_(True)
+        y = conv(xg, hg)
+        y.backward(g)
+        torch.cuda.synchronize()
+        r = (y.detach().clone(), xg.grad.clone(), hg.grad.clone())
+        if on in out:
+            assert all(torch.equal(a, b) for a, b in zip(out[on], r)), on
+        out[on] = r
+    assert all(torch.equal(a, b) for a, b in zip(out["off"], out["on"]))
+    assert all(torch.equal(a, b) for a, b in zip(out["off"], out["captured"]))
+    with pytest.raises(KeyError):
+        conv.set_level_streams("sometimes")
+    # and against FP64
+    xn, hn, gn = x.cpu().numpy(), h.cpu().numpy(), g.cpu().numpy()
+    yr = np.stack([O.direct_conv_f64(xn[c], hn) for c in range(C)])
+    y = out["on"][0].cpu().numpy()
+    assert float(np.max(np.abs(y - yr)) / np.sqrt(np.mean(yr ** 2))) < 1e-5
+    assert relmax(out["on"][1].cpu().numpy(), f64_corr_x(gn, hn)) < 1e-4
+    assert relmax(out["on"][2].cpu().numpy(), f64_corr_h(gn, xn, taps)) < 1e-4
+
+
 def test_forward_and_adjoint_replay_in_a_cuda_graph():
     """A step (forward + adjoint, dL/dx and dL/dh) captured once in a HIP graph
     and replayed with new inputs, output gradients and filter values written
