@@ -330,7 +330,7 @@ k_off_mac(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y
 // once (plus the P-1 blocks before a chunk) instead of once per partition.
 // Terms are summed in partition order, as k_off_mac does.  Grid:
 // (ceil(C N/2 / 256), ceil(M / MC)).
-template <int PW, int DIR>
+template <int PW, int DIR, int U = 8>
 __global__ void __launch_bounds__(256)
 k_off_mac_w(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__ Y, int C, int M, int N, int P,
             int MC)
@@ -352,37 +352,48 @@ k_off_mac_w(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__
     const int m0 = blockIdx.y * MC, m1 = min(M, m0 + MC);
     const int mfirst = (DIR > 0) ? max(0, m0 - (P - 1)) : min(M - 1, m1 - 1 + (P - 1));
     const int nstep = (DIR > 0) ? (m1 - mfirst) : (mfirst - m0 + 1);
-#pragma unroll 2
-    for (int s = 0; s < nstep; s++) {
-        const int m = mfirst + DIR * s;
+    // U block spectra are requested together, then shifted through the window
+    // one by one: U loads in flight per thread instead of one (with cold
+    // caches 63 -> 46 us at the 256 and 2048 levels of 64 clips x 131072)
+    for (int s0 = 0; s0 < nstep; s0 += U) {
+        float4 xl[U];
 #pragma unroll
-        for (int t = PW - 1; t > 0; t--) xw[t] = xw[t - 1];
-        xw[0] = Xr[(size_t)m * half];
-        if ((DIR > 0) ? (m < m0) : (m >= m1)) continue;
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < U; u++)
+            xl[u] = (s0 + u < nstep) ? Xr[(size_t)(mfirst + DIR * (s0 + u)) * half] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int t = 0; t < PW; t++) {
-            // hr[t] is zero for t >= P: those terms add exact zeros
-            const float4 a = xw[t], h = hr[t];
-            if (bin0) {
-                acc.x += a.x * h.x;
-                acc.y += a.y * h.y;
-            } else if (DIR > 0) {
-                acc.x += a.x * h.x - a.y * h.y;
-                acc.y += a.x * h.y + a.y * h.x;
-            } else {
-                acc.x += a.x * h.x + a.y * h.y;
-                acc.y += a.y * h.x - a.x * h.y;
+        for (int u = 0; u < U; u++) {
+            const int s = s0 + u;
+            if (s >= nstep) break;
+            const int m = mfirst + DIR * s;
+#pragma unroll
+            for (int t = PW - 1; t > 0; t--) xw[t] = xw[t - 1];
+            xw[0] = xl[u];
+            if ((DIR > 0) ? (m < m0) : (m >= m1)) continue;
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int t = 0; t < PW; t++) {
+                // hr[t] is zero for t >= P: those terms add exact zeros
+                const float4 a = xw[t], h = hr[t];
+                if (bin0) {
+                    acc.x += a.x * h.x;
+                    acc.y += a.y * h.y;
+                } else if (DIR > 0) {
+                    acc.x += a.x * h.x - a.y * h.y;
+                    acc.y += a.x * h.y + a.y * h.x;
+                } else {
+                    acc.x += a.x * h.x + a.y * h.y;
+                    acc.y += a.y * h.x - a.x * h.y;
+                }
+                if (DIR > 0) {
+                    acc.z += a.z * h.z - a.w * h.w;
+                    acc.w += a.z * h.w + a.w * h.z;
+                } else {
+                    acc.z += a.z * h.z + a.w * h.w;
+                    acc.w += a.w * h.z - a.z * h.w;
+                }
             }
-            if (DIR > 0) {
-                acc.z += a.z * h.z - a.w * h.w;
-                acc.w += a.z * h.w + a.w * h.z;
-            } else {
-                acc.z += a.z * h.z + a.w * h.w;
-                acc.w += a.w * h.z - a.z * h.w;
-            }
+            Yr[(size_t)m * half] = acc;
         }
-        Yr[(size_t)m * half] = acc;
     }
 }
 
@@ -394,7 +405,7 @@ k_off_mac_w(const cf *__restrict__ X, const cf *__restrict__ H, cf *__restrict__
 // (xw[t] = X[c,m-p0-t]; zero before block 0), so a spectrum is read once per
 // group of PT partitions, not once per partition.  A chunk first shifts in the
 // PT-1 blocks before it.  Grid: (ceil(C N/2 / 256), ceil(P / PT), nch).
-template <int PT>
+template <int PT, int U = 4>
 __global__ void __launch_bounds__(256)
 k_off_dh_part(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict__ part, int C, int M, int N,
               int P, int MC)
@@ -415,25 +426,37 @@ k_off_dh_part(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict
         xw[t] = make_float4(0.f, 0.f, 0.f, 0.f);
         acc[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-#pragma unroll 4
-    for (int m = max(p0, m0 - (PT - 1)); m < m1; m++) {
+    // U steps' spectra are requested together, then used one step at a time
+    for (int mb = max(p0, m0 - (PT - 1)); mb < m1; mb += U) {
+        float4 xl[U], gl[U];
 #pragma unroll
-        for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
-        xw[0] = Xr[(size_t)(m - p0) * half];
-        if (m < m0) continue;
-        const float4 g = Gr[(size_t)m * half];
+        for (int u = 0; u < U; u++) {
+            const int m = mb + u;
+            xl[u] = (m < m1) ? Xr[(size_t)(m - p0) * half] : make_float4(0.f, 0.f, 0.f, 0.f);
+            gl[u] = (m >= m0 && m < m1) ? Gr[(size_t)m * half] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
 #pragma unroll
-        for (int t = 0; t < PT; t++) {
-            const float4 x = xw[t];
-            if (bin0) {
-                acc[t].x += g.x * x.x;
-                acc[t].y += g.y * x.y;
-            } else {
-                acc[t].x += g.x * x.x + g.y * x.y;
-                acc[t].y += g.y * x.x - g.x * x.y;
+        for (int u = 0; u < U; u++) {
+            const int m = mb + u;
+            if (m >= m1) break;
+#pragma unroll
+            for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
+            xw[0] = xl[u];
+            if (m < m0) continue;
+            const float4 g = gl[u];
+#pragma unroll
+            for (int t = 0; t < PT; t++) {
+                const float4 x = xw[t];
+                if (bin0) {
+                    acc[t].x += g.x * x.x;
+                    acc[t].y += g.y * x.y;
+                } else {
+                    acc[t].x += g.x * x.x + g.y * x.y;
+                    acc[t].y += g.y * x.x - g.x * x.y;
+                }
+                acc[t].z += g.z * x.z + g.w * x.w;
+                acc[t].w += g.w * x.z - g.z * x.w;
             }
-            acc[t].z += g.z * x.z + g.w * x.w;
-            acc[t].w += g.w * x.z - g.z * x.w;
         }
     }
     float4 *dst = reinterpret_cast<float4 *>(part) + ((size_t)c * nch + j) * half + k2;
@@ -654,7 +677,7 @@ __global__ void k_h_transpose(const cf *__restrict__ src, cf *__restrict__ dst, 
 // them twice for P = 14), with twice the waves in flight.  Packed bin 0 holds
 // the real DC and Nyquist bins.  Same partial-sum layout and summation order
 // as k_off_dh_part.  Grid: (ceil(C N / 256), ceil(P / PT), nch).
-template <int PT>
+template <int PT, int U = 4>
 __global__ void __launch_bounds__(256)
 k_off_dh_part_b(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restrict__ part, int C, int M, int N,
                 int P, int MC)
@@ -674,22 +697,33 @@ k_off_dh_part_b(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restri
         xw[t] = make_float2(0.f, 0.f);
         acc[t] = make_float2(0.f, 0.f);
     }
-#pragma unroll 4
-    for (int m = max(p0, m0 - (PT - 1)); m < m1; m++) {
+    for (int mb = max(p0, m0 - (PT - 1)); mb < m1; mb += U) {
+        cf xl[U], gl[U];
 #pragma unroll
-        for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
-        xw[0] = Xr[(size_t)(m - p0) * N];
-        if (m < m0) continue;
-        const cf g = Gr[(size_t)m * N];
+        for (int u = 0; u < U; u++) {
+            const int m = mb + u;
+            xl[u] = (m < m1) ? Xr[(size_t)(m - p0) * N] : make_float2(0.f, 0.f);
+            gl[u] = (m >= m0 && m < m1) ? Gr[(size_t)m * N] : make_float2(0.f, 0.f);
+        }
 #pragma unroll
-        for (int t = 0; t < PT; t++) {
-            const cf x = xw[t];
-            if (bin0) {
-                acc[t].x += g.x * x.x;
-                acc[t].y += g.y * x.y;
-            } else {
-                acc[t].x += g.x * x.x + g.y * x.y;
-                acc[t].y += g.y * x.x - g.x * x.y;
+        for (int u = 0; u < U; u++) {
+            const int m = mb + u;
+            if (m >= m1) break;
+#pragma unroll
+            for (int t = PT - 1; t > 0; t--) xw[t] = xw[t - 1];
+            xw[0] = xl[u];
+            if (m < m0) continue;
+            const cf g = gl[u];
+#pragma unroll
+            for (int t = 0; t < PT; t++) {
+                const cf x = xw[t];
+                if (bin0) {
+                    acc[t].x += g.x * x.x;
+                    acc[t].y += g.y * x.y;
+                } else {
+                    acc[t].x += g.x * x.x + g.y * x.y;
+                    acc[t].y += g.y * x.x - g.x * x.y;
+                }
             }
         }
     }
