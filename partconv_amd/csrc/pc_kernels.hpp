@@ -368,6 +368,10 @@ struct TickCfg {
     // wavefronts: more instances' tick workgroups fit beside the level kernels)
     static constexpr int NT = (N <= 64) ? 64 : ((N >= 512 || N == 128) ? 256 : 128);
     static constexpr int IPT = (N + NT - 1) / NT;
+    // persistent kernel: waves per SIMD it must keep (<= 512 / MINW VGPRs).  At
+    // N = 128 four 4-wave tick workgroups per CU keep 512 instances within half
+    // of the device (the other half is left to the host-launched levels)
+    static constexpr int MINW = (N == 128) ? 4 : 1;
 };
 
 // Per-thread state of one instance's level 0 that does not depend on the new
@@ -954,6 +958,182 @@ __device__ __forceinline__ void tick64_phaseB(const TickJob &TJ, uint64_t f, cf 
     st.sp = s;
 }
 
+// ---------------------------------------------------------------------------
+// buffer_size 128: level 0 of a tick in one wavefront with the 128-point complex
+// transform in registers, two values per lane (window elements l and l + 64):
+// one decimation-in-frequency radix-2 stage inside the lane, then the 64-point
+// lane-exchange transform on each half, so lane l holds bins 2k and 2k + 1 with
+// k = brev6(l).  The real split pairs bin 2k with 128 - 2k (the 64-point
+// partner lane) and 2k + 1 with 128 - 2k - 1 (a second partner lane).  The
+// inverse runs the 64-point inverse on each half and a final in-lane stage that
+// forms only the kept half of the window (samples 2l, 2l + 1 of the block).
+// H_0, S and the delay line stay in natural bin order, as the other tick
+// kernels and the level kernels lay them out.  Waves 1.. of the workgroup wait
+// at the barrier that hands the output to the publishing threads.
+// ---------------------------------------------------------------------------
+struct TickState128 {
+    W64 w;           // 64-point stage twiddles; w.k = k, w.partner: lane of bin 64 - k
+    cf w1;           // W_128^lane: the in-lane stage
+    cf wsa, wsb;     // split twiddles exp(-i pi b / 128) of bins b = 2k, 2k + 1
+    int partner_b;   // lane holding bin 128 - (2k + 1)
+    cf h0a, h0b, spa, spb;
+    float2 pv;       // previous frame samples 2 lane, 2 lane + 1
+};
+
+__device__ __forceinline__ void tick128_load_state(const LevelJob &J, uint64_t f, int lane, const cf *__restrict__ tw,
+                                                   TickState128 &st)
+{
+    if (lane >= 64) return;
+    w64_setup(st.w, tw, lane);
+    const int ka = 2 * st.w.k, kb = ka + 1;
+    st.w1 = tw[128 + lane];
+    st.wsa = tw[256 + ka];
+    st.wsb = tw[256 + kb];
+    st.partner_b = brev6((63 - st.w.k) & 63);
+    st.h0a = J.H[ka];
+    st.h0b = J.H[kb];
+    st.spa = J.S[ka];
+    st.spb = J.S[kb];
+    const float *prev = J.ring + (((long long)f * 128 - 128) & (long long)J.ring_mask);
+    st.pv = make_float2(prev[2 * lane], prev[2 * lane + 1]);
+}
+
+// Phase A for buffer_size 128 (wave 0 computes; every thread of the workgroup
+// calls it): leaves output sample tid in acc[0] for tid < 128 and X_f of bins
+// 2k, 2k + 1 in xr[0], xr[1] (wave 0).
+__device__ __forceinline__ void tick128_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, float *obuf,
+                                               int tid, TickState128 &st, float *acc, cf *xr,
+                                               const float *llx = nullptr)
+{
+    if (tid < 64) {
+        const int lane = tid;
+        const LevelJob &J = TJ.l0;
+        const long long mask = J.ring_mask;
+        const long long R = mask + 1;
+        const long long t0 = (long long)f * 128;
+        const float *inf = A.in + (long long)(f - A.fbase) * A.in_stride;
+        const uint64_t fr = A.relaxed ? f : f + 1;
+        // loads: the new frame's samples 2 lane, 2 lane + 1 and the level blocks due
+        float2 xv, lev = make_float2(0.f, 0.f), y0p = make_float2(0.f, 0.f);
+        if (llx) {
+            xv = make_float2(llx[0], llx[1]);
+        } else {
+            const float *ic = inf + (long long)TJ.in_ch * A.in_cs;
+            xv.x = ic[(long long)(2 * lane) * A.in_ss];
+            xv.y = ic[(long long)(2 * lane + 1) * A.in_ss];
+        }
+        if (A.relaxed && f > 0)
+            y0p = gload2(reinterpret_cast<const float2 *>(J.ybuf + (size_t)((f - 1) & 1) * 128) + lane);
+        for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
+            float2 v[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int L = L0 + c;
+                const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
+                v[c] = make_float2(0.f, 0.f);
+                if (qb != 0) {
+                    const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
+                    v[c] = gload2(reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] +
+                                                                   sub * 128) + lane);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                lev.x += v[c].x;
+                lev.y += v[c].y;
+            }
+        }
+        // window [previous frame | frame f * scale] (src/partconvMulti.cpp:839-847)
+        const cf z0 = make_float2(st.pv.x, st.pv.y);
+        const cf z1 = make_float2(xv.x * TJ.scale, xv.y * TJ.scale);
+        st.pv = make_float2(z1.x, z1.y);
+        // in-lane DIF stage, then the two 64-point halves
+        const cf a = w64_fwd(cadd(z0, z1), st.w, lane);
+        const cf b = w64_fwd(cmul(csub(z0, z1), st.w1), st.w, lane);
+        const int k = st.w.k;
+        const cf ap = shfl_cf(a, st.w.partner);
+        const cf bp = shfl_cf(b, st.partner_b);
+        cf Xa, Xb, dummy;
+        if (k == 0) Xa = make_float2(a.x + a.y, a.x - a.y);
+        else rsplit_fwd(a, ap, st.wsa, &Xa, &dummy);
+        rsplit_fwd(b, bp, st.wsb, &Xb, &dummy);
+        xr[0] = Xa;
+        xr[1] = Xb;
+        cf Ya = pmul(Xa, st.h0a, k == 0);
+        Ya.x += st.spa.x;
+        Ya.y += st.spa.y;
+        cf Yb = cmul(Xb, st.h0b);
+        Yb.x += st.spb.x;
+        Yb.y += st.spb.y;
+        const cf yap = shfl_cf(Ya, st.w.partner);
+        const cf ybp = shfl_cf(Yb, st.partner_b);
+        cf Ca, Cb;
+        if (k == 0) Ca = make_float2(Ya.x + Ya.y, Ya.x - Ya.y);
+        else rsplit_inv(Ya, yap, st.wsa, &Ca, &dummy);
+        rsplit_inv(Yb, ybp, st.wsb, &Cb, &dummy);
+        const cf ai = w64_inv(Ca, st.w, lane);
+        const cf bi = w64_inv(Cb, st.w, lane);
+        const cf zo = cadd(ai, cmulc(bi, st.w1));   // z'[lane] = (y[2 lane], y[2 lane + 1])
+        // ring write of the scaled frame, after the transforms
+        float *ring = const_cast<float *>(J.ring);
+        const long long p = (t0 + 2 * lane) & mask;
+        gstore2(reinterpret_cast<float2 *>(ring + p), make_float2(z1.x, z1.y));
+        gstore2(reinterpret_cast<float2 *>(ring + p + R), make_float2(z1.x, z1.y));
+        float2 o;
+        if (A.relaxed) {
+            gstore2(reinterpret_cast<float2 *>(J.ybuf + (size_t)(f & 1) * 128) + lane, make_float2(zo.x, zo.y));
+            o = make_float2(y0p.x + lev.x, y0p.y + lev.y);
+        } else {
+            o = make_float2(zo.x + lev.x, zo.y + lev.y);
+        }
+        reinterpret_cast<float2 *>(obuf)[lane] = o;
+    }
+    __syncthreads();
+    if (tid < 128) *acc = obuf[tid];
+}
+
+// Phase B for buffer_size 128 (wave 0): FDL row f % P <- X_f; S for tick f+1
+// (bins 2k, 2k + 1).
+__device__ __forceinline__ void tick128_phaseB(const TickJob &TJ, uint64_t f, int tid, const cf *xr,
+                                               TickState128 &st)
+{
+    if (tid >= 64) return;
+    const LevelJob &J = TJ.l0;
+    const int P = J.P;
+    const int ka = 2 * st.w.k, kb = ka + 1;
+    const int slot = (int)(f % (uint64_t)P);
+    J.fdl[(size_t)slot * 128 + ka] = xr[0];
+    J.fdl[(size_t)slot * 128 + kb] = xr[1];
+    cf sa = (P > 1) ? pmul(xr[0], J.H[128 + ka], ka == 0) : make_float2(0.f, 0.f);
+    cf sb = (P > 1) ? cmul(xr[1], J.H[128 + kb]) : make_float2(0.f, 0.f);
+    // partitions 2..P-1 in batches of 4: loads in flight together, summed in order
+    for (int p0 = 2; p0 < P; p0 += 4) {
+        cf av[4], bv[4], ha[4], hb[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int pp = p0 + c;
+            int r = slot + 1 - pp;  // (f + 1 - p) mod P
+            if (r < 0) r += P;
+            if (pp < P) {
+                av[c] = gload2(J.fdl + (size_t)r * 128 + ka);
+                bv[c] = gload2(J.fdl + (size_t)r * 128 + kb);
+                ha[c] = gload2(J.H + (size_t)pp * 128 + ka);
+                hb[c] = gload2(J.H + (size_t)pp * 128 + kb);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (p0 + c < P) {
+                sa = pmac(sa, av[c], ha[c], ka == 0);
+                sb = pmac(sb, bv[c], hb[c], false);
+            }
+    }
+    J.S[ka] = sa;
+    J.S[kb] = sb;
+    st.spa = sa;
+    st.spb = sb;
+}
+
 template <int LOGN>
 __global__ void __launch_bounds__(TickCfg<LOGN>::NT)
 k_tick(TickArgs A, const cf *__restrict__ tw)
@@ -972,6 +1152,13 @@ k_tick(TickArgs A, const cf *__restrict__ tw)
         tick64_phaseA(TJ, A, A.f, reinterpret_cast<float *>(bufA), tid, st, &acc[0], &xr);
         tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
         tick64_phaseB(TJ, A.f, xr, st);
+    } else if constexpr (LOGN == 7) {
+        TickState128 st;
+        tick128_load_state(TJ.l0, A.f, tid, tw, st);
+        cf xr[2];
+        tick128_phaseA(TJ, A, A.f, reinterpret_cast<float *>(bufA), tid, st, &acc[0], xr);
+        tick_arrive<LOGN>(TJ, A, A.f, tid, acc, &s_last);
+        tick128_phaseB(TJ, A.f, tid, xr, st);
     } else {
         TickState<LOGN> st;
         tick_load_state<LOGN>(TJ.l0, A.f, tid, st);
@@ -1275,7 +1462,7 @@ __device__ __forceinline__ unsigned long long poll_host(const unsigned long long
 // lanes 32..63 of the 64-point form, else IPT.
 template <int LOGN>
 struct LLCnt {
-    static constexpr int v = (LOGN == 6) ? 2 : TickCfg<LOGN>::IPT;
+    static constexpr int v = (LOGN == 6 || LOGN == 7) ? 2 : TickCfg<LOGN>::IPT;
 };
 
 // Host sessions with tagged words: wait until this workgroup's input words of
@@ -1303,6 +1490,9 @@ __device__ __forceinline__ int ll_wait_frame(const TickJob &TJ, const TickArgs &
             if constexpr (LOGN == 6) {
                 valid = tid >= 32;
                 j = 2 * (tid & 31) + q;
+            } else if constexpr (LOGN == 7) {
+                valid = tid < 64;
+                j = 2 * tid + q;
             } else {
                 j = tid + q * NT;
                 valid = j < N;
@@ -1330,7 +1520,7 @@ __device__ __forceinline__ int ll_wait_frame(const TickJob &TJ, const TickArgs &
 }
 
 template <int LOGN>
-__global__ void __launch_bounds__(TickCfg<LOGN>::NT)
+__global__ void __launch_bounds__(TickCfg<LOGN>::NT, TickCfg<LOGN>::MINW)
 k_persist(PersistArgs P, const cf *__restrict__ tw)
 {
     constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
@@ -1357,7 +1547,9 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     __syncthreads();
     TickState<LOGN> st;
     TickState64 st64;
+    TickState128 st128;
     if constexpr (LOGN == 6) tick64_load_state(sTJ.l0, P.f0, tid, tw, st64);
+    else if constexpr (LOGN == 7) tick128_load_state(sTJ.l0, P.f0, tid, tw, st128);
     else tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
@@ -1423,6 +1615,13 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
             tick64_phaseB(sTJ, f, xr, st64);
+        } else if constexpr (LOGN == 7) {
+            cf xr[2];
+            tick128_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st128, &acc[0], xr,
+                           P.A.ll ? llx : nullptr);
+            tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
+            if (tr) trace_stamp(P.A, f, tslot + 1);
+            tick128_phaseB(sTJ, f, tid, xr, st128);
         } else {
             cf xr[IPT];
             tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr, P.A.ll ? llx : nullptr);
