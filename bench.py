@@ -608,6 +608,7 @@ def bench_diff(args):
         step()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
+    cgraphs = conv.call_graph_stats()
     # each kernel of each level, timed by HIP events its own launch stamps,
     # over a few more steps, with the levels on one stream (each kernel alone
     # on the device)
@@ -621,7 +622,15 @@ def bench_diff(args):
             torch.cuda.synchronize()
             probes[(kind, nl)] = conv.probe_read()
             conv.probe(-1)
-    # the eager step with every level on its own stream, for comparison
+    # without the library's call graphs: every launch eager, the levels on one
+    # stream, then each level on its own stream
+    conv.set_call_graphs(False)
+    step()
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    secs_eager = time.perf_counter() - t3
     conv.set_level_streams("on")
     step()
     t2 = time.perf_counter()
@@ -630,6 +639,7 @@ def bench_diff(args):
     torch.cuda.synchronize()
     secs_par = time.perf_counter() - t2
     conv.set_level_streams("captured")
+    conv.set_call_graphs(True)
     # the same step captured once in a HIP graph and replayed (the MSE gradient
     # formed elementwise, 2 (y - tgt) / numel: no reduction inside the graph)
     scale = 2.0 / (C * T)
@@ -735,10 +745,16 @@ def bench_diff(args):
         "long_ir_recompute": {"taps": taps2, "partition": "[256x14, 2048x14, 16384x63]",
                               "ms_per_step": 1e3 * secs2 / args.steps,
                               "recomputed_levels": int(conv2.recomputed_levels)},
+        "call_graphs": {"forward_replays_captures": list(cgraphs["forward"]),
+                        "backward_replays_captures": list(cgraphs["backward"]),
+                        "note": "value: the library's default, each pcc_forward / pcc_backward call replayed "
+                                "from a HIP graph of its own launches (levels as parallel branches) once its "
+                                "arguments repeat; counts over warm-up + timed steps"},
+        "eager_no_call_graphs": {"ms_per_step": 1e3 * secs_eager / args.steps,
+                                 "note": "the same step with call graphs off: every launch eager, every level "
+                                         "on the caller's stream; bitwise the same results"},
         "eager_level_streams": {"ms_per_step": 1e3 * secs_par / args.steps,
-                                "note": "the same eager step with each level on its own stream (value: every "
-                                        "level on the caller's stream, the default for eager calls; the graph "
-                                        "replay below runs the levels as parallel branches); bitwise the same "
+                                "note": "call graphs off, each level on its own stream; bitwise the same "
                                         "results"},
         "tuned_chunks": {"levels": list(conv.N), "mac_dh_blocks_per_chunk": tuning,
                          "from_cache": tinfo["from_cache"]},

@@ -398,6 +398,21 @@ int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, fl
 #define PCC_LEVEL_STREAMS_ON 1
 #define PCC_LEVEL_STREAMS_CAPTURED 2
 int pcc_set_level_streams(pc_conv *c, int mode);
+/* Call graphs (no reference counterpart; on by default): the second
+ * pcc_forward (pcc_backward) call with the same pointers, clip count, length
+ * and kept-spectra state as the previous eager one captures its launches into
+ * a HIP graph on a private stream and launches the graph on the caller's
+ * stream; later calls with those arguments replay it (up to 4 graphs per
+ * direction, most recently used kept).  A graph is replayed only while every
+ * buffer, chunk size and level-stream mode it embeds is unchanged.  Calls
+ * while a kernel probe is set or while the caller's stream is itself being
+ * captured always launch eagerly.  Results are bitwise the same as eager
+ * calls.  on = 0 turns them off and frees the graphs. */
+int pcc_set_call_graphs(pc_conv *c, int on);
+/* Replays and captures so far of the forward's and the backward's call graphs
+ * (any pointer may be null). */
+int pcc_call_graph_stats(pc_conv *c, long long *fwd_replays, long long *fwd_captures, long long *bwd_replays,
+                         long long *bwd_captures);
 /* Memory budget (bytes) for keeping every level's input spectra from forward
  * for backward; above it backward recomputes them from the stored input. */
 int pcc_set_budget(pc_conv *c, long long bytes);

@@ -116,6 +116,8 @@ SIGNATURES = {
     "pcc_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcc_probe_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "pcc_set_level_streams": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcc_set_call_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcc_call_graph_stats": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "pcc_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_double)]),
     "pcc_destroy": (None, [_vp]),

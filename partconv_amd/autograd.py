@@ -85,6 +85,18 @@ class PartitionedConv:
             mode = "on" if mode else "off"
         self._check(lib().pcc_set_level_streams(self._h, self.LEVEL_STREAMS[mode]))
 
+    def set_call_graphs(self, on):
+        """Replay each forward / backward call with the same arguments as an
+        earlier one from a HIP graph of its launches (default on; bitwise the
+        same results)."""
+        self._check(lib().pcc_set_call_graphs(self._h, 1 if on else 0))
+
+    def call_graph_stats(self):
+        """{"forward": (replays, captures), "backward": (replays, captures)}."""
+        v = [ctypes.c_longlong() for _ in range(4)]
+        self._check(lib().pcc_call_graph_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return {"forward": (v[0].value, v[1].value), "backward": (v[2].value, v[3].value)}
+
     def tuning(self):
         """[(mac_chunk, dh_chunk)] per level: the chunk sizes in use."""
         out = []

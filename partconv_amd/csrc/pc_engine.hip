@@ -3111,6 +3111,26 @@ struct ConvLevelWork {
     size_t Yc = 0, Gc = 0, Wc = 0, Xrc = 0, partc = 0, Dc = 0, scrc = 0;
 };
 
+// A forward or backward call's launch sequence captured as a graph and
+// replayed by later calls with the same arguments: key (the call's pointers),
+// shape, flags (kept spectra) and fingerprint (every buffer, chunk size and
+// mode the captured launches embed) must all match.
+struct CallGraph {
+    hipGraphExec_t exec = nullptr;
+    const void *key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int C = 0, T = 0, flags = 0;
+    std::vector<uintptr_t> fp;
+};
+
+// Graphs of one direction and the last eager calls' arguments, most recent
+// first: a call is captured the second time its arguments are seen.
+#define PC_CALL_GRAPHS 4
+struct CallGraphs {
+    bool bwd = false;
+    std::vector<CallGraph> g, seen;
+    long long replays = 0, captures = 0;
+};
+
 struct pc_conv {
     int dev = 0;
     int taps = 0;
@@ -3118,6 +3138,9 @@ struct pc_conv {
     std::vector<ConvLevelWork> lw;
     hipEvent_t fork_ev = nullptr;
     int level_streams = PCC_LEVEL_STREAMS_CAPTURED;
+    bool call_graphs = true;
+    CallGraphs fg, bg;  // forward, backward
+    hipStream_t cap_s = nullptr;
     cf *tw = nullptr;
     // filter state
     const float *h_src = nullptr;
@@ -3391,6 +3414,7 @@ static pc_conv *conv_create(int M, const int *N, const int *P, int taps)
             return nullptr;
         }
     pc_conv *c = new pc_conv();
+    c->bg.bwd = true;
     if (hipGetDevice(&c->dev) != hipSuccess || get_twiddles(c->dev, &c->tw) != PC_OK) {
         std::string msg = g_err.empty() ? "no HIP device" : g_err;
         delete c;
@@ -3511,6 +3535,19 @@ static InvDst conv_ydst(const ConvLevel &L, float *y, int T, int M, bool first)
     return D;
 }
 
+// Create every level's stream and events (once per handle; stream creation
+// costs milliseconds, so the first eager call with call graphs on does it
+// rather than the capture that follows).
+static int conv_level_streams_create(pc_conv *c)
+{
+    if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    for (ConvLevelWork &w : c->lw) {
+        if (!w.s) HIPCHK(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
+        if (!w.ev) HIPCHK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+    }
+    return PC_OK;
+}
+
 // The stream each level of a call runs on: with level streams on (or, by
 // default, while the caller's stream is being captured into a graph) and
 // more than one level, the level's own stream, made to wait for the work
@@ -3530,12 +3567,11 @@ static int conv_fork(pc_conv *c, hipStream_t s, std::vector<hipStream_t> &st, bo
     }
     *par = on && G > 1;
     if (!*par) return PC_OK;
-    if (!c->fork_ev) HIPCHK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    int rc = conv_level_streams_create(c);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(c->fork_ev, s));
     for (size_t li = 0; li < G; li++) {
         ConvLevelWork &w = c->lw[li];
-        if (!w.s) HIPCHK(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking));
-        if (!w.ev) HIPCHK(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
         HIPCHK(hipStreamWaitEvent(w.s, c->fork_ev, 0));
         st[li] = w.s;
     }
@@ -3553,17 +3589,140 @@ static int conv_level_scr(const ConvLevel &L, ConvLevelWork &w, size_t n, cf **s
     return PC_OK;
 }
 
-// y = sum over levels of the partitioned convolution of every clip (x, y: [C][T])
-PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
+// Everything a captured forward (bwd false) or backward call embeds besides
+// its arguments: the buffers, chunk sizes and mode its launches use.
+static std::vector<uintptr_t> conv_fingerprint(const pc_conv *c, bool bwd)
 {
-    if (!c || !d_x || !d_y || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
-    if (!c->filter_ready) return fail(PC_ERR_STATE, "forward before set_filter");
-    hipStream_t s = (hipStream_t)stream;
+    std::vector<uintptr_t> fp;
+    fp.push_back((uintptr_t)c->level_streams);
+    fp.push_back((uintptr_t)c->tw);
+    for (size_t li = 0; li < c->lv.size(); li++) {
+        const ConvLevel &L = c->lv[li];
+        const ConvLevelWork &w = c->lw[li];
+        for (const void *q : {(const void *)L.H, (const void *)c->keep[li], (const void *)w.Xr, (const void *)w.scr})
+            fp.push_back((uintptr_t)q);
+        fp.push_back((uintptr_t)L.mac_mc);
+        if (bwd) {
+            for (const void *q : {(const void *)w.G, (const void *)w.W, (const void *)w.part, (const void *)w.D})
+                fp.push_back((uintptr_t)q);
+            fp.push_back((uintptr_t)L.dh_mc);
+        } else {
+            fp.push_back((uintptr_t)w.Y);
+        }
+    }
+    return fp;
+}
+
+static bool call_graph_match(const CallGraph &g, const void *const key[4], int C, int T, int flags,
+                             const std::vector<uintptr_t> &fp)
+{
+    if (g.C != C || g.T != T || g.flags != flags) return false;
+    for (int i = 0; i < 4; i++)
+        if (g.key[i] != key[i]) return false;
+    return g.fp == fp;
+}
+
+static void call_graph_set(CallGraph &g, const void *const key[4], int C, int T, int flags,
+                           std::vector<uintptr_t> fp)
+{
+    for (int i = 0; i < 4; i++) g.key[i] = key[i];
+    g.C = C;
+    g.T = T;
+    g.flags = flags;
+    g.fp = std::move(fp);
+}
+
+static void call_graphs_drop(CallGraphs &G)
+{
+    for (CallGraph &g : G.g)
+        if (g.exec) hipGraphExecDestroy(g.exec);
+    G.g.clear();
+    G.seen.clear();
+}
+
+// Capture launch(stream) on the handle's private stream into an executable
+// graph (null on any failure, with the error state left as it was).  The
+// launches find every buffer already sized by the eager call with the same
+// arguments, so capture allocates nothing.
+template <typename F>
+static hipGraphExec_t call_graph_capture(pc_conv *c, F launch)
+{
+    if (!c->cap_s && hipStreamCreateWithFlags(&c->cap_s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        c->cap_s = nullptr;
+        return nullptr;
+    }
+    const std::string err = g_err;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(c->cap_s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int rc = launch(c->cap_s);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->cap_s, &graph);
+        if (rc != PC_OK || e != hipSuccess || !graph ||
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            exec = nullptr;
+        if (graph) hipGraphDestroy(graph);
+    }
+    (void)hipGetLastError();
+    g_err = err;
+    return exec;
+}
+
+// Run one forward or backward call's launches on s: replay its graph if one
+// was captured for these arguments, capture one if one of the last eager
+// calls had them, else launch eagerly.  Bypassed (always eager) when call graphs
+// are off, a kernel probe is set, or s is itself being captured.
+template <typename F>
+static int call_graph_run(pc_conv *c, CallGraphs &G, hipStream_t s, const void *const key[4], int C, int T,
+                          int flags, F launch)
+{
+    if (!c->call_graphs || c->probe_level >= 0) return launch(s);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return launch(s);
+    std::vector<uintptr_t> fp = conv_fingerprint(c, G.bwd);
+    for (size_t i = 0; i < G.g.size(); i++)
+        if (call_graph_match(G.g[i], key, C, T, flags, fp)) {
+            HIPCHK(hipGraphLaunch(G.g[i].exec, s));
+            std::rotate(G.g.begin(), G.g.begin() + i, G.g.begin() + i + 1);
+            G.replays++;
+            return PC_OK;
+        }
+    size_t si = 0;
+    while (si < G.seen.size() && !call_graph_match(G.seen[si], key, C, T, flags, fp)) si++;
+    if (si < G.seen.size()) {
+        G.seen.erase(G.seen.begin() + si);
+        hipGraphExec_t exec = call_graph_capture(c, launch);
+        if (exec) {
+            HIPCHK(hipGraphLaunch(exec, s));
+            CallGraph g;
+            g.exec = exec;
+            call_graph_set(g, key, C, T, flags, std::move(fp));
+            G.g.insert(G.g.begin(), std::move(g));
+            if (G.g.size() > PC_CALL_GRAPHS) {
+                hipGraphExecDestroy(G.g.back().exec);
+                G.g.pop_back();
+            }
+            G.captures++;
+            return PC_OK;
+        }
+    }
+    int rc = launch(s);
+    if (rc) return rc;
+    if (c->level_streams != PCC_LEVEL_STREAMS_OFF && c->lv.size() > 1 && (rc = conv_level_streams_create(c)))
+        return rc;
+    CallGraph e;
+    call_graph_set(e, key, C, T, flags, conv_fingerprint(c, G.bwd));
+    G.seen.insert(G.seen.begin(), std::move(e));
+    if (G.seen.size() > PC_CALL_GRAPHS) G.seen.pop_back();
+    return PC_OK;
+}
+
+// The forward's launches on stream s (levels forked from s when level streams
+// apply, joined back at the end); sizes the work space on first use.
+static int conv_forward_launch(pc_conv *c, const float *d_x, float *d_y, int C, int T, hipStream_t s, bool keep)
+{
     int rc = PC_OK;
-    size_t need_keep = 0;
-    for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
-    const bool keep = (long long)need_keep <= c->budget;
-    c->keep_valid = false;
     std::vector<hipStream_t> st;
     bool par;
     if ((rc = conv_fork(c, s, st, &par))) return rc;
@@ -3603,6 +3762,24 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
         if (par) HIPCHK(hipEventRecord(w.ev, ls));
     }
     if (par) HIPCHK(hipStreamWaitEvent(s, c->lw[c->lv.size() - 1].ev, 0));
+    return PC_OK;
+}
+
+// y = sum over levels of the partitioned convolution of every clip (x, y: [C][T])
+PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, void *stream)
+{
+    if (!c || !d_x || !d_y || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "forward before set_filter");
+    hipStream_t s = (hipStream_t)stream;
+    size_t need_keep = 0;
+    for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
+    const bool keep = (long long)need_keep <= c->budget;
+    c->keep_valid = false;
+    const void *key[4] = {d_x, d_y, nullptr, nullptr};
+    const int rc = call_graph_run(c, c->fg, s, key, C, T, keep ? 1 : 0, [&](hipStream_t ls) {
+        return conv_forward_launch(c, d_x, d_y, C, T, ls, keep);
+    });
+    if (rc) return rc;
     if (keep) {
         c->keep_valid = true;
         c->keep_x = d_x;
@@ -3680,18 +3857,12 @@ PC_API int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, 
     return PC_OK;
 }
 
-// Adjoint of pcc_forward for upstream gradient g: dx (may be null) and dh (may
-// be null).  Uses the forward's kept spectra when they are for the same x and
-// within the budget, else recomputes them from x.
-PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
-                        void *stream)
+// The adjoint's launches on stream s (see conv_forward_launch); kept: the
+// forward's spectra of x are valid, else they are recomputed.
+static int conv_backward_launch(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C,
+                                int T, hipStream_t s, bool kept)
 {
-    if (!c || !d_x || !d_g || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
-    if (!c->filter_ready) return fail(PC_ERR_STATE, "backward before set_filter");
-    hipStream_t s = (hipStream_t)stream;
-    int rc;
-    if (d_dx && (rc = conv_fill_zero(d_dx, (long long)C * T, s))) return rc;
-    const bool kept = c->keep_valid && c->keep_x == d_x && c->keep_C == C && c->keep_T == T;
+    int rc = PC_OK;
     std::vector<hipStream_t> st;
     bool par;
     if ((rc = conv_fork(c, s, st, &par))) return rc;
@@ -3724,7 +3895,6 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
                 if ((rc = conv_grow(&w.Xr, &w.Xrc, n))) return rc;
                 X = w.Xr;
                 if ((rc = off_rfft(c, L, conv_xwin(L, d_x, T), C, X, ls, scr))) return rc;
-                c->recomputed_levels++;
             }
             if ((rc = conv_grow(&w.D, &w.Dc, (size_t)L.P * L.N))) return rc;
             hipEvent_t e0, e1;
@@ -3773,6 +3943,50 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
     }
     if (par)
         for (size_t li = 0; li < c->lv.size(); li++) HIPCHK(hipStreamWaitEvent(s, c->lw[li].ev, 0));
+    return PC_OK;
+}
+
+// Adjoint of pcc_forward for upstream gradient g: dx (may be null) and dh (may
+// be null).  Uses the forward's kept spectra when they are for the same x and
+// within the budget, else recomputes them from x.
+PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
+                        void *stream)
+{
+    if (!c || !d_x || !d_g || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->filter_ready) return fail(PC_ERR_STATE, "backward before set_filter");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (d_dx && (rc = conv_fill_zero(d_dx, (long long)C * T, s))) return rc;
+    const bool kept = c->keep_valid && c->keep_x == d_x && c->keep_C == C && c->keep_T == T;
+    const void *key[4] = {d_x, d_g, d_dx, d_dh};
+    if ((rc = call_graph_run(c, c->bg, s, key, C, T, kept ? 1 : 0, [&](hipStream_t ls) {
+             return conv_backward_launch(c, d_x, d_g, d_dx, d_dh, C, T, ls, kept);
+         })))
+        return rc;
+    if (d_dh && !kept) c->recomputed_levels += (long long)c->lv.size();
+    return PC_OK;
+}
+
+PC_API int pcc_set_call_graphs(pc_conv *c, int on)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    c->call_graphs = on != 0;
+    if (!c->call_graphs) {
+        call_graphs_drop(c->fg);
+        call_graphs_drop(c->bg);
+    }
+    return PC_OK;
+}
+
+// Replays and captures of the forward's and the backward's call graphs.
+PC_API int pcc_call_graph_stats(pc_conv *c, long long *fwd_replays, long long *fwd_captures, long long *bwd_replays,
+                                long long *bwd_captures)
+{
+    if (!c) return fail(PC_ERR_ARG, "null handle");
+    if (fwd_replays) *fwd_replays = c->fg.replays;
+    if (fwd_captures) *fwd_captures = c->fg.captures;
+    if (bwd_replays) *bwd_replays = c->bg.replays;
+    if (bwd_captures) *bwd_captures = c->bg.captures;
     return PC_OK;
 }
 
@@ -3924,6 +4138,9 @@ static void conv_destroy(pc_conv *c)
         if (w.s) hipStreamDestroy(w.s);
     }
     if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    call_graphs_drop(c->fg);
+    call_graphs_drop(c->bg);
+    if (c->cap_s) hipStreamDestroy(c->cap_s);
     hipFree(c->work);
     hipFree(c->scr);
     hipFree(c->dpart);

@@ -251,6 +251,65 @@ def test_level_streams_bitwise_equal_to_one_stream(budget):
     assert relmax(out["on"][2].cpu().numpy(), f64_corr_h(gn, xn, taps)) < 1e-4
 
 
+@pytest.mark.parametrize("budget", [1 << 30, 0])
+def test_call_graphs_bitwise_equal_to_eager(budget):
+    """pcc_set_call_graphs: forward and adjoint calls replayed from the
+    library's own graphs (captured on the second call with the same
+    arguments) give y, dL/dx and dL/dh bitwise equal to eager calls on the
+    same values, with new values written into the same tensors between calls,
+    with pointers alternating between two sets (two graphs per direction),
+    after a level-stream mode change, a smaller clip count and a larger one
+    (buffers grown: the old graphs are never replayed)."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    taps = 70000
+    N, P = [256, 2048, 32768], [14, 30, 0]
+    conv = PartitionedConv(N, P, taps, budget_bytes=budget)
+    ref = PartitionedConv(N, P, taps, budget_bytes=budget)
+    ref.set_call_graphs(False)
+
+    def call(cv, x, h, g, y, dx, dh):
+        gen = cv.forward_raw(x, h, y)
+        cv.backward_raw(x, h, g, dx, dh, gen)
+
+    held = []  # every buffer set stays alive, so no address is reused
+
+    def bufs(C, T):
+        held.append([torch.empty((C, T), device="cuda") for _ in range(4)] + [torch.empty(taps, device="cuda")])
+        return held[-1]
+
+    def check(C, T, sets, calls):
+        h = torch.empty(taps, device="cuda")
+        for k in range(calls):
+            x, g, y, dx, dh = sets[k % len(sets)]
+            x.copy_(torch.randn((C, T), device="cuda"))
+            g.copy_(torch.randn((C, T), device="cuda"))
+            h.copy_(torch.from_numpy(decaying_noise_ir(taps, 0.3, 48000, 11 + k)).cuda())
+            call(conv, x, h, g, y, dx, dh)
+            r = [torch.empty_like(t) for t in (y, dx, dh)]
+            call(ref, x, h, g, *r)
+            torch.cuda.synchronize()
+            for a, b, name in zip((y, dx, dh), r, ("y", "dx", "dh")):
+                assert torch.equal(a, b), (C, T, k, name)
+
+    C, T = 3, 90000
+    check(C, T, [bufs(C, T)], 5)
+    st = conv.call_graph_stats()
+    assert st == {"forward": (3, 1), "backward": (3, 1)}, st
+    check(C, T, [bufs(C, T), bufs(C, T)], 8)  # alternating: each set captured on its second call
+    st = conv.call_graph_stats()
+    assert st == {"forward": (7, 3), "backward": (7, 3)}, st
+    conv.set_level_streams("on")
+    check(C, T, [bufs(C, T)], 3)
+    conv.set_level_streams("captured")
+    check(2, 50000, [bufs(2, 50000)], 3)
+    check(4, 120000, [bufs(4, 120000)], 3)
+    conv.set_call_graphs(False)
+    before = conv.call_graph_stats()
+    check(C, T, [bufs(C, T)], 3)
+    assert conv.call_graph_stats() == before
+
+
 def test_forward_and_adjoint_replay_in_a_cuda_graph():
     """A step (forward + adjoint, dL/dx and dL/dh) captured once in a HIP graph
     and replayed with new inputs, output gradients and filter values written
