@@ -630,6 +630,7 @@ struct Engine {
     std::vector<WorkJob> wjobs_h;
     WorkJob *d_wjobs = nullptr;
     int nwjobs = 0, lds_w = 0;        // worker-level jobs, largest worker block size
+    int tw_lds = 0;                   // persistent runs with workers: twiddle-table prefix staged in LDS
     int nworkers = -1;                // worker workgroups (-1: not planned yet)
     bool self_levels = false;         // no room for workers: tick workgroups run those levels
     long long persist_budget = 0;     // workgroups a persistent kernel may take (half the device)
@@ -1778,7 +1779,8 @@ static hipError_t launch_persist_t(Engine &E, PersistArgs &pa, hipStream_t s, in
 static hipError_t launch_persist(Engine &E, PersistArgs &p, hipStream_t s, int lds_n, bool *plain)
 {
     const int nblocks = p.A.njobs + p.nworkers;
-    const size_t lds = (size_t)2 * lds_n * sizeof(cf);
+    p.tw_lds_off = 2 * lds_n;
+    const size_t lds = (size_t)(2 * lds_n + p.tw_lds) * sizeof(cf);
     switch (ilog2(E.N0)) {
     case 4: return launch_persist_t<4>(E, p, s, nblocks, lds, plain);
     case 5: return launch_persist_t<5>(E, p, s, nblocks, lds, plain);
@@ -1828,7 +1830,18 @@ static int plan_workers(Engine &E)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, E.dev));
     int nb = 0;
-    HIPCHK(persist_occupancy(E.N0, &nb, (size_t)2 * std::max(E.N0, E.lds_w) * sizeof(cf)));
+    const size_t lds_buf = (size_t)2 * std::max(E.N0, E.lds_w) * sizeof(cf);
+    HIPCHK(persist_occupancy(E.N0, &nb, lds_buf));
+    // the level blocks' twiddles (tw[n .. 2n) per transform size n <= lds_w and
+    // the real-split entries tw[2n .. 2.5n]) staged in LDS, unless that would
+    // lower the number of workgroups a CU holds
+    E.tw_lds = 0;
+    if (E.nwjobs > 0) {
+        const int n = 2 * E.lds_w + E.lds_w / 2 + 1;
+        int nb2 = 0;
+        if (persist_occupancy(E.N0, &nb2, lds_buf + (size_t)n * sizeof(cf)) == hipSuccess && nb2 >= nb) E.tw_lds = n;
+        (void)hipGetLastError();
+    }
     const long long budget = (long long)prop.multiProcessorCount * nb / 2;
     const long long ntick = (long long)E.groups[0].members.size();
     E.persist_budget = budget;
@@ -2025,6 +2038,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         P.self_levels = E.self_levels ? 1 : 0;
         P.dev_tick = E.d_dev_tick;
         P.lvl_done = E.d_lvl_done;
+        P.tw_lds = E.tw_lds;
         set_masks(true, E.nworkers > 0);
         le = launch_persist(E, P, E.s_main, std::max(E.N0, E.lds_w), &plain);
         if (le != hipSuccess) (void)hipGetLastError();  // too many workgroups to co-reside: no workers
@@ -2036,6 +2050,7 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
         P.nworkers = 0;
         P.self_levels = 0;
         P.dev_tick = P.lvl_done = nullptr;
+        P.tw_lds = 0;
         set_masks(false, false);
         le = launch_persist(E, P, E.s_main, E.N0, &plain);
     }

@@ -250,6 +250,20 @@ __device__ __forceinline__ float4 dl_mac2(float4 acc, float4 a, float4 h, bool b
     return acc;
 }
 
+// One bin of dl_mac2: the same fused multiply-adds in the same order, so a sum
+// formed bin by bin rounds exactly as the float4 form.
+__device__ __forceinline__ cf dl_mac1(cf acc, cf a, cf h, bool bin0)
+{
+    if (bin0) {
+        acc.x = fmaf(a.x, h.x, acc.x);
+        acc.y = fmaf(a.y, h.y, acc.y);
+    } else {
+        acc.x = fmaf(-a.y, h.y, fmaf(a.x, h.x, acc.x));
+        acc.y = fmaf(a.y, h.x, fmaf(a.x, h.y, acc.y));
+    }
+    return acc;
+}
+
 // Block output spectrum Y[k] = X[k] H_0[k] + S[k], where S (written by
 // k_partial after the previous block) holds sum_{p=1}^{P-1} X_{b-p}[k] H_p[k].
 // Stores X into FDL row b % P for later blocks.  X and Y alias.
@@ -1222,6 +1236,8 @@ struct PersistArgs {
     unsigned long long *lvl_done;  // per worker slot: 2m + 1 (block m out), 2m + 2 (S of m + 1 too)
     unsigned *present;             // plain (non-cooperative) launch: workgroup w stores present_tag
     unsigned present_tag;          // into present[w] (host-visible) on entry; null otherwise
+    int tw_lds;                    // twiddle-table entries tw[0 .. tw_lds) copied into LDS at entry (0: none)
+    int tw_lds_off;                // their offset (float2) in the dynamic LDS, after the block buffers
 };
 
 // Residency stamp of a persistent kernel launched without the cooperative
@@ -1275,21 +1291,60 @@ __device__ __forceinline__ int spin_geq(const unsigned long long *p, unsigned lo
 
 // Block m of one worker-level job (window -> r2c -> FDL slot -> X H_0 + S ->
 // c2r -> double-buffer half m & 1), all in LDS, by one workgroup.
+//
+// With P <= 2 the delay-line sum of block m + 1 is X_m H_1 (P = 2) or zero
+// (P = 1): it is formed here from X_m in registers, with the float4 kernels'
+// rounding (dl_mac1), instead of reading X_m back from the delay line after
+// the block -- where H_1's registers fit beside the persistent kernel's tick
+// state (at most 2 bins per thread).  Returns true when S for block m + 1 was
+// written.
 template <int LOGNL, int NT>
-__device__ __forceinline__ void level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf *bufB, const cf *__restrict__ tw,
+__device__ __forceinline__ constexpr bool block_forms_partial(int P)
+{
+    return ((1 << LOGNL) + NT - 1) / NT <= 2 && P <= 2;
+}
+
+template <int LOGNL, int NT>
+__device__ __forceinline__ bool level_block(const LevelJob &J, uint64_t m, int N0, cf *bufA, cf *bufB, const cf *__restrict__ tw,
                             int tid)
 {
     constexpr int N = 1 << LOGNL, IPT = (N + NT - 1) / NT;
-    cf h[IPT], sv[IPT];
+    cf h[IPT], sv[IPT], h1[IPT];
+    const int P = J.P;
+    const bool fuse = block_forms_partial<LOGNL, NT>(P);
     small_prefetch<LOGNL, NT>(J, tid, h, sv);
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int k = tid + q * NT;
+        h1[q] = (IPT <= 2 && P == 2 && k < N) ? J.H[N + k] : make_float2(0.f, 0.f);
+    }
     __syncthreads();  // the LDS buffers may still be read by the previous user
     small_load_window<LOGNL, NT>(bufA, J, PC_MODE_RUN, (long long)m * N - N0 - J.T, tid);
     cf *X = small_forward<LOGNL, NT>(bufA, bufB, tw, tid);
     cf *other = (X == bufA) ? bufB : bufA;
-    small_mac<LOGNL, NT>(X, J, m, tid, h, sv);
+    if (fuse) {
+        const int slot = (int)(m % (uint64_t)P);
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int k = tid + q * NT;
+            if (k < N) {
+                const cf x = X[k];
+                J.fdl[(size_t)slot * N + k] = x;
+                cf y = pmul(x, h[q], k == 0);
+                y.x += sv[q].x;
+                y.y += sv[q].y;
+                X[k] = y;
+                J.S[k] = dl_mac1(make_float2(0.f, 0.f), x, h1[q], k == 0);
+            }
+        }
+        __syncthreads();
+    } else {
+        small_mac<LOGNL, NT>(X, J, m, tid, h, sv);
+    }
     cf *z = small_inverse<LOGNL, NT>(X, other, tw, tid);
     cf *yo = reinterpret_cast<cf *>(J.ybuf + (size_t)(m & 1) * N);
     for (int n = tid; n < N / 2; n += NT) yo[n] = z[n];
+    return fuse;
 }
 
 // One level worker: every block of its jobs whose start tick lies in the run.
@@ -1321,7 +1376,7 @@ __device__ __forceinline__ void level_worker(const PersistArgs &P, int2 range, c
         }
         // delay-line sums for the next blocks, off the deadline path
         for (int q = range.x; q < range.x + range.y; q++) {
-            wg_partial<NT>(P.wjobs[q].J, N, m + 1, tid);
+            if (!block_forms_partial<LOGNL, NT>(P.wjobs[q].J.P)) wg_partial<NT>(P.wjobs[q].J, N, m + 1, tid);
             wg_stores_done();
             if (tid == 0) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1340,24 +1395,27 @@ __device__ __forceinline__ void self_level(const PersistArgs &P, int q, uint64_t
     constexpr int NT = TickCfg<LOGN>::NT;
     const WorkJob &W = P.wjobs[q];
     const int d = W.logN - LOGN;
+    bool formed = false;  // S for block m + 1 written by the block itself
     if constexpr (worker_fits<LOGN + 1, TickCfg<LOGN>::NT>()) {
         if (d == 1) {
-            level_block<LOGN + 1, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 1)), tw, tid);
+            formed = level_block<LOGN + 1, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 1)), tw, tid);
         }
     }
     if constexpr (worker_fits<LOGN + 2, TickCfg<LOGN>::NT>()) {
         if (d == 2) {
-            level_block<LOGN + 2, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 2)), tw, tid);
+            formed = level_block<LOGN + 2, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 2)), tw, tid);
         }
     }
     if constexpr (worker_fits<LOGN + 3, TickCfg<LOGN>::NT>()) {
         if (d == 3) {
-            level_block<LOGN + 3, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 3)), tw, tid);
+            formed = level_block<LOGN + 3, NT>(W.J, m, P.A.N0, lds, lds + (1 << (LOGN + 3)), tw, tid);
         }
     }
-    if (tid == 0) __hip_atomic_store(P.lvl_done + q, 2 * m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    wg_partial<NT>(W.J, 1 << W.logN, m + 1, tid);
+    if (!formed) {
+        if (tid == 0) __hip_atomic_store(P.lvl_done + q, 2 * m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        wg_partial<NT>(W.J, 1 << W.logN, m + 1, tid);
+    }
     __syncthreads();
     if (tid == 0) __hip_atomic_store(P.lvl_done + q, 2 * m + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1412,9 +1470,10 @@ __device__ __forceinline__ void worker_fix(const WorkJob &W, unsigned long long 
         __syncthreads();
     }
     for (uint64_t m = next; m < m_end; m++) {
-        level_block<LOGNL, NT>(W.J, m, N0, lds, lds + N, tw, tid);
-        __syncthreads();
-        wg_partial<NT>(W.J, N, m + 1, tid);
+        if (!level_block<LOGNL, NT>(W.J, m, N0, lds, lds + N, tw, tid)) {
+            __syncthreads();
+            wg_partial<NT>(W.J, N, m + 1, tid);
+        }
         __syncthreads();
     }
     if (tid == 0 && (next < m_end || (d & 1))) *done = 2 * m_end;
@@ -1524,17 +1583,29 @@ __global__ void __launch_bounds__(TickCfg<LOGN>::NT, TickCfg<LOGN>::MINW)
 k_persist(PersistArgs P, const cf *__restrict__ tw)
 {
     constexpr int N = TickCfg<LOGN>::N, NT = TickCfg<LOGN>::NT, IPT = TickCfg<LOGN>::IPT;
-    // 2 x max(N0, largest worker block) float2, sized at launch
+    // 2 x max(N0, largest worker block) float2, sized at launch; then, with
+    // level workers, the twiddle-table prefix the level blocks read
     extern __shared__ cf pc_lds[];
     __shared__ unsigned s_last;
     __shared__ int s_go;
     __shared__ TickJob sTJ;
     const int tid = threadIdx.x;
     stamp_present(P.present, P.present_tag, tid);
+    // The level blocks' transform passes each read their twiddles after a
+    // barrier; from global memory every pass waited for an L2 (after a
+    // device-scope acquire, a memory) round trip.  Copied once per launch into
+    // LDS they are the same values, so results do not change.
+    const cf *twl = tw;
+    if (P.tw_lds > 0) {
+        cf *t = pc_lds + P.tw_lds_off;
+        for (int i = tid; i < P.tw_lds; i += NT) t[i] = tw[i];
+        __syncthreads();
+        twl = t;
+    }
     if ((int)blockIdx.x >= P.A.njobs) {
         const int2 range = P.wrange[blockIdx.x - P.A.njobs];
         const int lw = P.wjobs[range.x].logN;
-        level_worker_dispatch<LOGN>(P, range, pc_lds, pc_lds + (1 << lw), tw, tid, &s_go);
+        level_worker_dispatch<LOGN>(P, range, pc_lds, pc_lds + (1 << lw), twl, tid, &s_go);
         return;
     }
     cf *bufA = pc_lds;
@@ -1624,7 +1695,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
             tick128_phaseB(sTJ, f, tid, xr, st128);
         } else {
             cf xr[IPT];
-            tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, tw, tid, st, acc, xr, P.A.ll ? llx : nullptr);
+            tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, twl, tid, st, acc, xr, P.A.ll ? llx : nullptr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
             tick_phaseB<LOGN>(sTJ, f, tid, xr, st);
@@ -1639,7 +1710,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
                 const int ks = sTJ.kshift[L];
                 const uint64_t d = (1ull << (ks - 1)) - 1;
                 if (f < P.f0 + d || ((f - d) & ((1ull << ks) - 1)) != 0) continue;
-                self_level<LOGN>(P, q, (f - d) >> ks, pc_lds, tw, tid);
+                self_level<LOGN>(P, q, (f - d) >> ks, pc_lds, twl, tid);
             }
         }
         if (P.nworkers > 0 && (f & P.pub_mask) == 0) {
