@@ -994,8 +994,11 @@ struct TickState128 {
     float2 pv;       // previous frame samples 2 lane, 2 lane + 1
 };
 
+// h1l (LDS, 128 float2, or null): this lane's bins of H_1 are copied there, so
+// that phase B reads no global memory when P = 2 (the lane that stores them is
+// the lane that reads them).
 __device__ __forceinline__ void tick128_load_state(const LevelJob &J, uint64_t f, int lane, const cf *__restrict__ tw,
-                                                   TickState128 &st)
+                                                   TickState128 &st, cf *h1l = nullptr)
 {
     if (lane >= 64) return;
     w64_setup(st.w, tw, lane);
@@ -1008,6 +1011,10 @@ __device__ __forceinline__ void tick128_load_state(const LevelJob &J, uint64_t f
     st.h0b = J.H[kb];
     st.spa = J.S[ka];
     st.spb = J.S[kb];
+    if (h1l && J.P > 1) {
+        h1l[ka] = J.H[128 + ka];
+        h1l[kb] = J.H[128 + kb];
+    }
     const float *prev = J.ring + (((long long)f * 128 - 128) & (long long)J.ring_mask);
     st.pv = make_float2(prev[2 * lane], prev[2 * lane + 1]);
 }
@@ -1109,7 +1116,7 @@ __device__ __forceinline__ void tick128_phaseA(const TickJob &TJ, const TickArgs
 // Phase B for buffer_size 128 (wave 0): FDL row f % P <- X_f; S for tick f+1
 // (bins 2k, 2k + 1).
 __device__ __forceinline__ void tick128_phaseB(const TickJob &TJ, uint64_t f, int tid, const cf *xr,
-                                               TickState128 &st)
+                                               TickState128 &st, const cf *h1l = nullptr)
 {
     if (tid >= 64) return;
     const LevelJob &J = TJ.l0;
@@ -1118,8 +1125,11 @@ __device__ __forceinline__ void tick128_phaseB(const TickJob &TJ, uint64_t f, in
     const int slot = (int)(f % (uint64_t)P);
     J.fdl[(size_t)slot * 128 + ka] = xr[0];
     J.fdl[(size_t)slot * 128 + kb] = xr[1];
-    cf sa = (P > 1) ? pmul(xr[0], J.H[128 + ka], ka == 0) : make_float2(0.f, 0.f);
-    cf sb = (P > 1) ? cmul(xr[1], J.H[128 + kb]) : make_float2(0.f, 0.f);
+    const cf *h1 = h1l ? h1l : J.H + 128;
+    const cf h1a = (P > 1) ? h1[ka] : make_float2(0.f, 0.f);
+    const cf h1b = (P > 1) ? h1[kb] : make_float2(0.f, 0.f);
+    cf sa = (P > 1) ? pmul(xr[0], h1a, ka == 0) : make_float2(0.f, 0.f);
+    cf sb = (P > 1) ? cmul(xr[1], h1b) : make_float2(0.f, 0.f);
     // partitions 2..P-1 in batches of 4: loads in flight together, summed in order
     for (int p0 = 2; p0 < P; p0 += 4) {
         cf av[4], bv[4], ha[4], hb[4];
@@ -1589,6 +1599,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     __shared__ unsigned s_last;
     __shared__ int s_go;
     __shared__ TickJob sTJ;
+    __shared__ cf s_h1[(LOGN == 7) ? 128 : 1];  // level 0's H_1 (buffer 128)
     const int tid = threadIdx.x;
     stamp_present(P.present, P.present_tag, tid);
     // The level blocks' transform passes each read their twiddles after a
@@ -1620,7 +1631,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     TickState64 st64;
     TickState128 st128;
     if constexpr (LOGN == 6) tick64_load_state(sTJ.l0, P.f0, tid, tw, st64);
-    else if constexpr (LOGN == 7) tick128_load_state(sTJ.l0, P.f0, tid, tw, st128);
+    else if constexpr (LOGN == 7) tick128_load_state(sTJ.l0, P.f0, tid, tw, st128, s_h1);
     else tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
@@ -1692,7 +1703,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
                            P.A.ll ? llx : nullptr);
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
-            tick128_phaseB(sTJ, f, tid, xr, st128);
+            tick128_phaseB(sTJ, f, tid, xr, st128, s_h1);
         } else {
             cf xr[IPT];
             tick_phaseA<LOGN>(sTJ, P.A, f, bufA, bufB, twl, tid, st, acc, xr, P.A.ll ? llx : nullptr);
