@@ -4679,8 +4679,10 @@ static hipError_t mxr_launch_t(pc_mimo_rt *r, int G, bool *plain)
     // two FFT buffers and the twiddles, plus the output workgroups' filter
     // cache when it leaves room for two workgroups per CU
     const size_t fft = 5 * (size_t)(1 << LOGN) * sizeof(cf);
-    const size_t hc = (size_t)(1 << LOGN) * r->n_in * sizeof(cf);
+    const size_t hc = (size_t)(1 << LOGN) * a.lv[0].ni * sizeof(cf);
     a.hcache = (fft + hc <= 80 * 1024) ? 1 : 0;
+    a.lpb = 2;
+    while (a.lpb < a.lv[0].ni / 2) a.lpb *= 2;
     const size_t lds = fft + (a.hcache ? hc : 0);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_mxr_persist<LOGN>,

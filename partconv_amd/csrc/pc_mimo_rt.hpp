@@ -67,6 +67,7 @@ struct MxrArgs {
                                 // before each launch)
     unsigned long long spin_limit;
     int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
+    int lpb;                // lanes per bin of level 0's p = 0 product: next power of two >= ni / 2
     unsigned *present;      // plain launch: residency stamps (see stamp_present), else null
     unsigned present_tag;
 };
@@ -200,6 +201,69 @@ __device__ __forceinline__ void mxr_arrive(unsigned long long *ctr, int tid)
     }
 }
 
+// Level 0's p = 0 term for output o over every bin, without S:
+//   Y[k] = sum_i H_0[o][i][k] X_i[k]   (X: delay-line row r0).
+// A bin's ni inputs are 8 ni contiguous bytes of the delay line, so lpb lanes
+// (the next power of two >= ni / 2) share a bin and each takes one input pair:
+// every load instruction of a wave then reads 64 / lpb whole bins, one
+// contiguous run, instead of one bin per lane 8 ni bytes apart (which touched
+// 64 lines per instruction and fetched each line again for every 16 bytes).
+// The pair products are summed over the bin's lanes in a fixed xor tree, so
+// the result does not depend on timing.  Hc: H_0[o] bin-major (Hc[k ni + i])
+// in LDS, or null to read H from memory.  B steps' delay-line loads (and,
+// from memory, filter loads) are in flight together.
+template <int B, bool HC>
+__device__ __forceinline__ void mxr_out_product(const MxrLevel &L0, int o, int n_out, int N0, int r0, const cf *Hc,
+                                                cf *Y, int lpb, int tid)
+{
+    const int bpi = 64 / lpb;           // bins per load instruction of a wave
+    const int steps = lpb;              // a wave takes 64 bins: 64 / bpi steps
+    const int lane = tid & 63, wv = tid >> 6;
+    const int q = lane & (lpb - 1), bsub = lane / lpb;
+    const int ni = L0.ni, np = ni / 2;  // inputs per bin row (padded to 4), input pairs
+    const bool lq = q < np;
+    const float4 *X4 = reinterpret_cast<const float4 *>(L0.fdl + (size_t)r0 * N0 * ni);
+    const float4 *H4 = reinterpret_cast<const float4 *>(L0.H);
+    const float4 *Hc4 = reinterpret_cast<const float4 *>(Hc);
+#pragma unroll 1
+    for (int k0 = wv * 64; k0 < N0; k0 += MXR_NTT) {
+#pragma unroll 1
+        for (int j0 = 0; j0 < steps; j0 += B) {
+            float4 xv[B], hv[HC ? 1 : B];
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int k = k0 + (j0 + b) * bpi + bsub;
+                xv[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if constexpr (!HC) hv[b] = xv[b];
+                if (j0 + b < steps && k < N0 && lq) {
+                    xv[b] = X4[(size_t)k * np + q];
+                    if constexpr (!HC) hv[b] = H4[((size_t)k * L0.P * n_out + o) * np + q];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                if (j0 + b >= steps) break;
+                const int k = k0 + (j0 + b) * bpi + bsub;
+                const bool bin0 = (k == 0);
+                float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+                if constexpr (HC) {
+                    if (k < N0 && lq) h = Hc4[(size_t)k * np + q];
+                } else {
+                    h = hv[b];
+                }
+                cf a = pmul(make_float2(xv[b].x, xv[b].y), make_float2(h.x, h.y), bin0);
+                a = pmac(a, make_float2(xv[b].z, xv[b].w), make_float2(h.z, h.w), bin0);
+                // fixed xor tree over the bin's lanes (lpb is uniform)
+                a = cadd(a, xor_cf<1>(a, lane));
+                if (lpb > 2) a = cadd(a, xor_cf<2>(a, lane));
+                if (lpb > 4) a = cadd(a, xor_cf<4>(a, lane));
+                if (lpb > 8) a = cadd(a, xor_cf<8>(a, lane));
+                if (q == 0 && k < N0) Y[k] = a;
+            }
+        }
+    }
+}
+
 // Workgroups by role: [0, n_in) input, [n_in, n_in + n_out) output, the rest
 // (G4 of them) the delay-line sum.  Per tick t of the launch (f = f0 + t):
 //   input:  release; frame -> ring, window -> r2c -> delay-line row f % P;
@@ -232,17 +296,17 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
     stamp_present(A.present, A.present_tag, tid);
     // input and output workgroups: the transforms' twiddles (entries below
     // 2 N0 + N0 / 2 + 1) in LDS, so the transforms make no global loads;
-    // output workgroups also keep their partition-0 filter spectra,
-    // input-major (Hc[i N0 + k]: lanes read consecutive words)
+    // output workgroups also keep their partition-0 filter spectra, bin-major
+    // as in H (Hc[k ni + i], padded inputs included)
     cf *twl = mxr_lds + 2 * N0;
     cf *Hc = mxr_lds + 5 * N0;
     if (w < n_io) {
         for (int e = tid; e <= 2 * N0 + N0 / 2; e += MXR_NTT) twl[e] = tw[e];
         if (A.hcache && w >= A.n_in) {
             const int o = w - A.n_in;
-            for (int e = tid; e < N0 * A.n_in; e += MXR_NTT) {
-                const int k = e / A.n_in, i = e - k * A.n_in;
-                Hc[(size_t)i * N0 + k] = L0.H[((size_t)k * L0.P * A.n_out + o) * L0.ni + i];
+            for (int e = tid; e < N0 * L0.ni; e += MXR_NTT) {
+                const int k = e / L0.ni, i = e - k * L0.ni;
+                Hc[e] = L0.H[((size_t)k * L0.P * A.n_out + o) * L0.ni + i];
             }
         }
         __syncthreads();
@@ -315,31 +379,27 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
             if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * (G4 / MXR_SG), A.spin_limit,
                            tid, &s_ok))
                 return;
-            const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0;
+            // S for this tick, read coalesced now; added after the product
+            // (sk + a: the same sum as before, addition being commutative)
+            const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0 + (size_t)o * N0;
+            constexpr int JS = (N0 + MXR_NTT - 1) / MXR_NTT;
+            cf sk[JS];
+#pragma unroll
+            for (int q = 0; q < JS; q++) {
+                const int k = tid + q * MXR_NTT;
+                sk[q] = (k < N0) ? gload2(S + k) : make_float2(0.f, 0.f);
+            }
             const int r0 = (int)(f % (uint64_t)L0.P);
-            for (int k = tid; k < N0; k += MXR_NTT) {
-                const bool bin0 = (k == 0);
-                const cf *h = L0.H + ((size_t)k * L0.P * A.n_out + o) * L0.ni;  // partition 0
-                cf acc = gload2(S + (size_t)o * N0 + k);
-                // the bin's inputs (contiguous) in rounds of 16
-                const float4 *xk = reinterpret_cast<const float4 *>(L0.fdl + ((size_t)r0 * N0 + k) * L0.ni);
-                for (int i0 = 0; i0 < A.n_in; i0 += 16) {
-                    float4 xv[8];
+            if (A.hcache) mxr_out_product<8, true>(L0, o, A.n_out, N0, r0, Hc, bufA, A.lpb, tid);
+            else mxr_out_product<4, false>(L0, o, A.n_out, N0, r0, nullptr, bufA, A.lpb, tid);
+            __syncthreads();
 #pragma unroll
-                    for (int q = 0; q < 8; q++)
-                        if (i0 + 2 * q < A.n_in) xv[q] = xk[i0 / 2 + q];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const int i = i0 + 2 * q;
-                        if (i < A.n_in)
-                            acc = pmac(acc, make_float2(xv[q].x, xv[q].y),
-                                       A.hcache ? Hc[(size_t)i * N0 + k] : h[i], bin0);
-                        if (i + 1 < A.n_in)
-                            acc = pmac(acc, make_float2(xv[q].z, xv[q].w),
-                                       A.hcache ? Hc[(size_t)(i + 1) * N0 + k] : h[i + 1], bin0);
-                    }
+            for (int q = 0; q < JS; q++) {
+                const int k = tid + q * MXR_NTT;
+                if (k < N0) {
+                    const cf a = bufA[k];
+                    bufA[k] = make_float2(sk[q].x + a.x, sk[q].y + a.y);
                 }
-                bufA[k] = acc;
             }
             __syncthreads();
             const cf *z = small_inverse<LOGN, MXR_NTT>(bufA, bufB, twl, tid);
