@@ -806,7 +806,9 @@ def cpu_baseline_diff(x, h, tgt, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 30 s of input per channel for cfg2, the configuration's "
+                         "length; 40 for the others)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -822,6 +824,9 @@ def main():
                     help="cfg5: after set-up and warm-up run one step between two 4 MiB device copies (markers "
                          "for per-step counter sums, prof_summary step) and stop")
     args = ap.parse_args()
+    if args.steps is None:
+        # cfg2 states 30 s of input per channel at 48 kHz: 88 steps of 256 ticks x 64 samples
+        args.steps = -(-30 * FS // (TICKS_PER_STEP * N0)) if args.workload == "cfg2" else 40
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: each GPU runs its own share of channels independently "
               "(no collective on this path); run it with --gpus 1", file=sys.stderr)
