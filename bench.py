@@ -584,6 +584,14 @@ def bench_diff(args):
     tuning = tinfo["levels"]
 
     def step():
+        # the fused loss: pcc_mse forms the loss and its gradient in one pass
+        # over y and the target, pcc_backward_scaled applies the upstream factor
+        h0.grad = None
+        loss = conv.mse_loss(x, h0, tgt)
+        loss.backward()
+        return loss
+
+    def step_torch_loss():
         h0.grad = None
         loss = torch.nn.functional.mse_loss(conv(x, h0), tgt)
         loss.backward()
@@ -622,6 +630,15 @@ def bench_diff(args):
             torch.cuda.synchronize()
             probes[(kind, nl)] = conv.probe_read()
             conv.probe(-1)
+    # the same step with torch's mse_loss over the convolution's output
+    for _ in range(2):
+        step_torch_loss()
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    for _ in range(args.steps):
+        step_torch_loss()
+    torch.cuda.synchronize()
+    secs_tl = time.perf_counter() - t5
     # without the library's call graphs: every launch eager, the levels on one
     # stream, then each level on its own stream
     conv.set_call_graphs(False)
@@ -640,13 +657,9 @@ def bench_diff(args):
     secs_par = time.perf_counter() - t2
     conv.set_level_streams("captured")
     conv.set_call_graphs(True)
-    # the same step captured once in a HIP graph and replayed (the MSE gradient
-    # formed elementwise, 2 (y - tgt) / numel: no reduction inside the graph)
-    scale = 2.0 / (C * T)
-
+    # the same step (fused loss) captured once in a HIP graph and replayed
     def gstep():
-        y = conv(x, h0)
-        y.backward((y - tgt) * scale)
+        conv.mse_loss(x, h0, tgt).backward()
 
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -750,6 +763,9 @@ def bench_diff(args):
                         "note": "value: the library's default, each pcc_forward / pcc_backward call replayed "
                                 "from a HIP graph of its own launches (levels as parallel branches) once its "
                                 "arguments repeat; counts over warm-up + timed steps"},
+        "torch_mse_loss": {"ms_per_step": 1e3 * secs_tl / args.steps,
+                           "note": "the same step with torch.nn.functional.mse_loss over the convolution's output "
+                                   "instead of the fused loss (pcc_mse)"},
         "eager_no_call_graphs": {"ms_per_step": 1e3 * secs_eager / args.steps,
                                  "note": "the same step with call graphs off: every launch eager, every level "
                                          "on the caller's stream; bitwise the same results"},
@@ -760,8 +776,8 @@ def bench_diff(args):
                          "from_cache": tinfo["from_cache"]},
         "hip_graph_replay": {"ms_per_step": 1e3 * gsecs / args.steps, "value": C * T * args.steps / gsecs,
                              "grad_equals_eager": graph_ok,
-                             "note": "forward + adjoint captured once (torch.cuda.graph) and replayed, the "
-                                     "levels as parallel branches; loss gradient formed elementwise"},
+                             "note": "forward + fused loss + adjoint captured once (torch.cuda.graph) and replayed, "
+                                     "the levels as parallel branches"},
         "roofline": roof, "kernel_rooflines": kroofs,
         "canonical_bytes_per_clip_sample": {"value": canon, "use": "reference's per-block formula for one forward "
                                             "pass; not a traffic count for this form (H shared by the clips, each "

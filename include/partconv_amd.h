@@ -385,6 +385,20 @@ int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, int T, 
  * be null). */
 int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
                  void *stream);
+/* Fused mean-squared-error loss (no reference counterpart: the reference has
+ * no loss or adjoint; for training with the differentiable form, where a
+ * separate framework loss would read y and the target and write the gradient
+ * in several more passes).  From y and t ([C][T], device): *d_loss =
+ * sum((y - t)^2) / (C T) (one device float, a fixed-order sum that does not
+ * depend on timing) and d_g[C][T] = (2 / (C T)) (y - t), in one read of y and
+ * t and one write of g. */
+int pcc_mse(pc_conv *c, const float *d_y, const float *d_t, float *d_g, float *d_loss, int C, int T, void *stream);
+/* pcc_backward for the upstream gradient d_g times the device scalar
+ * d_scale[0] (a loss's own upstream gradient, read on the device: no host
+ * synchronisation), applied inside the gradient transforms.  No reference
+ * counterpart; for pcc_mse's gradient. */
+int pcc_backward_scaled(pc_conv *c, const float *d_x, const float *d_g, const float *d_scale, float *d_dx,
+                        float *d_dh, int C, int T, void *stream);
 /* Level-parallel calls (no reference counterpart): with mode
  * PCC_LEVEL_STREAMS_ON pcc_forward and pcc_backward run each level on its own
  * stream, forked from the caller's stream with events and joined back to it,

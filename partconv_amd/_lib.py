@@ -107,6 +107,8 @@ SIGNATURES = {
     "pcc_set_filter": (ctypes.c_int, [_vp, _vp, _vp]),
     "pcc_forward": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pcc_backward_scaled": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pcc_mse": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_forward_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pcc_set_budget": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcc_recomputed_levels": (ctypes.c_longlong, [_vp]),

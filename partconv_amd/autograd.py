@@ -4,6 +4,7 @@ over the pcc_* entry points of libpartconv_amd.so).
     conv = PartitionedConv([256, 2048, 16384], [14, 14, 0], taps=65536)
     y = conv(x, h)          # x: [C, T] float32 cuda, h: [taps] float32 cuda
     loss = ((y - target) ** 2).mean(); loss.backward()
+    loss = conv.mse_loss(x, h, target); loss.backward()   # the same, fused
 
 Forward: y[c] = conv(x[c], h) truncated to T samples, computed as the
 reference's multi-level partitioned convolution (partition rules of
@@ -138,16 +139,29 @@ class PartitionedConv:
         self._check(lib().pcc_forward(self._h, _p(x), _p(y), x.shape[0], x.shape[1], _stream()))
         return gen
 
-    def backward_raw(self, x, h, g, dx, dh, gen=None):
-        """Adjoint.  The spectra are recomputed unless they are still generation
-        `gen` (the forward that saved h computed them and no set_filter ran
-        since; autograd rejects a saved h that was modified in place).  Under
-        graph capture they are always recomputed, so a replay sees the filter's
-        values at replay time."""
+    def backward_raw(self, x, h, g, dx, dh, gen=None, scale=None):
+        """Adjoint for upstream gradient g (times the one-element device tensor
+        `scale` when given).  The spectra are recomputed unless they are still
+        generation `gen` (the forward that saved h computed them and no
+        set_filter ran since; autograd rejects a saved h that was modified in
+        place).  Under graph capture they are always recomputed, so a replay
+        sees the filter's values at replay time."""
         if gen is None or gen != self._filter_gen or torch.cuda.is_current_stream_capturing():
             self.set_filter(h)
-        self._check(lib().pcc_backward(self._h, _p(x), _p(g), None if dx is None else _p(dx),
-                                       None if dh is None else _p(dh), x.shape[0], x.shape[1], _stream()))
+        odx = None if dx is None else _p(dx)
+        odh = None if dh is None else _p(dh)
+        if scale is None:
+            self._check(lib().pcc_backward(self._h, _p(x), _p(g), odx, odh, x.shape[0], x.shape[1], _stream()))
+        else:
+            self._check(lib().pcc_backward_scaled(self._h, _p(x), _p(g), _p(scale), odx, odh, x.shape[0],
+                                                  x.shape[1], _stream()))
+
+    def mse_loss(self, x, h, target):
+        """mean((conv(x, h) - target) ** 2) as one differentiable call: the loss
+        and its gradient come from one fused pass over y and the target
+        (pcc_mse), and the backward scales that gradient by the loss's upstream
+        gradient inside its transforms (pcc_backward_scaled)."""
+        return PartConvMSEFunction.apply(x, h, target, self)
 
     def __call__(self, x, h):
         return PartConvFunction.apply(x, h, self)
@@ -189,3 +203,34 @@ class PartConvFunction(torch.autograd.Function):
         if dx is not None or dh is not None:
             conv.backward_raw(x, h, g, dx, dh, ctx.filter_gen)
         return dx, dh, None
+
+
+class PartConvMSEFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, h, target, conv):
+        x = _prep(x, "x")
+        h = _prep(h, "h")
+        target = _prep(target, "target")
+        if x.dim() != 2 or h.dim() != 1 or h.numel() != conv.taps or target.shape != x.shape:
+            raise ValueError("x and target must be [C, T] and h [taps]")
+        y = torch.empty_like(x)
+        ctx.filter_gen = conv.forward_raw(x, h, y)
+        g = torch.empty_like(x)
+        loss = torch.empty((), dtype=torch.float32, device=x.device)
+        conv._check(lib().pcc_mse(conv._h, _p(y), _p(target), _p(g), _p(loss), x.shape[0], x.shape[1],
+                                  _stream()))
+        ctx.save_for_backward(x, h, g)
+        ctx.conv = conv
+        return loss
+
+    @staticmethod
+    def backward(ctx, gl):
+        x, h, g = ctx.saved_tensors
+        conv = ctx.conv
+        gl = gl.reshape(1).contiguous().float()
+        dx = torch.empty_like(x) if ctx.needs_input_grad[0] else None
+        dh = torch.empty_like(h) if ctx.needs_input_grad[1] else None
+        if dx is not None or dh is not None:
+            conv.backward_raw(x, h, g, dx, dh, ctx.filter_gen, scale=gl)
+        dt = -(g * gl) if ctx.needs_input_grad[2] else None
+        return dx, dh, dt, None

@@ -3130,9 +3130,10 @@ struct ConvLevelWork {
 // replayed by later calls with the same arguments: key (the call's pointers),
 // shape, flags (kept spectra) and fingerprint (every buffer, chunk size and
 // mode the captured launches embed) must all match.
+#define PC_CG_KEYS 6  // pointers that identify a call's arguments
 struct CallGraph {
     hipGraphExec_t exec = nullptr;
-    const void *key[4] = {nullptr, nullptr, nullptr, nullptr};
+    const void *key[PC_CG_KEYS] = {};
     int C = 0, T = 0, flags = 0;
     std::vector<uintptr_t> fp;
 };
@@ -3171,6 +3172,7 @@ struct pc_conv {
     int keep_C = 0, keep_T = 0;
     bool keep_valid = false;
     long long recomputed_levels = 0;
+    float *mse_part = nullptr;  // pcc_mse's per-block partial sums (PC_MSE_BLOCKS)
     // probe: HIP events around the forward spectral product of one level
     int probe_level = -1, probe_kind = PCC_PROBE_PRODUCT;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev;
@@ -3343,7 +3345,7 @@ static int off_rfft(pc_conv *c, const ConvLevel &L, const WinSrc &W, int rows, c
         off_launch_ev(k_off_big_fwd1, dim3((A / BIG_G) * nt), dim3(BIG_NT), big1_lds(L.sh), s, e0, nullptr, W,
                       L.sh, scr, c->tw);
         off_launch_ev(k_off_big_fwd2, dim3((B / 2) * nt), dim3(BIG_NT), big2_lds(L.sh), s, nullptr, e1, L.sh, scr,
-                      X, W.scale, c->tw);
+                      X, W.scale, W.dscale, c->tw);
     } else {
         switch (ilog2(L.N)) {
         case 4: off_rfft_t<4>(W, nt, X, c->tw, s, e0, e1); break;
@@ -3628,19 +3630,19 @@ static std::vector<uintptr_t> conv_fingerprint(const pc_conv *c, bool bwd)
     return fp;
 }
 
-static bool call_graph_match(const CallGraph &g, const void *const key[4], int C, int T, int flags,
+static bool call_graph_match(const CallGraph &g, const void *const key[PC_CG_KEYS], int C, int T, int flags,
                              const std::vector<uintptr_t> &fp)
 {
     if (g.C != C || g.T != T || g.flags != flags) return false;
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < PC_CG_KEYS; i++)
         if (g.key[i] != key[i]) return false;
     return g.fp == fp;
 }
 
-static void call_graph_set(CallGraph &g, const void *const key[4], int C, int T, int flags,
+static void call_graph_set(CallGraph &g, const void *const key[PC_CG_KEYS], int C, int T, int flags,
                            std::vector<uintptr_t> fp)
 {
-    for (int i = 0; i < 4; i++) g.key[i] = key[i];
+    for (int i = 0; i < PC_CG_KEYS; i++) g.key[i] = key[i];
     g.C = C;
     g.T = T;
     g.flags = flags;
@@ -3688,7 +3690,7 @@ static hipGraphExec_t call_graph_capture(pc_conv *c, F launch)
 // calls had them, else launch eagerly.  Bypassed (always eager) when call graphs
 // are off, a kernel probe is set, or s is itself being captured.
 template <typename F>
-static int call_graph_run(pc_conv *c, CallGraphs &G, hipStream_t s, const void *const key[4], int C, int T,
+static int call_graph_run(pc_conv *c, CallGraphs &G, hipStream_t s, const void *const key[PC_CG_KEYS], int C, int T,
                           int flags, F launch)
 {
     if (!c->call_graphs || c->probe_level >= 0) return launch(s);
@@ -3790,7 +3792,7 @@ PC_API int pcc_forward(pc_conv *c, const float *d_x, float *d_y, int C, int T, v
     for (ConvLevel &L : c->lv) need_keep += (size_t)C * ((T + L.N - 1) / L.N) * L.N * sizeof(cf);
     const bool keep = (long long)need_keep <= c->budget;
     c->keep_valid = false;
-    const void *key[4] = {d_x, d_y, nullptr, nullptr};
+    const void *key[PC_CG_KEYS] = {d_x, d_y};
     const int rc = call_graph_run(c, c->fg, s, key, C, T, keep ? 1 : 0, [&](hipStream_t ls) {
         return conv_forward_launch(c, d_x, d_y, C, T, ls, keep);
     });
@@ -3874,8 +3876,8 @@ PC_API int pcc_forward_noalloc(pc_conv *c, const float *d_x, float *d_y, int C, 
 
 // The adjoint's launches on stream s (see conv_forward_launch); kept: the
 // forward's spectra of x are valid, else they are recomputed.
-static int conv_backward_launch(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C,
-                                int T, hipStream_t s, bool kept)
+static int conv_backward_launch(pc_conv *c, const float *d_x, const float *d_g, const float *d_scale, float *d_dx,
+                                float *d_dh, int C, int T, hipStream_t s, bool kept)
 {
     int rc = PC_OK;
     std::vector<hipStream_t> st;
@@ -3901,6 +3903,7 @@ static int conv_backward_launch(pc_conv *c, const float *d_x, const float *d_g, 
         Wg.ulo = 0;
         Wg.uhi = L.N;
         Wg.scale = 1.f;
+        Wg.dscale = d_scale;  // the upstream gradient's device-side factor, if any
         if ((rc = off_rfft(c, L, Wg, C, w.G, ls, scr))) return rc;
         if (d_dh) {
             cf *X;
@@ -3961,11 +3964,11 @@ static int conv_backward_launch(pc_conv *c, const float *d_x, const float *d_g, 
     return PC_OK;
 }
 
-// Adjoint of pcc_forward for upstream gradient g: dx (may be null) and dh (may
-// be null).  Uses the forward's kept spectra when they are for the same x and
-// within the budget, else recomputes them from x.
-PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
-                        void *stream)
+// Adjoint of pcc_forward for upstream gradient g (times *d_scale when set):
+// dx (may be null) and dh (may be null).  Uses the forward's kept spectra when
+// they are for the same x and within the budget, else recomputes them from x.
+static int conv_backward(pc_conv *c, const float *d_x, const float *d_g, const float *d_scale, float *d_dx,
+                         float *d_dh, int C, int T, void *stream)
 {
     if (!c || !d_x || !d_g || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
     if (!c->filter_ready) return fail(PC_ERR_STATE, "backward before set_filter");
@@ -3973,12 +3976,45 @@ PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d
     int rc;
     if (d_dx && (rc = conv_fill_zero(d_dx, (long long)C * T, s))) return rc;
     const bool kept = c->keep_valid && c->keep_x == d_x && c->keep_C == C && c->keep_T == T;
-    const void *key[4] = {d_x, d_g, d_dx, d_dh};
+    const void *key[PC_CG_KEYS] = {d_x, d_g, d_dx, d_dh, d_scale};
     if ((rc = call_graph_run(c, c->bg, s, key, C, T, kept ? 1 : 0, [&](hipStream_t ls) {
-             return conv_backward_launch(c, d_x, d_g, d_dx, d_dh, C, T, ls, kept);
+             return conv_backward_launch(c, d_x, d_g, d_scale, d_dx, d_dh, C, T, ls, kept);
          })))
         return rc;
     if (d_dh && !kept) c->recomputed_levels += (long long)c->lv.size();
+    return PC_OK;
+}
+
+PC_API int pcc_backward(pc_conv *c, const float *d_x, const float *d_g, float *d_dx, float *d_dh, int C, int T,
+                        void *stream)
+{
+    return conv_backward(c, d_x, d_g, nullptr, d_dx, d_dh, C, T, stream);
+}
+
+PC_API int pcc_backward_scaled(pc_conv *c, const float *d_x, const float *d_g, const float *d_scale, float *d_dx,
+                               float *d_dh, int C, int T, void *stream)
+{
+    if (!d_scale) return fail(PC_ERR_ARG, "bad argument");
+    return conv_backward(c, d_x, d_g, d_scale, d_dx, d_dh, C, T, stream);
+}
+
+// Mean squared error of y against t with its gradient (k_mse_grad, then
+// k_mse_final): one read of y and t, one write of g, a fixed-order sum.
+PC_API int pcc_mse(pc_conv *c, const float *d_y, const float *d_t, float *d_g, float *d_loss, int C, int T,
+                   void *stream)
+{
+    if (!c || !d_y || !d_t || !d_g || !d_loss || C < 1 || T < 1) return fail(PC_ERR_ARG, "bad argument");
+    if (!c->mse_part) {
+        HIPCHK(hipMalloc(&c->mse_part, PC_MSE_BLOCKS * sizeof(float)));
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = (long long)C * T;
+    const long long chunk = (n + PC_MSE_BLOCKS - 1) / PC_MSE_BLOCKS;
+    const int nb = (int)((n + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(PC_MSE_NT), 0, s, d_y, d_t, d_g, (float)(2.0 / (double)n), n, chunk,
+                       c->mse_part);
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(PC_MSE_NT), 0, s, c->mse_part, nb, (float)(1.0 / (double)n), d_loss);
+    HIPCHK(hipGetLastError());
     return PC_OK;
 }
 
@@ -4160,6 +4196,7 @@ static void conv_destroy(pc_conv *c)
     hipFree(c->scr);
     hipFree(c->dpart);
     hipFree(c->dsum);
+    hipFree(c->mse_part);
     delete c;
 }
 

@@ -314,18 +314,19 @@ k_off_rfft_reg(WinSrc W, int ntrans, cf *__restrict__ X, const cf *__restrict__ 
     __syncthreads();
     if (!live) return;
     cf *o = X + (size_t)tr * N;
+    const float sc = win_scale(W);
 #pragma unroll
     for (int i = 0; i <= 8; i++) {
         const int k = t + i * TT;
         if (k > N / 2) break;
         if (k == 0) {
             const cf z = L[0];
-            o[0] = cscale(make_float2(z.x + z.y, z.x - z.y), W.scale);
+            o[0] = cscale(make_float2(z.x + z.y, z.x - z.y), sc);
         } else {
             cf xk, xnk;
             rsplit_fwd(L[rpad(k)], L[rpad(N - k)], i ? cmul(wsplit, split_rot(i)) : wsplit, &xk, &xnk);
-            o[k] = cscale(xk, W.scale);
-            o[N - k] = cscale(xnk, W.scale);
+            o[k] = cscale(xk, sc);
+            o[N - k] = cscale(xnk, sc);
         }
     }
 }

@@ -21,7 +21,7 @@
 
 // Window source of a batched r2c: transform t = r * nb + m reads
 //   w[u] = src[r*row_stride + base0 + m*base_step + u]  for ulo <= u < uhi and
-//          0 <= base0 + m*base_step + u < len, else 0;   X *= scale.
+//          0 <= base0 + m*base_step + u < len, else 0;   X *= scale (* *dscale).
 struct WinSrc {
     const float *src;
     long long row_stride;
@@ -30,7 +30,11 @@ struct WinSrc {
     int nb;
     int ulo, uhi;
     float scale;
+    const float *dscale;  // device scalar the spectra are also scaled by (null: none)
 };
+
+// The spectra's scale factor of a window source: scale, times *dscale if set.
+__device__ __forceinline__ float win_scale(const WinSrc &W) { return W.dscale ? W.scale * W.dscale[0] : W.scale; }
 
 // Destination of a batched c2r: transform t = r * nb + m writes samples u of
 // [ulo, uhi) (scaled) to dst[r*row_stride + base0 + m*base_step + u] when that
@@ -126,9 +130,10 @@ k_off_big_fwd1(WinSrc W, BigShape sh, cf *__restrict__ S, const cf *__restrict__
 
 // column-pair DFTs + real split -> X (four-step slot order), scaled
 __global__ void __launch_bounds__(BIG_NT)
-k_off_big_fwd2(BigShape sh, const cf *__restrict__ S, cf *__restrict__ X, float scale,
+k_off_big_fwd2(BigShape sh, const cf *__restrict__ S, cf *__restrict__ X, float scale, const float *dscale,
                const cf *__restrict__ tw)
 {
+    const float sc = dscale ? scale * dscale[0] : scale;
     extern __shared__ cf lds[];
     const int A = 1 << sh.logA, B = 1 << sh.logB, N = A * B;
     const int sst = A + 1;
@@ -178,7 +183,7 @@ k_off_big_fwd2(BigShape sh, const cf *__restrict__ S, cf *__restrict__ X, float 
     cf *Xt = X + (size_t)t * N + (size_t)u * 2 * A;
     for (int e = tid; e < 2 * A; e += BIG_NT) {
         const int q = e >> sh.logA, d = e & (A - 1);
-        Xt[e] = cscale(W[q * sst + d], scale);
+        Xt[e] = cscale(W[q * sst + d], sc);
     }
 }
 
@@ -731,4 +736,55 @@ k_off_dh_part_b(const cf *__restrict__ G, const cf *__restrict__ X, cf *__restri
 #pragma unroll
     for (int t = 0; t < PT; t++)
         if (p0 + t < P) dst[(size_t)(p0 + t) * C * nch * N] = acc[t];
+}
+
+// ---------------------------------------------------------------------------
+// Mean squared error of y against t over n samples, with its gradient:
+//   g = (2 / n) (y - t),  loss = sum (y - t)^2 / n.
+// k_mse_grad: block b covers the contiguous samples [b chunk, (b + 1) chunk),
+// each thread a strided subset in order, then a fixed tree over the block, so
+// part[b] does not depend on timing; k_mse_final sums part[0 .. nb) the same
+// way in one block.  Fused, the loss reads y and t once and writes g once.
+// ---------------------------------------------------------------------------
+#define PC_MSE_NT 256
+#define PC_MSE_BLOCKS 1024
+
+__device__ __forceinline__ float block_sum_fixed(float v, float *red, int tid)
+{
+    red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int d = PC_MSE_NT / 2; d > 0; d >>= 1) {
+        if (tid < d) red[tid] += red[tid + d];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+__global__ void __launch_bounds__(PC_MSE_NT)
+k_mse_grad(const float *__restrict__ y, const float *__restrict__ t, float *__restrict__ g, float gfac, long long n,
+           long long chunk, float *__restrict__ part)
+{
+    __shared__ float red[PC_MSE_NT];
+    const int tid = threadIdx.x;
+    const long long lo = (long long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    float acc = 0.f;
+    for (long long i = lo + tid; i < hi; i += PC_MSE_NT) {
+        const float d = y[i] - t[i];
+        g[i] = gfac * d;
+        acc = fmaf(d, d, acc);
+    }
+    const float sum = block_sum_fixed(acc, red, tid);
+    if (tid == 0) part[blockIdx.x] = sum;
+}
+
+__global__ void __launch_bounds__(PC_MSE_NT)
+k_mse_final(const float *__restrict__ part, int nb, float inv_n, float *__restrict__ loss)
+{
+    __shared__ float red[PC_MSE_NT];
+    const int tid = threadIdx.x;
+    float acc = 0.f;
+    for (int i = tid; i < nb; i += PC_MSE_NT) acc += part[i];
+    const float sum = block_sum_fixed(acc, red, tid);
+    if (tid == 0) *loss = sum * inv_n;
 }

@@ -16,7 +16,7 @@ the last two 4 MiB copies of a run (the markers bench.py --workload cfg5
 --profile-step puts around one step; told apart from other copies by size), per kernel of the differentiable form:
 per level (from the transforms' LOGN template argument, levels ascending) the
 filter, forward, gradient and inverse transforms, the product, the dL/dh
-partials, reduce and inverse; torch's kernels count as the loss.
+partials, reduce and inverse; every other kernel (the fused loss's, torch's) counts as the loss.
 """
 from __future__ import annotations
 

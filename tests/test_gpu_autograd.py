@@ -361,6 +361,89 @@ def test_forward_and_adjoint_replay_in_a_cuda_graph():
         assert torch.equal(hs.grad, he.grad), r
 
 
+def test_fused_mse_loss_matches_fp64_and_replays_in_a_graph():
+    """conv.mse_loss (pcc_mse + pcc_backward_scaled): the loss against the fp64
+    loss; dL/dx, dL/dh and dL/dtarget against the fp64 direct gradient of
+    3 * MSE (the factor exercises the device-side upstream scale); agreement
+    with the unfused path (torch's mse_loss over conv); and a captured step,
+    replayed with new values written into its tensors, bitwise equal to the
+    eager fused step every time."""
+    import torch
+    from partconv_amd.autograd import PartitionedConv
+    N, P, taps = [256, 2048], [14, 0], 9000
+    C, T = 3, 12000
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((C, T)).astype(np.float32)
+    h = decaying_noise_ir(taps, 0.8, 48000, 22)
+    hid = decaying_noise_ir(taps, 1.5, 48000, 23)
+    t = (np.stack([O.direct_conv_f64(x[c], hid) for c in range(C)])
+         + 1e-3 * rng.standard_normal((C, T))).astype(np.float32)
+    conv = PartitionedConv(N, P, taps)
+    xt = torch.from_numpy(x).cuda().requires_grad_(True)
+    ht = torch.from_numpy(h).cuda().requires_grad_(True)
+    tt = torch.from_numpy(t).cuda().requires_grad_(True)
+    loss = conv.mse_loss(xt, ht, tt)
+    (3.0 * loss).backward()
+    torch.cuda.synchronize()
+    y64 = np.stack([O.direct_conv_f64(x[c], h) for c in range(C)])
+    r64 = y64 - t.astype(np.float64)
+    loss64 = float(np.mean(r64 ** 2))
+    g64 = 3.0 * 2.0 * r64 / (C * T)
+    lf = float(loss.detach())
+    assert abs(lf - loss64) / loss64 < 1e-4, (lf, loss64)
+    assert relmax(xt.grad.cpu().numpy(), f64_corr_x(g64, h)) < 1e-4
+    assert relmax(ht.grad.cpu().numpy(), f64_corr_h(g64, x, taps)) < 1e-4
+    assert relmax(tt.grad.cpu().numpy(), -g64) < 1e-4
+    # the unfused path: torch's loss over the convolution
+    xu = torch.from_numpy(x).cuda().requires_grad_(True)
+    hu = torch.from_numpy(h).cuda().requires_grad_(True)
+    lu = torch.nn.functional.mse_loss(conv(xu, hu), torch.from_numpy(t).cuda())
+    (3.0 * lu).backward()
+    torch.cuda.synchronize()
+    assert abs(float(lu.detach()) - lf) / lf < 1e-5
+    assert relmax(ht.grad.cpu().numpy(), hu.grad.cpu().numpy()) < 1e-4
+    assert relmax(xt.grad.cpu().numpy(), xu.grad.cpu().numpy()) < 1e-4
+    # a captured step against eager fused steps on the same values
+    gen = torch.Generator(device="cuda").manual_seed(24)
+    xs = torch.randn((C, T), device="cuda", generator=gen).requires_grad_(True)
+    hs = (0.01 * torch.randn(taps, device="cuda", generator=gen)).requires_grad_(True)
+    ts = torch.randn((C, T), device="cuda", generator=gen)
+
+    def step():
+        lo = conv.mse_loss(xs, hs, ts)
+        (3.0 * lo).backward()
+        return lo
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):  # sizes the work space outside the capture
+            xs.grad = hs.grad = None
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    xs.grad = hs.grad = None
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        ls = step()
+    ref = PartitionedConv(N, P, taps)
+    for r in range(3):
+        with torch.no_grad():
+            xs.copy_(torch.randn((C, T), device="cuda", generator=gen))
+            hs.copy_(0.01 * torch.randn(taps, device="cuda", generator=gen))
+            ts.copy_(torch.randn((C, T), device="cuda", generator=gen))
+        graph.replay()
+        torch.cuda.synchronize()
+        xe = xs.detach().clone().requires_grad_(True)
+        he = hs.detach().clone().requires_grad_(True)
+        le = ref.mse_loss(xe, he, ts)
+        (3.0 * le).backward()
+        torch.cuda.synchronize()
+        assert torch.equal(ls, le), r
+        assert torch.equal(xs.grad, xe.grad), r
+        assert torch.equal(hs.grad, he.grad), r
+
+
 def test_tuned_chunk_sizes_keep_results():
     """pcc_tune picks each level's chunk sizes from its candidates by timing;
     the forward output is bitwise that of the untuned form, and both gradients
