@@ -4009,10 +4009,14 @@ PC_API int pcc_mse(pc_conv *c, const float *d_y, const float *d_t, float *d_g, f
     }
     hipStream_t s = (hipStream_t)stream;
     const long long n = (long long)C * T;
-    const long long chunk = (n + PC_MSE_BLOCKS - 1) / PC_MSE_BLOCKS;
+    // contiguous chunks, a multiple of 4 samples (float4 groups) when y, t and g are 16-byte aligned
+    const bool v4 = ((reinterpret_cast<uintptr_t>(d_y) | reinterpret_cast<uintptr_t>(d_t) |
+                      reinterpret_cast<uintptr_t>(d_g)) & 15) == 0;
+    long long chunk = (n + PC_MSE_BLOCKS - 1) / PC_MSE_BLOCKS;
+    chunk = v4 ? (chunk + 3) & ~3ll : chunk;
     const int nb = (int)((n + chunk - 1) / chunk);
     hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(PC_MSE_NT), 0, s, d_y, d_t, d_g, (float)(2.0 / (double)n), n, chunk,
-                       c->mse_part);
+                       v4 ? 1 : 0, c->mse_part);
     hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(PC_MSE_NT), 0, s, c->mse_part, nb, (float)(1.0 / (double)n), d_loss);
     HIPCHK(hipGetLastError());
     return PC_OK;

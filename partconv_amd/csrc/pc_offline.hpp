@@ -763,13 +763,27 @@ __device__ __forceinline__ float block_sum_fixed(float v, float *red, int tid)
 
 __global__ void __launch_bounds__(PC_MSE_NT)
 k_mse_grad(const float *__restrict__ y, const float *__restrict__ t, float *__restrict__ g, float gfac, long long n,
-           long long chunk, float *__restrict__ part)
+           long long chunk, int vec, float *__restrict__ part)
 {
     __shared__ float red[PC_MSE_NT];
     const int tid = threadIdx.x;
     const long long lo = (long long)blockIdx.x * chunk, hi = min(n, lo + chunk);
     float acc = 0.f;
-    for (long long i = lo + tid; i < hi; i += PC_MSE_NT) {
+    // float4 groups (vec: 16-byte aligned arrays and chunk a multiple of 4), the rest by scalars
+    const long long hi4 = vec ? lo + ((hi - lo) & ~3ll) : lo;
+    const float4 *y4 = reinterpret_cast<const float4 *>(y), *t4 = reinterpret_cast<const float4 *>(t);
+    float4 *g4 = reinterpret_cast<float4 *>(g);
+#pragma unroll 4
+    for (long long i = lo / 4 + tid; i < hi4 / 4; i += PC_MSE_NT) {
+        const float4 a = y4[i], b = t4[i];
+        const float4 d = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+        g4[i] = make_float4(gfac * d.x, gfac * d.y, gfac * d.z, gfac * d.w);
+        acc = fmaf(d.x, d.x, acc);
+        acc = fmaf(d.y, d.y, acc);
+        acc = fmaf(d.z, d.z, acc);
+        acc = fmaf(d.w, d.w, acc);
+    }
+    for (long long i = hi4 + tid; i < hi; i += PC_MSE_NT) {
         const float d = y[i] - t[i];
         g[i] = gfac * d;
         acc = fmaf(d, d, acc);
