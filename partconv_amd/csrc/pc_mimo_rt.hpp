@@ -81,62 +81,78 @@ struct MxrArgs {
 // partition.  The partial sums of an output are added in a fixed tree over
 // its consecutive lanes, so the result does not depend on timing.
 template <int NT, int PB>
-__device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, int k, uint64_t b, int p_lo,
-                                        int p_hi, const cf *base, cf *dst, int tid)
+__device__ __forceinline__ void mxr_bin_acc(const MxrLevel &L, int n_in, int n_out, int k, uint64_t b, int p_lo,
+                                            int p_hi, cf &acc, int tid)
 {
     constexpr int LPO = NT / 32, II = 32 / LPO;  // lanes per output, inputs per lane
     static_assert(II == 2 || II == 4, "256 or 512 threads");
     const int o = tid / LPO, ig = tid % LPO;
     const bool bin0 = (k == 0);
     const bool act = (o < n_out) && (II * ig < n_in);
-    cf acc = make_float2(0.f, 0.f);
-    cf s0 = make_float2(0.f, 0.f);
-    if (base && o < n_out && ig == 0) s0 = gload2(base + (size_t)o * L.N + k);
-    if (act) {
-        const int bm = (int)(b % (uint64_t)L.P);
-        for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
-            float4 hv[PB][II / 2], xv[PB][II / 2];
+    if (!act) return;
+    const int bm = (int)(b % (uint64_t)L.P);
+    for (int p0 = p_lo; p0 < p_hi; p0 += PB) {
+        float4 hv[PB][II / 2], xv[PB][II / 2];
 #pragma unroll
-            for (int j = 0; j < PB; j++) {
-                const int p = p0 + j;
-                if (p < p_hi) {
-                    int r = bm - p;
-                    if (r < 0) r += L.P;
-                    const float4 *h = reinterpret_cast<const float4 *>(
-                        L.H + (((size_t)k * L.P + p) * n_out + o) * L.ni + II * ig);
-                    const float4 *x = reinterpret_cast<const float4 *>(
-                        L.fdl + ((size_t)r * L.N + k) * L.ni + II * ig);
+        for (int j = 0; j < PB; j++) {
+            const int p = p0 + j;
+            if (p < p_hi) {
+                int r = bm - p;
+                if (r < 0) r += L.P;
+                const float4 *h = reinterpret_cast<const float4 *>(
+                    L.H + (((size_t)k * L.P + p) * n_out + o) * L.ni + II * ig);
+                const float4 *x = reinterpret_cast<const float4 *>(
+                    L.fdl + ((size_t)r * L.N + k) * L.ni + II * ig);
 #pragma unroll
-                    for (int q = 0; q < II / 2; q++) {
-                        hv[j][q] = h[q];
-                        xv[j][q] = x[q];
-                    }
+                for (int q = 0; q < II / 2; q++) {
+                    hv[j][q] = h[q];
+                    xv[j][q] = x[q];
                 }
             }
+        }
 #pragma unroll
-            for (int j = 0; j < PB; j++) {
-                if (p0 + j < p_hi) {
+        for (int j = 0; j < PB; j++) {
+            if (p0 + j < p_hi) {
 #pragma unroll
-                    for (int q = 0; q < II / 2; q++) {
-                        acc = pmac(acc, make_float2(xv[j][q].x, xv[j][q].y), make_float2(hv[j][q].x, hv[j][q].y),
-                                   bin0);
-                        acc = pmac(acc, make_float2(xv[j][q].z, xv[j][q].w), make_float2(hv[j][q].z, hv[j][q].w),
-                                   bin0);
-                    }
+                for (int q = 0; q < II / 2; q++) {
+                    acc = pmac(acc, make_float2(xv[j][q].x, xv[j][q].y), make_float2(hv[j][q].x, hv[j][q].y),
+                               bin0);
+                    acc = pmac(acc, make_float2(xv[j][q].z, xv[j][q].w), make_float2(hv[j][q].z, hv[j][q].w),
+                               bin0);
                 }
             }
         }
     }
+}
+
+template <int NT>
+__device__ __forceinline__ void mxr_bin_finish(const MxrLevel &L, int n_out, int k, cf acc, const cf *base, cf *dst,
+                                               int tid)
+{
+    constexpr int LPO = NT / 32;
+    const int o = tid / LPO, ig = tid % LPO;
 #pragma unroll
     for (int d = 1; d < LPO; d <<= 1) {
         acc.x += __shfl_xor(acc.x, d);
         acc.y += __shfl_xor(acc.y, d);
     }
     if (o < n_out && ig == 0) {
-        acc.x += s0.x;
-        acc.y += s0.y;
+        if (base) {
+            const cf s0 = gload2(base + (size_t)o * L.N + k);
+            acc.x += s0.x;
+            acc.y += s0.y;
+        }
         dst[(size_t)o * L.N + k] = acc;
     }
+}
+
+template <int NT, int PB>
+__device__ __forceinline__ void mxr_bin(const MxrLevel &L, int n_in, int n_out, int k, uint64_t b, int p_lo,
+                                        int p_hi, const cf *base, cf *dst, int tid)
+{
+    cf acc = make_float2(0.f, 0.f);
+    mxr_bin_acc<NT, PB>(L, n_in, n_out, k, b, p_lo, p_hi, acc, tid);
+    mxr_bin_finish<NT>(L, n_out, k, acc, base, dst, tid);
 }
 
 // Workgroup-wide bounded wait until *ctr >= target (thread 0 polls), then a
@@ -415,11 +431,29 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                 __hip_atomic_store(A.flags + (size_t)o * 8, (unsigned long long)(f + 1), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_SYSTEM);
         } else {
-            // ---- delay-line sum for tick f + 1: partitions 1 .. P-1 of this workgroup's bins
+            // ---- delay-line sum for tick f + 1 of this workgroup's bins.  Partitions
+            // 2 .. P-1 read delay-line rows of earlier ticks (visible since this
+            // workgroup's acquire at the previous tick), so they are summed right
+            // after the release; partition 1, this tick's row, once every input has
+            // arrived, then the lanes' sums are combined.
+            constexpr int KB = (N0 + MXR_NTT - 1) / MXR_NTT;  // bins per workgroup at most (G4 >= N0 / KB)
+            cf acc[KB];
+#pragma unroll
+            for (int j = 0; j < KB; j++) {
+                acc[j] = make_float2(0.f, 0.f);
+                const int k = w - n_io + j * G4;
+                if (k < N0) mxr_bin_acc<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 2, L0.P, acc[j], tid);
+            }
             if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
             cf *Sn = L0.S + (size_t)((f + 1) & 1) * A.n_out * N0;
-            for (int k = w - n_io; k < N0; k += G4)
-                mxr_bin<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 1, L0.P, nullptr, Sn, tid);
+#pragma unroll
+            for (int j = 0; j < KB; j++) {
+                const int k = w - n_io + j * G4;
+                if (k < N0) {
+                    mxr_bin_acc<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 1, min(2, L0.P), acc[j], tid);
+                    mxr_bin_finish<MXR_NTT>(L0, A.n_out, k, acc[j], nullptr, Sn, tid);
+                }
+            }
             mxr_arrive(cS + 8 * ((w - n_io) % MXR_SG), tid);  // group counters: 8 lines
         }
         __syncthreads();
