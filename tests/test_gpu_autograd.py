@@ -444,6 +444,37 @@ def test_fused_mse_loss_matches_fp64_and_replays_in_a_graph():
         assert torch.equal(hs.grad, he.grad), r
 
 
+def test_fused_mse_kernel_unaligned_and_ragged():
+    """pcc_mse on arrays that are not 16-byte aligned (the scalar path) and on
+    a sample count that is not a multiple of 4 or of the block count: the loss
+    and the gradient against fp64, and the aligned path equal to the scalar one
+    on the same values within fp32 rounding of the loss's sum."""
+    import ctypes
+    import torch
+    from partconv_amd import lib
+    from partconv_amd.autograd import PartitionedConv
+    conv = PartitionedConv([64], [2], 100)
+    rng = np.random.default_rng(31)
+    s = torch.cuda.current_stream().cuda_stream
+    for C, T, off in ((1, 12345, 1), (3, 1001, 3), (2, 4096, 0), (1, 7, 2)):
+        n = C * T
+        yb = torch.from_numpy(rng.standard_normal(n + 4).astype(np.float32)).cuda()
+        tb = torch.from_numpy(rng.standard_normal(n + 4).astype(np.float32)).cuda()
+        gb = torch.zeros(n + 4, device="cuda")
+        loss = torch.zeros(1, device="cuda")
+        y, t, g = yb[off:off + n], tb[off:off + n], gb[off:off + n]
+        rc = lib().pcc_mse(conv._h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(t.data_ptr()),
+                           ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(loss.data_ptr()), C, T, ctypes.c_void_p(s))
+        assert rc == 0
+        torch.cuda.synchronize()
+        d = y.cpu().numpy().astype(np.float64) - t.cpu().numpy().astype(np.float64)
+        l64 = float(np.mean(d ** 2))
+        assert abs(float(loss) - l64) / l64 < 1e-5, (C, T, off)
+        assert relmax(g.cpu().numpy(), 2.0 * d / n) < 1e-6, (C, T, off)
+        # nothing outside the gradient's range was written
+        assert float(gb[:off].abs().sum()) == 0.0 and float(gb[off + n:].abs().sum()) == 0.0
+
+
 def test_tuned_chunk_sizes_keep_results():
     """pcc_tune picks each level's chunk sizes from its candidates by timing;
     the forward output is bitwise that of the untuned form, and both gradients
