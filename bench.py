@@ -367,7 +367,8 @@ def bench_uneven(args):
         "config": {"workload": f"{n} instances, IR lengths log-uniform in [2^12, 2^20], greedy partitions "
                                "over 128..32768, buffer 128 at 48 kHz, batched engine, persistent tick kernel",
                    "instances": n, "total_taps": int(np.sum(lens)), "setup_s": t_setup},
-        "tuning": {k: tuning[k] for k in ("workers", "bins_per_thread", "from_cache", "measured") if k in tuning},
+        "tuning": {k: tuning[k] for k in ("workers", "bins_per_thread", "release_window", "from_cache", "measured")
+                   if k in tuning},
         "latency_us": {"p50": st.p50_us, "p99": st.p99_us, "max": st.max_us, "period": st.period_us,
                        "misses": int(st.deadline_misses), "ticks": int(st.ticks),
                        "persistent_ticks": int(st.persistent_ticks)},
@@ -898,6 +899,17 @@ def main():
     lo = args.warmup * TICKS_PER_STEP * N0
     err = float(np.max(np.abs(y[lo:hi, c] - r[lo:hi])) / np.sqrt(np.mean(r[lo:hi] ** 2)))
 
+    # the same persistent kernel with one tick released at a time (no release window)
+    window = int(tuning.get("release_window", 1))
+    serial = None
+    if window > 1:
+        mb.reset()
+        mb.set_release_window(1)
+        ssteps = max(1, args.steps // 4)
+        ssecs, sst = timed_run(mb, xd, yd, 1, ssteps, torch, persistent=True)
+        serial = {"value": CHANNELS * ssteps * TICKS_PER_STEP * N0 / ssecs, "p50_us": sst.p50_us,
+                  "p99_us": sst.p99_us, "misses": int(sst.deadline_misses), "steps": ssteps}
+        mb.set_release_window(window)
     # batched engine with one tick-kernel launch per tick (same outputs)
     mb.reset()
     lsecs, lst = timed_run(mb, xd, yd, 1, max(1, args.steps // 4), torch, persistent=False)
@@ -926,11 +938,13 @@ def main():
         "config": {"workload": "16 channels x 262144-tap IRs, 48 kHz, partition [64x6,256x14,2048x14,16384x15], "
                                "buffer 64, inputs resident in HBM, batched engine, persistent tick kernel",
                    "channels": CHANNELS, "taps": TAPS, "buffer_size": N0,
-                   "ticks_per_step": TICKS_PER_STEP, "release_word": release},
-        "tuning": {k: tuning[k] for k in ("workers", "bins_per_thread", "from_cache", "measured") if k in tuning},
+                   "ticks_per_step": TICKS_PER_STEP, "release_word": release, "release_window": window},
+        "tuning": {k: tuning[k] for k in ("workers", "bins_per_thread", "release_window", "from_cache", "measured")
+                   if k in tuning},
         "latency_us": {"p50": st.p50_us, "p99": st.p99_us, "max": st.max_us, "period": st.period_us,
                        "misses": int(st.deadline_misses), "ticks": int(st.ticks),
                        "persistent_ticks": int(st.persistent_ticks)},
+        "persistent_one_tick_released_at_a_time": serial,
         "batched_launch_per_tick": {"value": lval, "p50_us": lst.p50_us, "p99_us": lst.p99_us,
                                     "misses": int(lst.deadline_misses)},
         "p99_below_period": bool(st.p99_us < st.period_us),
@@ -938,8 +952,10 @@ def main():
         "roofline": workload_roofline(tick_bytes(parts, N0), ticks, secs, CHANNELS * N0,
                                       "whole timed region: k_persist<6> resident for every tick (level 0 and the "
                                       "256 level inside it) + the 2048 and 16384 levels' kernels beside it",
-                                      "serial host round trip per tick (release -> completion flags): one tick "
-                                      "moves about 0.9 MB"),
+                                      "per-tick latency chain, not HBM: one tick moves about 0.9 MB; with the "
+                                      "release window the host round trip overlaps the device's ticks and each "
+                                      "tick workgroup's own chain (frame and level loads, the 64-point transforms, "
+                                      "the handshake with the 256 level's worker every 4 ticks) bounds the rate"),
         "kernel_rooflines": {
             "k_partial": with_traffic(kernel_roofline(avg_us, launches, bytes_per,
                                                       "k_partial (16384-level delay-line sum over 14 partitions, "

@@ -207,6 +207,17 @@ int pcm_set_host_session(pc_multi *h, int on);
 int pcm_run_device(pc_multi *h, float *d_out, const float *d_in, long long nframes,
                    int sync_each_tick);
 
+/* Release window of pcm_run_device's persistent mode (sync_each_tick = 2),
+ * 1 .. 16 ticks (default 1): up to this many ticks are released ahead of the
+ * oldest tick not yet complete, so the host's completion poll and the next
+ * release overlap the device's work on released ticks.  Applies where each
+ * instance writes its own output channel (direct mix with per-workgroup
+ * completion flags); else 1.  Deadlines are unchanged: a tick is released only
+ * once the level blocks it reads first are complete.  Latency counts from each
+ * tick's release.  Identical output for every window.  No reference
+ * counterpart (the reference's run() serves one tick per call). */
+int pcm_set_release_window(pc_multi *h, int ticks);
+
 /* PartConvMulti::reset (src/partconvMulti.cpp:806-813).  Unlike the
  * reference, also rewinds the global level counters and frameNum. */
 int pcm_reset(pc_multi *h);

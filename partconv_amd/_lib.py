@@ -72,6 +72,7 @@ SIGNATURES = {
     "pcm_trace": (ctypes.c_int, [_vp, ctypes.c_longlong]),
     "pcm_set_persistent_workers": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_set_host_session": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pcm_set_release_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcm_process": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong]),
     "pcm_setup_max": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]),

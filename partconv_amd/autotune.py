@@ -13,6 +13,9 @@ cache keyed by (device, workload shape):
     kernels) -- chosen by the mean tick latency of persistent device-resident
     ticks; then the delay-line-sum kernel's width (pcm_set_partial_width: 2, 4
     or 8 bins per thread) -- chosen by its HIP-event time on the longest level;
+    then the release window of persistent device-resident runs
+    (pcm_set_release_window: 1 to 4 ticks released ahead) -- chosen by the wall
+    time per tick, so the smallest window within the tie of the fastest wins;
   * differentiable form (PartitionedConv): each level's chunk sizes of the
     spectral products and of dL/dh (pcc_tune), replayed from the cache with
     pcc_set_level_tuning.
@@ -41,7 +44,8 @@ from ._lib import last_error, lib
 TIE = 0.05
 WORKER_MODES = (1, 2, 0)      # default first
 PARTIAL_WIDTHS = (2, 4, 8)    # default first
-VERSION = 1
+RELEASE_WINDOWS = (1, 2, 3, 4)  # default first
+VERSION = 2
 
 
 def default_cache_path():
@@ -105,25 +109,27 @@ def engine_key(m, device=None):
             f"|inst={info.numPCs}|parts={digest}")
 
 
-def _apply_engine(m, workers, width):
+def _apply_engine(m, workers, width, window=1):
     if lib().pcm_set_persistent_workers(m._h, int(workers)) != 0:
         raise RuntimeError(last_error())
     if lib().pcm_set_partial_width(m._h, int(width)) != 0:
         raise RuntimeError(last_error())
+    if lib().pcm_set_release_window(m._h, int(window)) != 0:
+        raise RuntimeError(last_error())
 
 
 def tune_engine(m, nticks=None, cache=None, force=False):
-    """Choose the persistent-run workers mode and the delay-line-sum width of a
-    set-up PartConvMulti (batched engine) on this device; apply them; return
-    {"workers": w, "bins_per_thread": b, "from_cache": bool, "key": ...,
-    "measured": {...}}.  A measurement runs persistent device-resident ticks
+    """Choose the persistent-run workers mode, the delay-line-sum width and the
+    release window of a set-up PartConvMulti (batched engine) on this device;
+    apply them; return {"workers": w, "bins_per_thread": b, "release_window":
+    r, "from_cache": bool, "key": ..., "measured": {...}}.  A measurement runs persistent device-resident ticks
     of zero input and resets the engine afterwards (state and frame counter)."""
     import torch
     path = default_cache_path() if cache is None else pathlib.Path(cache)
     key = engine_key(m)
     ent = load_cache(path)["entries"].get(key)
     if ent and not force:
-        _apply_engine(m, ent["workers"], ent["bins_per_thread"])
+        _apply_engine(m, ent["workers"], ent["bins_per_thread"], ent["release_window"])
         return dict(ent, from_cache=True, key=key)
     info = m.info()
     n0, G = info.buffer_size, info.num_levels
@@ -142,7 +148,7 @@ def tune_engine(m, nticks=None, cache=None, force=False):
             raise RuntimeError(last_error())
         return m.stats()
 
-    measured = {"workers_mean_us": {}, "partial_us": {}}
+    measured = {"workers_mean_us": {}, "partial_us": {}, "window_us_per_tick": {}}
     costs = []
     for w in WORKER_MODES:
         _apply_engine(m, w, PARTIAL_WIDTHS[0])
@@ -165,11 +171,28 @@ def tune_engine(m, nticks=None, cache=None, force=False):
         costs.append(avg_us if n else 0.0)
         measured["partial_us"][str(b)] = avg_us if n else None
     width = choose(list(PARTIAL_WIDTHS), costs, PARTIAL_WIDTHS[0]) if any(costs) else PARTIAL_WIDTHS[0]
-    _apply_engine(m, workers, width)
+    # release window: wall time per tick of persistent runs (the engine's reset
+    # outside the timed call; every call pays the same launch)
+    costs = []
+    for r in RELEASE_WINDOWS:
+        _apply_engine(m, workers, width, r)
+        run(nwide)                              # warm-up
+        m.reset()
+        m.clear_stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if m.run_device(y.data_ptr(), x.data_ptr(), nwide, True, persistent=True) != 0:
+            raise RuntimeError(last_error())
+        torch.cuda.synchronize()
+        us = 1e6 * (time.perf_counter() - t0) / nwide
+        costs.append(us)
+        measured["window_us_per_tick"][str(r)] = us
+    window = choose(list(RELEASE_WINDOWS), costs, RELEASE_WINDOWS[0])
+    _apply_engine(m, workers, width, window)
     m.reset()
     m.clear_stats()
-    ent = {"workers": workers, "bins_per_thread": width, "measured": measured, "ticks": [nticks, nwide],
-           "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    ent = {"workers": workers, "bins_per_thread": width, "release_window": window, "measured": measured,
+           "ticks": [nticks, nwide], "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
     store_entry(path, key, ent)
     return dict(ent, from_cache=False, key=key)
 

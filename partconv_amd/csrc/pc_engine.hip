@@ -46,6 +46,7 @@
 #define PC_SPIN_LIMIT 200000000ull  // persistent-kernel waits: 2 s at 100 MHz (s_memrealtime)
 #define PC_SESSION_IDLE_S 1.0       // host sessions restart after this long without a release
 #define PC_TICK_TIMEOUT_S 10.0
+#define PC_RELEASE_WINDOW_MAX 16
 #define PC_LL_MAX_WORDS 16384       // host sessions use tagged input words up to nIn * N0 of these
 #define PC_WG_FLAGS_MAX 1024        // direct mix: per-workgroup host flags up to this many instances
                                     // (the host polls them; beyond, device tickets raise one flag)
@@ -646,6 +647,9 @@ struct Engine {
     int sess_host = 0;                // the active session serves host buffers
     int persist_workers = 1;          // short levels in persistent runs: 1 in-kernel, 2 self mode, 0 host
     double pend_t0 = -1;              // submit time of a tick a stalled session left unserved (-1: none)
+    int release_window = 1;           // device-resident persistent runs: ticks in flight (pcm_set_release_window)
+    uint64_t win_rel = 0;             // release-window run: first tick not released (0: none released)
+    std::vector<double> win_trel;     // release-window run: release time of tick f at f % window
     int ncu = 0;                      // compute units of the device
     int part_q = 1;                   // k_partial float4 groups per thread (pcm_set_partial_width)
     bool level_graphs = false;        // level blocks launched as graphs (pcm_set_level_graphs)
@@ -2130,14 +2134,40 @@ static int persist_end(Engine &E, bool stop)
     const int lrc = launcher_drain(E);
     std::fill(E.lvl_async.begin(), E.lvl_async.end(), false);
     if (stop) {
-        store_go(E, PC_GO_STOP);
+        // the ticks a release-window run released (f < win_rel) are served
+        // before the kernel stops; otherwise it stops at once
+        const uint64_t drain = E.win_rel;
+        E.win_rel = 0;
+        store_go(E, PC_GO_STOP | drain);
         HIPCHK(hipStreamSynchronize(E.s_main));
         // a stopped tick may have counted only some of its arrivals
         HIPCHK(hipMemsetAsync(E.d_ticket, 0, 64, E.s_main));
         HIPCHK(hipMemsetAsync(E.d_ticket_ch, 0, std::max(E.nOut, 1) * sizeof(unsigned), E.s_main));
         HIPCHK(hipMemsetAsync(E.d_ticket_grp, 0, E.ticket_grp_bytes, E.s_main));
         HIPCHK(hipStreamSynchronize(E.s_main));
-        if (E.frame < E.sess_end && tick_done(E, E.frame + 1)) {
+        if (drain > E.frame) {
+            // every released tick the kernel completed, in order, from its release
+            const size_t W = E.win_trel.size();
+            while (E.frame < drain && tick_done(E, E.frame + 1)) {
+                const uint64_t f = E.frame;
+                record_latency(E, 1e6 * (now_s() - E.win_trel[f % W]));
+                E.frame++;
+                E.sess_served_late = true;
+                int rc = start_levels(E, f, true);
+                if (rc) return rc;
+            }
+            if (E.frame < drain) {
+                // A workgroup ends only at a tick boundary.  Per-tick launches
+                // from tick E.frame serve every instance again, which repeats
+                // exactly what a workgroup already did for tick E.frame but not
+                // what it did for a later tick: refuse that state.
+                const volatile unsigned long long *wf = E.h_wg;
+                for (int i = 0; i < E.nwg; i++)
+                    if (wf[(size_t)i * PC_WG_FLAG_STRIDE] > E.frame + 1)
+                        return fail(PC_ERR_STATE, "persistent run ended with workgroups past the first unserved tick");
+                E.pend_t0 = E.win_trel[E.frame % W];
+            }
+        } else if (E.frame < E.sess_end && tick_done(E, E.frame + 1)) {
             // the kernel served the released tick after all (slow, not stalled)
             const uint64_t f = E.frame;
             if (E.pend_t0 >= 0) record_latency(E, 1e6 * (now_s() - E.pend_t0));
@@ -2148,6 +2178,7 @@ static int persist_end(Engine &E, bool stop)
             if (rc) return rc;
         }
     }
+    E.win_rel = 0;
     E.wg_persist = false;  // per-tick launches use the packed flags
     E.ll_active = false;
     if (E.workers_active) {
@@ -2156,6 +2187,90 @@ static int persist_end(Engine &E, bool stop)
         if (rc) return rc;
     }
     return lrc;
+}
+
+// Release window: the host-launched level blocks that tick r reads for the
+// first time are complete (0), not yet (1: a block still runs, or its start
+// tick has not been seen complete, so it is not launched yet), or an error.
+static int window_levels_ready(Engine &E, uint64_t r)
+{
+    const int G = (int)E.gsize.size();
+    for (int g = 1; g < G; g++) {
+        if (!level_due(E, g, r)) continue;
+        if (E.workers_active && E.g_worker[g]) continue;  // the kernel's own workers
+        const uint64_t k = (uint64_t)E.gsize[g];
+        const uint64_t s = E.relaxed ? r - k : r + 1 - k;  // the block's start tick
+        if (s >= E.frame) return 1;
+        if (!E.lvl_pending[g]) continue;
+        if (E.lvl_async[g]) {
+            if (E.lau->done[g].load(std::memory_order_acquire) < E.lvl_start[g] + 1) return 1;
+            E.lvl_async[g] = false;
+        }
+        const hipError_t q = hipEventQuery(E.ev_lvl[g]);
+        if (q == hipErrorNotReady) return 1;
+        if (q != hipSuccess) return fail(PC_ERR_HIP, std::string("level block: ") + hipGetErrorString(q));
+        E.lvl_pending[g] = false;
+    }
+    return 0;
+}
+
+// Device-resident ticks with a release window of W = release_window ticks
+// (direct mix with a completion flag per workgroup: each workgroup writes its
+// own output of each tick, so workgroups may be at different ticks).  Up to W
+// ticks are released ahead of the oldest one not yet seen complete, so the
+// host's round trip (the completion flags across the bus, then the next
+// release) overlaps the device's work on the ticks already released; every
+// workgroup still runs its ticks in order.  The deadline rule is that of
+// persist_tick: tick r is released only once every host-launched level block
+// it reads first is complete, and such a block is launched once its start
+// tick is seen complete, so the window narrows where a level's period is
+// short.  A tick's latency counts from its release.  Returns 0 when the ticks
+// to fend are served, 1 on a stall (win_rel: the first tick not released;
+// persist_end stops the kernel after the released ticks), or an error.
+static int persist_run_window(Engine &E, uint64_t fend)
+{
+    const uint64_t W = (uint64_t)E.release_window;
+    E.win_trel.assign((size_t)W, 0.0);
+    E.win_rel = E.frame;
+    double t_idle = now_s();  // the window emptied: tick win_rel's submission
+    double t_wait = -1;       // the oldest tick's completion first found pending
+    unsigned spins = 0;
+    while (E.frame < fend) {
+        while (E.win_rel < fend && E.win_rel < E.frame + W) {
+            const int lr = window_levels_ready(E, E.win_rel);
+            if (lr == 1) break;
+            if (lr) return lr;
+            store_go(E, E.win_rel + 1);
+            E.win_trel[E.win_rel % W] = E.sess_last = now_s();
+            E.win_rel++;
+        }
+        if (E.win_rel == E.frame) {
+            // nothing in flight: tick win_rel waits for a level block
+            if (now_s() - t_idle > PC_LEVEL_LATE_S) {
+                E.pend_t0 = t_idle;
+                return 1;
+            }
+            continue;
+        }
+        if (!tick_done(E, E.frame + 1)) {
+            if (t_wait < 0) t_wait = now_s();
+            if ((++spins & 0x3FFF) == 0) {
+                const hipError_t q = hipStreamQuery(E.s_main);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    return fail(PC_ERR_HIP, std::string("tick failed: ") + hipGetErrorString(q));
+                if ((q == hipSuccess && !tick_done(E, E.frame + 1)) || now_s() - t_wait > PC_TICK_TIMEOUT_S)
+                    return 1;
+            }
+            continue;
+        }
+        t_wait = -1;
+        record_latency(E, 1e6 * (now_s() - E.win_trel[E.frame % W]));
+        const uint64_t f = E.frame++;
+        const int rc = start_levels(E, f, false);
+        if (rc) return rc;
+        if (E.win_rel == E.frame) t_idle = now_s();
+    }
+    return PC_OK;
 }
 
 // Device-resident ticks served by one persistent tick kernel (batched engine);
@@ -2169,7 +2284,16 @@ static int engine_run_persistent(Engine &E, const float *d_in, float *d_out, lon
     int rc = persist_begin(E, d_in, in_stride, d_out, out_stride, nframes, 0);
     if (rc) return rc;  // 1: per-tick launches
     bool stalled = false;
-    while (E.frame < f0 + (uint64_t)nframes) {
+    if (E.release_window > 1 && E.use_wgflags && !E.ll_active) {
+        rc = persist_run_window(E, f0 + (uint64_t)nframes);
+        if (rc == 1) {
+            stalled = true;
+            E.stalls++;
+        } else if (rc) {
+            return rc;
+        }
+    }
+    while (!stalled && E.frame < f0 + (uint64_t)nframes) {
         rc = persist_tick(E);
         if (rc == 1) {
             stalled = true;
@@ -2742,6 +2866,14 @@ PC_API int pcm_get_tuning(const pc_multi *h, int *workers_mode, int *bins_per_th
     if (!h) return fail(PC_ERR_ARG, "null handle");
     if (workers_mode) *workers_mode = h->E.persist_workers;
     if (bins_per_thread) *bins_per_thread = 2 * h->E.part_q;
+    return PC_OK;
+}
+
+PC_API int pcm_set_release_window(pc_multi *h, int ticks)
+{
+    if (!h) return fail(PC_ERR_ARG, "null handle");
+    if (ticks < 1 || ticks > PC_RELEASE_WINDOW_MAX) return fail(PC_ERR_ARG, "release window must be 1 .. 16 ticks");
+    h->E.release_window = ticks;
     return PC_OK;
 }
 

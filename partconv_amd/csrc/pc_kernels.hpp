@@ -829,20 +829,37 @@ __device__ __forceinline__ cf w64_inv(cf v, const W64 &w, int lane)
     return w64_inv_stage<32, 0>(v, w, lane);
 }
 
+// Level 0's partitions 1 .. T64_R of the 64-point tick live in registers: a
+// lane's delay-line sum only reads its own bin's filter spectra and its own
+// bin's spectra of the last ticks (which it wrote itself), so they stay in the
+// lane between ticks; partitions past T64_R are read from memory.
+#define T64_R 8
 struct TickState64 {
     W64 w;
     cf h0, sp;       // H_0 and S of bin w.k
     float2 pv;       // previous frame samples 2*lane, 2*lane+1 (lanes < 32)
+    cf hr[T64_R];    // H_1 .. H_T64_R of bin w.k (those with p < P)
+    cf xq[T64_R - 1];  // X_{f-1}, X_{f-2}, ... of bin w.k (delay-line rows of the last ticks)
 };
 
 __device__ __forceinline__ void tick64_load_state(const LevelJob &J, uint64_t f, int lane, const cf *__restrict__ tw,
                                                   TickState64 &st)
 {
     w64_setup(st.w, tw, lane);
-    st.h0 = J.H[st.w.k];
-    st.sp = J.S[st.w.k];
+    const int k = st.w.k, P = J.P;
+    st.h0 = J.H[k];
+    st.sp = J.S[k];
     const float *prev = J.ring + (((long long)f * 64 - 64) & (long long)J.ring_mask);
     st.pv = (lane < 32) ? make_float2(prev[2 * lane], prev[2 * lane + 1]) : make_float2(0.f, 0.f);
+    const int slot = (int)(f % (uint64_t)P);
+#pragma unroll
+    for (int j = 0; j < T64_R; j++) st.hr[j] = (j + 1 < P) ? J.H[(size_t)(j + 1) * 64 + k] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < T64_R - 1; j++) {
+        int r = slot - 1 - j;  // row of tick f - 1 - j
+        if (r < 0) r += P;
+        st.xq[j] = (j + 2 < P) ? J.fdl[(size_t)r * 64 + k] : make_float2(0.f, 0.f);
+    }
 }
 
 // Phase A for buffer_size 64; leaves the instance's output sample `lane` in
@@ -950,9 +967,13 @@ __device__ __forceinline__ void tick64_phaseB(const TickJob &TJ, uint64_t f, cf 
     const int k = st.w.k;
     const int slot = (int)(f % (uint64_t)P);
     J.fdl[(size_t)slot * 64 + k] = xr;
-    cf s = (P > 1) ? pmul(xr, J.H[64 + k], k == 0) : make_float2(0.f, 0.f);
-    // partitions 2..P-1 in batches of 8: loads in flight together, summed in order
-    for (int p0 = 2; p0 < P; p0 += 8) {
+    // partitions 1 .. T64_R from registers, then the rest from memory in batches
+    // of 8 (loads in flight together), every bin summed in partition order
+    cf s = (P > 1) ? pmul(xr, st.hr[0], k == 0) : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < T64_R - 1; j++)
+        if (j + 2 < P) s = pmac(s, st.xq[j], st.hr[j + 1], k == 0);
+    for (int p0 = T64_R + 1; p0 < P; p0 += 8) {
         cf av[8], hv[8];
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -970,6 +991,9 @@ __device__ __forceinline__ void tick64_phaseB(const TickJob &TJ, uint64_t f, cf 
     }
     J.S[k] = s;
     st.sp = s;
+#pragma unroll
+    for (int j = T64_R - 2; j > 0; j--) st.xq[j] = st.xq[j - 1];
+    st.xq[0] = xr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1201,9 +1225,15 @@ k_tick(TickArgs A, const cf *__restrict__ tw)
 // k_tick with its per-instance state kept in registers and its job descriptor
 // in LDS, and publishes its output.  Every wait is bounded: a release that
 // does not come within `spin_limit` s_memrealtime ticks (100 MHz) ends the
-// kernel, and go = PC_GO_STOP ends it at once.
+// kernel, and go = PC_GO_STOP | r ends it at the first tick >= r (r = 0: at
+// once; r > 0: the ticks already released, f < r, are served first).
 // ---------------------------------------------------------------------------
 #define PC_GO_STOP (1ull << 62)
+// Tick f may run under release word g (stop bit or not).
+__device__ __forceinline__ bool go_released(unsigned long long g, uint64_t f)
+{
+    return (g & ~PC_GO_STOP) >= f + 1;
+}
 
 // Level workers: workgroups of the persistent kernel that run the blocks of
 // the short-period levels (k = N_L / N0 <= 8, N_L <= 2048), so those blocks
@@ -1633,31 +1663,38 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     if constexpr (LOGN == 6) tick64_load_state(sTJ.l0, P.f0, tid, tw, st64);
     else if constexpr (LOGN == 7) tick128_load_state(sTJ.l0, P.f0, tid, tw, st128, s_h1);
     else tick_load_state<LOGN>(sTJ.l0, P.f0, tid, st);
+    // release word last read by thread 0: when the host releases ticks ahead
+    // (a release window), the ticks it already covers need no new read
+    unsigned long long g_seen = 0;
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
         float llx[LLCnt<LOGN>::v];
         int ll_in = 0;
         if (P.A.ll) ll_in = ll_wait_frame<LOGN>(sTJ, P.A, P.go, f, tid, P.spin_limit, llx);
         if (tid == 0) {
-            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + P.spin_limit;
             int go = 0;
             // with many workgroups only workgroup 0 polls host memory over the bus;
             // it relays the counter through device memory to the others
             const bool host_poll = (P.dev_go == nullptr) || blockIdx.x == 0;
             if (P.A.ll) {
                 go = ll_in;  // the tagged input words were the release
+            } else if (go_released(g_seen, f)) {
+                go = 1;
             } else if (host_poll) {
-                const unsigned long long g = poll_host(P.go, f + 1, t_end);
+                const unsigned long long g = poll_host(P.go, f + 1, __builtin_amdgcn_s_memrealtime() + P.spin_limit);
                 if (P.dev_go && g) __hip_atomic_store(P.dev_go, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                go = (g != 0 && !(g & PC_GO_STOP));
+                go = go_released(g, f);
+                g_seen = g;
             } else {
+                const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + P.spin_limit;
                 for (;;) {
                     const unsigned long long g = __hip_atomic_load(P.dev_go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (g & PC_GO_STOP) break;
-                    if (g >= f + 1) {
+                    if (go_released(g, f)) {
                         go = 1;
+                        g_seen = g;
                         break;
                     }
+                    if (g & PC_GO_STOP) break;
                     if (__builtin_amdgcn_s_memrealtime() > t_end) break;
                     __builtin_amdgcn_s_sleep(1);
                 }

@@ -368,6 +368,12 @@ class PartConvMulti:
         mode = 1 if on is True else (0 if on is False else int(on))
         return _report(lib().pcm_set_host_session(self._h, mode), "set_host_session")
 
+    def set_release_window(self, ticks):
+        """Ticks the persistent device-resident run (run_device(..., persistent=True))
+        may release ahead of the oldest incomplete tick (1 .. 16, default 1),
+        where each instance has its own output channel; identical output."""
+        return _report(lib().pcm_set_release_window(self._h, int(ticks)), "set_release_window")
+
     def set_persistent_workers(self, mode):
         """Short-period levels in persistent runs: True/1 inside the kernel (worker
         workgroups, or self mode when they do not fit), 2 always self mode,

@@ -40,12 +40,13 @@ def test_engine_tuning_cached_and_output_unchanged(tmp_path):
     info = autotune.tune_engine(m, nticks=600, cache=cache)
     print("tuned:", info["workers"], info["bins_per_thread"], info["measured"], flush=True)
     assert not info["from_cache"] and info["workers"] in (0, 1, 2) and info["bins_per_thread"] in (2, 4, 8)
+    assert info["release_window"] in autotune.RELEASE_WINDOWS
     y = run(m, xd, nfr)
     assert torch.equal(y, y_ref)
     m2 = engine(irs)
     info2 = autotune.tune_engine(m2, cache=cache)
-    assert info2["from_cache"] and (info2["workers"], info2["bins_per_thread"]) == (info["workers"],
-                                                                                     info["bins_per_thread"])
+    assert info2["from_cache"] and (info2["workers"], info2["bins_per_thread"], info2["release_window"]) == (
+        info["workers"], info["bins_per_thread"], info["release_window"])
     import ctypes
     from partconv_amd._lib import lib
     w, b = ctypes.c_int(), ctypes.c_int()
