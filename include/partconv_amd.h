@@ -515,6 +515,13 @@ int pcxr_set_filters(pc_mimo_rt *r, const float *d_h);
 /* nframes ticks; d_in / d_out: frames of buffer_size x n_in / n_out interleaved
  * samples on the device (PartConvMulti::run's layout, frame after frame) */
 int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long long nframes);
+/* Release window of pcxr_run_device, 1 .. 3 ticks (default 1; at most the
+ * level-0 partition count): up to this many ticks are released ahead of the
+ * oldest tick whose outputs are not all complete, so the input transforms of
+ * the next ticks overlap the outputs of the current one and the host's round
+ * trip.  Deadlines and output unchanged; latency counts from each tick's
+ * release.  No reference counterpart. */
+int pcxr_set_release_window(pc_mimo_rt *r, int ticks);
 /* One tick on host buffers, interleaved as PartConvMulti::run's
  * (src/partconvMulti.cpp:815-928; frameCount == buffer_size, else an error):
  * what a real-time callback calls for a MIMO network. */

@@ -138,6 +138,7 @@ SIGNATURES = {
     "pcxr_clear_stats": (ctypes.c_int, [_vp]),
     "pcxr_partition": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int]),
     "pcxr_stall_level": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
+    "pcxr_set_release_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcxr_destroy": (None, [_vp]),
     "pcxr_probe": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pcxr_probe_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),

@@ -2224,7 +2224,8 @@ static int window_levels_ready(Engine &E, uint64_t r)
 // persist_tick: tick r is released only once every host-launched level block
 // it reads first is complete, and such a block is launched once its start
 // tick is seen complete, so the window narrows where a level's period is
-// short.  A tick's latency counts from its release.  Returns 0 when the ticks
+// short.  A tick's latency counts from its release (or, if it waited for a
+// level block with nothing in flight, from when the window emptied).  Returns 0 when the ticks
 // to fend are served, 1 on a stall (win_rel: the first tick not released;
 // persist_end stops the kernel after the released ticks), or an error.
 static int persist_run_window(Engine &E, uint64_t fend)
@@ -2241,7 +2242,10 @@ static int persist_run_window(Engine &E, uint64_t fend)
             if (lr == 1) break;
             if (lr) return lr;
             store_go(E, E.win_rel + 1);
-            E.win_trel[E.win_rel % W] = E.sess_last = now_s();
+            // latency from the release, or from when the window emptied if the
+            // tick waited for a level block (as one tick at a time counts it)
+            E.sess_last = now_s();
+            E.win_trel[E.win_rel % W] = (E.win_rel == E.frame) ? t_idle : E.sess_last;
             E.win_rel++;
         }
         if (E.win_rel == E.frame) {
@@ -4509,6 +4513,7 @@ struct pc_mimo_rt {
     int go_slot = -1;
     unsigned long long *d_bar = nullptr;
     uint64_t frame = 0;
+    int window = 1;  // pcxr_set_release_window: ticks released ahead of the oldest incomplete one
     std::vector<double> lat;
     unsigned long long ticks = 0, misses = 0;
     double lat_sum = 0, lat_max = 0;
@@ -4688,7 +4693,7 @@ static int mxr_setup(pc_mimo_rt *r)
         if ((rc = mxr_alloc(r, (void **)&H, (size_t)L.P * r->n_out * L.ni * nb))) return rc;
         L.H = H;
         // level 0's S twice: the persistent kernel double-buffers it by tick parity
-        if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)(l == 0 ? 2 : 1) * r->n_out * nb))) return rc;
+        if ((rc = mxr_alloc(r, (void **)&L.S, (size_t)(l == 0 ? MXR_SLOTS : 1) * r->n_out * nb))) return rc;
         if ((rc = mxr_alloc(r, (void **)&L.Y, (size_t)r->n_out * nb))) return rc;
         if (l > 0 && (rc = mxr_alloc(r, (void **)&L.ybuf, (size_t)r->n_out * 2 * L.N * sizeof(float)))) return rc;
     }
@@ -4885,6 +4890,9 @@ static int mxr_persist_launch(pc_mimo_rt *r, long long nticks)
     MxrArgs &A = r->A;
     A.f0 = r->frame;
     A.nticks = nticks;
+    // ticks in flight: at most MXR_SLOTS - 1 (the S slots) and at most P_0 (an
+    // input overwrites the delay-line row of tick f - P_0, read by its outputs)
+    A.window = std::max(1, std::min({r->window, MXR_SLOTS - 1, A.lv[0].P}));
     // the grid: input and output workgroups, then one delay-line-sum workgroup
     // per bin (at most 256)
     const int G = r->n_in + r->n_out + std::min(r->N0, 256);
@@ -4923,6 +4931,92 @@ static int mxr_level_wait(pc_mimo_rt *r, int l, double t0, double limit)
     }
 }
 
+// Release-window run of the real-time MIMO engine (pcxr_set_release_window):
+// up to A.window ticks are released ahead of the oldest tick whose outputs are
+// not all seen; the kernel's input workgroups wait for the delay-line sums that
+// still read the row they overwrite, and S has a slot per tick in flight.  The
+// deadline rule is that of the loop below: tick r is released only once the
+// level blocks it reads first are complete, and a block is launched once its
+// start tick is seen complete.  A block still late PC_LEVEL_LATE_S after the
+// window emptied stops the kernel (every released tick is served by then), the
+// host waits for it and relaunches, as below.  A tick's latency counts from its
+// release, or from when the window emptied if it waited for a level block.
+static int mxr_run_window(pc_mimo_rt *r, uint64_t nframes, double period)
+{
+    MxrArgs &A = r->A;
+    const uint64_t W = (uint64_t)A.window, fend = r->frame + nframes;
+    std::vector<double> trel((size_t)W, 0.0);
+    uint64_t rel = r->frame;
+    double t_idle = now_s(), t_wait = -1;
+    int oscan = 0;  // outputs of the oldest tick seen done so far
+    while (r->frame < fend) {
+        while (rel < fend && rel < r->frame + W) {
+            bool ready = true;
+            for (int l = 1; l < r->M && ready; l++) {
+                const uint64_t k = 1ull << A.lv[l].kshift;
+                if ((rel + 1) % k != 0 || rel + 1 < k) continue;  // not a first read
+                if (rel + 1 - k >= r->frame) {
+                    ready = false;  // the block's start tick is not seen complete: not launched yet
+                    break;
+                }
+                if (!r->pending[l]) continue;
+                const int w = mxr_level_wait(r, l, now_s(), 0.0);
+                if (w == 1) ready = false;
+                else if (w) return w;
+                else r->pending[l] = false;
+            }
+            if (!ready) break;
+            trel[rel % W] = (rel == r->frame) ? t_idle : now_s();
+            mxr_store_go(r, rel + 1);
+            rel++;
+        }
+        if (rel == r->frame) {
+            // nothing in flight: tick rel waits for a level block
+            if (now_s() - t_idle <= PC_LEVEL_LATE_S) continue;
+            mxr_store_go(r, PC_GO_STOP);
+            HIPCHK(hipStreamSynchronize(r->s_main));
+            for (int l = 1; l < r->M; l++) {
+                const uint64_t k = 1ull << A.lv[l].kshift;
+                if ((rel + 1) % k != 0 || !r->pending[l]) continue;
+                const int w = mxr_level_wait(r, l, now_s(), PC_TICK_TIMEOUT_S);
+                if (w == 1) return fail(PC_ERR_TIMEOUT, "level block did not complete");
+                if (w) return w;
+                r->pending[l] = false;
+            }
+            r->stalls++;
+            const int rc = mxr_persist_launch(r, (long long)(fend - r->frame));
+            if (rc) return rc;
+            continue;  // t_idle stays: the late tick's latency counts from it
+        }
+        // the oldest tick in flight: every output's flag
+        const uint64_t f = r->frame;
+        while (oscan < r->n_out && *(const volatile unsigned long long *)(r->h_flags + (size_t)oscan * 8) >= f + 1)
+            oscan++;
+        if (oscan < r->n_out) {
+            if (t_wait < 0) t_wait = now_s();
+            else if (now_s() - t_wait > PC_TICK_TIMEOUT_S) return fail(PC_ERR_TIMEOUT, "tick did not complete");
+            continue;
+        }
+        oscan = 0;
+        t_wait = -1;
+        const double us = 1e6 * (now_s() - trel[f % W]);
+        r->lat.push_back(us);
+        r->ticks++;
+        r->lat_sum += us;
+        r->lat_max = std::max(r->lat_max, us);
+        if (us > period) r->misses++;
+        r->frame++;
+        for (int l = 1; l < r->M; l++)
+            if (f % (1ull << A.lv[l].kshift) == 0) {
+                r->lstart[l] = f;
+                r->pending[l] = true;
+                mxr_launcher_queue(r, l, f);
+            }
+        if (rel == r->frame) t_idle = now_s();
+    }
+    return PC_OK;
+}
+
 // d_in / d_out: nframes frames of buffer_size x n_in / n_out interleaved
 // samples on the device.  Every tick is released and waited for (real time).
 // The strict run of the reference waits for late levels
@@ -4952,6 +5046,10 @@ PC_API int pcxr_run_device(pc_mimo_rt *r, float *d_out, const float *d_in, long 
     if (r->M > 1) mxr_launcher_start(r);
     const double period = 1e6 * N0 / r->FS;
     int err = PC_OK;
+    if (A.window > 1) {
+        err = mxr_run_window(r, (uint64_t)nframes, period);
+        nframes = 0;  // served (or failed) above
+    }
     for (long long t = 0; t < nframes && !err; t++) {
         const uint64_t f = r->frame;
         const double t0 = now_s();
@@ -5051,7 +5149,7 @@ PC_API int pcxr_reset(pc_mimo_rt *r)
         const MxrLevel &L = A.lv[l];
         const size_t nb = (size_t)L.N * sizeof(cf);
         HIPCHK(hipMemsetAsync(L.fdl, 0, (size_t)L.ni * L.P * nb, r->s_main));
-        HIPCHK(hipMemsetAsync(L.S, 0, (size_t)(l == 0 ? 2 : 1) * r->n_out * nb, r->s_main));
+        HIPCHK(hipMemsetAsync(L.S, 0, (size_t)(l == 0 ? MXR_SLOTS : 1) * r->n_out * nb, r->s_main));
         if (L.ybuf) HIPCHK(hipMemsetAsync(L.ybuf, 0, (size_t)r->n_out * 2 * L.N * sizeof(float), r->s_main));
     }
     HIPCHK(hipStreamSynchronize(r->s_main));
@@ -5133,6 +5231,14 @@ PC_API int pcxr_probe_read(pc_mimo_rt *r, double *avg_us, long long *launches, d
         }
         *bytes_per_launch = b;
     }
+    return PC_OK;
+}
+
+PC_API int pcxr_set_release_window(pc_mimo_rt *r, int ticks)
+{
+    if (!r) return fail(PC_ERR_ARG, "null handle");
+    if (ticks < 1 || ticks > MXR_SLOTS - 1) return fail(PC_ERR_ARG, "release window must be 1 .. 3 ticks");
+    r->window = ticks;
     return PC_OK;
 }
 

@@ -46,6 +46,11 @@ struct MxrLevel {
     float *ybuf;
 };
 
+// Level 0's delay-line sums S are kept for MXR_SLOTS ticks: tick f reads slot
+// f % MXR_SLOTS, its delay-line-sum workgroups write slot (f + 1) % MXR_SLOTS,
+// whose last reader (tick f + 1 - MXR_SLOTS) is complete whenever the host
+// releases at most MXR_SLOTS - 1 ticks ahead of the oldest incomplete one.
+#define MXR_SLOTS 4
 struct MxrArgs {
     int n_in, n_out, N0, M;
     float *ring;            // [n_in][2R]
@@ -67,6 +72,7 @@ struct MxrArgs {
                                 // before each launch)
     unsigned long long spin_limit;
     int hcache;             // output workgroups keep H_0[o] in LDS (after the FFT buffers and twiddles)
+    int window;             // ticks the host may release ahead (release window, 1 .. MXR_SLOTS - 1)
     int lpb;                // lanes per bin of level 0's p = 0 product: next power of two >= ni / 2
     unsigned *present;      // plain launch: residency stamps (see stamp_present), else null
     unsigned present_tag;
@@ -292,10 +298,12 @@ __device__ __forceinline__ void mxr_out_product(const MxrLevel &L0, int o, int n
 //           arrive at its cS group (workgroup index mod MXR_SG: the 256
 //           arrivals of a tick on one address serialise, 8 groups on their
 //           own lines measured 28.0 -> 23.5 us per tick at 32 x 32).
-// S is double-buffered by tick parity (tick f reads S[f & 1]; its sum writes
-// S[(f+1) & 1]), so the sum for the next tick runs while this tick's outputs
-// are formed and while the host's round trip and the next tick's input
-// transforms proceed: the delay-line sum is off the critical path.  (A grid
+// S has MXR_SLOTS slots (tick f reads slot f % MXR_SLOTS; its sum writes slot
+// (f+1) % MXR_SLOTS), so the sum for the next tick runs while this tick's
+// outputs are formed and while the host's round trip and the next tick's input
+// transforms proceed: the delay-line sum is off the critical path.  With a
+// release window the roles may be at different ticks: input workgroups first
+// wait for the sums that still read the row they overwrite.  (A grid
 // barrier per tick instead, with every workgroup summing after it, measured
 // 43 us per tick against 32 x 32 channels.)
 template <int LOGN>
@@ -327,21 +335,26 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
         }
         __syncthreads();
     }
+    // release word last read by thread 0 (a release window covers later ticks too)
+    unsigned long long g_seen = 0;
     for (long long t = 0; t < A.nticks; t++) {
         const uint64_t f = A.f0 + (uint64_t)t;
         // ---- release
         if (tid == 0) {
-            const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + A.spin_limit;
-            int go = 0;
-            for (;;) {
-                const unsigned long long g = __hip_atomic_load(A.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                if (g & PC_GO_STOP) break;
-                if (g >= f + 1) {
-                    go = 1;
-                    break;
+            int go = go_released(g_seen, f);
+            if (!go) {
+                const unsigned long long t_end = __builtin_amdgcn_s_memrealtime() + A.spin_limit;
+                for (;;) {
+                    const unsigned long long g = __hip_atomic_load(A.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (go_released(g, f)) {
+                        go = 1;
+                        g_seen = g;
+                        break;
+                    }
+                    if (g & PC_GO_STOP) break;
+                    if (__builtin_amdgcn_s_memrealtime() > t_end) break;
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                if (__builtin_amdgcn_s_memrealtime() > t_end) break;
-                __builtin_amdgcn_s_sleep(1);
             }
             s_ok = go;
         }
@@ -363,6 +376,16 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
             }
             __syncthreads();
             const cf *X = small_forward<LOGN, MXR_NTT>(bufA, bufB, twl, tid);
+            // With ticks released ahead the roles' workgroups may be at different
+            // ticks.  The counters count arrivals, so an input workgroup arrives for
+            // tick f only once every input has arrived for tick f - 1; and row f % P
+            // (tick f - P's) may still be read by the delay-line sums of tick f - 2
+            // (partition P - 1 of S_{f-1}): wait for them too.  One tick at a time,
+            // both hold already (tick f - 1's outputs waited for them).
+            if (A.window > 1 && t >= 1 &&
+                !mxr_wait2(cX, (unsigned long long)t * A.n_in, cS,
+                           (t >= 2) ? (unsigned long long)(t - 1) * (G4 / MXR_SG) : 0ull, A.spin_limit, tid, &s_ok))
+                return;
             cf *row = L0.fdl + (size_t)(f % (uint64_t)L0.P) * N0 * L0.ni + w;
             for (int k = tid; k < N0; k += MXR_NTT) row[(size_t)k * L0.ni] = X[k];
             mxr_arrive(cX, tid);
@@ -397,7 +420,7 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                 return;
             // S for this tick, read coalesced now; added after the product
             // (sk + a: the same sum as before, addition being commutative)
-            const cf *S = L0.S + (size_t)(f & 1) * A.n_out * N0 + (size_t)o * N0;
+            const cf *S = L0.S + (size_t)(f % MXR_SLOTS) * A.n_out * N0 + (size_t)o * N0;
             constexpr int JS = (N0 + MXR_NTT - 1) / MXR_NTT;
             cf sk[JS];
 #pragma unroll
@@ -444,8 +467,16 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                 const int k = w - n_io + j * G4;
                 if (k < N0) mxr_bin_acc<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 2, L0.P, acc[j], tid);
             }
-            if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) return;
-            cf *Sn = L0.S + (size_t)((f + 1) & 1) * A.n_out * N0;
+            // this tick's rows; with ticks released ahead also every sum of tick f - 1,
+            // so that no group counter counts an arrival for tick f before them
+            if (A.window > 1) {
+                if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * (G4 / MXR_SG),
+                               A.spin_limit, tid, &s_ok))
+                    return;
+            } else if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) {
+                return;
+            }
+            cf *Sn = L0.S + (size_t)((f + 1) % MXR_SLOTS) * A.n_out * N0;
 #pragma unroll
             for (int j = 0; j < KB; j++) {
                 const int k = w - n_io + j * G4;

@@ -141,6 +141,12 @@ class MimoRealtime:
             raise RuntimeError(last_error())
         return a.value, n.value, b.value
 
+    def set_release_window(self, ticks):
+        """Ticks run_device may release ahead of the oldest incomplete one
+        (pcxr_set_release_window: 1 .. 3, default 1); identical output."""
+        if lib().pcxr_set_release_window(self._h, int(ticks)) != 0:
+            raise RuntimeError(last_error())
+
     def stall_level(self, level, seconds):
         """Deadline-handling test hook (pcxr_stall_level): the next block of
         level `level` waits `seconds` behind a host function on its stream."""

@@ -151,3 +151,81 @@ def test_release_window_argument_bounds():
     assert m.set_release_window(0) != 0
     assert m.set_release_window(17) != 0
     assert m.set_release_window(16) == 0
+
+
+@pytest.mark.parametrize("n_in,n_out,taps,N,P,nfr", [
+    (5, 3, 10000, [128, 1024], [14, 0], 3 * 8 + 77),
+    (32, 32, 20000, [256, 4096], [30, 0], 3 * 16 + 7),
+    (4, 4, 40000, [64, 512, 4096], [14, 14, 0], 5 * 64 + 13),
+    (3, 2, 3000, [128, 256], [2, 0], 61),                      # P_0 = 2 caps the window at 2
+])
+def test_mimo_release_window_bitwise_equal(n_in, n_out, taps, N, P, nfr):
+    """Real-time MIMO engine with ticks released ahead (pcxr_set_release_window):
+    outputs bitwise those of one tick at a time, over split calls."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    from partconv_amd.workloads import cfg4_irs
+    N0 = N[0]
+    irs = cfg4_irs(seed=9, n_in=n_in, n_out=n_out, taps=taps)
+    h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n_out)]).astype(np.float32)).cuda()
+    x = torch.from_numpy(np.random.default_rng(n_in + 7 * n_out).standard_normal((nfr * N0, n_in))
+                         .astype(np.float32)).cuda()
+    outs = []
+    for W in (1, 2, 3):
+        say(f"case mimo window {n_in}x{n_out} N={N} W={W}")
+        rt = MimoRealtime(48000, N0, N, P, taps, n_in, n_out)
+        rt.set_filters(h)
+        rt.set_release_window(W)
+        y = torch.zeros((nfr * N0, n_out), dtype=torch.float32, device="cuda")
+        a = nfr // 3
+        rt.run_device(y[:a * N0], x[:a * N0], a)
+        rt.run_device(y[a * N0:], x[a * N0:], nfr - a)
+        torch.cuda.synchronize()
+        st = rt.stats()
+        assert st.ticks == nfr and st.stalls == 0
+        outs.append(y.cpu().numpy())
+    assert np.array_equal(outs[1], outs[0])
+    assert np.array_equal(outs[2], outs[0])
+
+
+def test_mimo_release_window_late_level():
+    """A held level block inside a MIMO release window: waited for, counted,
+    output bitwise that of an unheld run."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    n_in, n_out, taps, nfr, hold = 2, 3, 3000, 200, 0.3
+    g = torch.Generator(device="cuda").manual_seed(48)
+    h = torch.randn((n_out, n_in, taps), device="cuda", generator=g) * 0.05
+    x = torch.randn((nfr * 64, n_in), device="cuda", generator=g)
+
+    def mk(W):
+        rt = MimoRealtime(48000, 64, [64, 512], [14, 0], taps, n_in, n_out)
+        rt.set_filters(h)
+        rt.set_release_window(W)
+        return rt
+    ref = mk(1)
+    yr = torch.zeros((nfr * 64, n_out), device="cuda")
+    ref.run_device(yr, x, nfr)
+    say("case mimo window late level")
+    rt = mk(3)
+    y = torch.zeros_like(yr)
+    rt.run_device(y[:40 * 64], x[:40 * 64], 40)
+    rt.stall_level(1, hold)
+    rt.clear_stats()
+    rt.run_device(y[40 * 64:80 * 64], x[40 * 64:80 * 64], 40)
+    st = rt.stats()
+    say(f"  ticks {st.ticks}, misses {st.deadline_misses}, stalls {st.stalls}, max {st.max_us:.0f} us")
+    assert st.deadline_misses >= 1 and st.stalls >= 1 and st.max_us > 0.9 * hold * 1e6
+    rt.run_device(y[80 * 64:], x[80 * 64:], nfr - 80)
+    torch.cuda.synchronize()
+    assert torch.equal(y, yr)
+
+
+def test_mimo_release_window_bounds():
+    from partconv_amd.mimo import MimoRealtime
+    rt = MimoRealtime(48000, 64, [64, 512], [14, 0], 3000, 2, 2)
+    with pytest.raises(RuntimeError):
+        rt.set_release_window(0)
+    with pytest.raises(RuntimeError):
+        rt.set_release_window(4)
+    rt.set_release_window(3)
