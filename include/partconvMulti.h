@@ -131,6 +131,8 @@ public:
     {
         return report(pcxr_run_device(h_, d_out, d_in, nframes));
     }
+    /* ticks run_device may release ahead of the oldest incomplete one (1 .. 3) */
+    int set_release_window(int ticks) { return report(pcxr_set_release_window(h_, ticks)); }
     /* one tick on host buffers (interleaved, frameCount == buffer_size) */
     int run(float *const output, const float *const input, const int frameCount)
     {
