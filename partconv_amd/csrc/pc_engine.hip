@@ -2133,6 +2133,7 @@ static int persist_end(Engine &E, bool stop)
     // anything below waits on or relaunches levels
     const int lrc = launcher_drain(E);
     std::fill(E.lvl_async.begin(), E.lvl_async.end(), false);
+    int serr = PC_OK;
     if (stop) {
         // the ticks a release-window run released (f < win_rel) are served
         // before the kernel stops; otherwise it stops at once
@@ -2162,9 +2163,9 @@ static int persist_end(Engine &E, bool stop)
                 // exactly what a workgroup already did for tick E.frame but not
                 // what it did for a later tick: refuse that state.
                 const volatile unsigned long long *wf = E.h_wg;
-                for (int i = 0; i < E.nwg; i++)
+                for (int i = 0; i < E.nwg && !serr; i++)
                     if (wf[(size_t)i * PC_WG_FLAG_STRIDE] > E.frame + 1)
-                        return fail(PC_ERR_STATE, "persistent run ended with workgroups past the first unserved tick");
+                        serr = fail(PC_ERR_STATE, "persistent run ended with workgroups past the first unserved tick");
                 E.pend_t0 = E.win_trel[E.frame % W];
             }
         } else if (E.frame < E.sess_end && tick_done(E, E.frame + 1)) {
@@ -2186,7 +2187,7 @@ static int persist_end(Engine &E, bool stop)
         int rc = finish_worker_levels(E, E.frame);
         if (rc) return rc;
     }
-    return lrc;
+    return serr ? serr : lrc;
 }
 
 // Release window: the host-launched level blocks that tick r reads for the
