@@ -467,15 +467,14 @@ __global__ void __launch_bounds__(MXR_NTT) k_mxr_persist(MxrArgs A, const cf *__
                 const int k = w - n_io + j * G4;
                 if (k < N0) mxr_bin_acc<MXR_NTT, 4>(L0, A.n_in, A.n_out, k, f + 1, 2, L0.P, acc[j], tid);
             }
-            // this tick's rows; with ticks released ahead also every sum of tick f - 1,
-            // so that no group counter counts an arrival for tick f before them
-            if (A.window > 1) {
-                if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * (G4 / MXR_SG),
-                               A.spin_limit, tid, &s_ok))
-                    return;
-            } else if (!mxr_wait(cX, (unsigned long long)(t + 1) * A.n_in, A.spin_limit, tid, &s_ok)) {
+            // this tick's rows, and every sum of tick f - 1: tick f is released once
+            // tick f - 1's outputs are out, which needs only the sums of tick f - 2,
+            // so a sum workgroup may start tick f while another still works on tick
+            // f - 1; the group counters count arrivals, and an arrival for tick f
+            // before all of tick f - 1's would let tick f's outputs read S early
+            if (!mxr_wait2(cX, (unsigned long long)(t + 1) * A.n_in, cS, (unsigned long long)t * (G4 / MXR_SG),
+                           A.spin_limit, tid, &s_ok))
                 return;
-            }
             cf *Sn = L0.S + (size_t)((f + 1) % MXR_SLOTS) * A.n_out * N0;
 #pragma unroll
             for (int j = 0; j < KB; j++) {
