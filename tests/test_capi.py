@@ -48,6 +48,12 @@ def test_errors_are_returned_not_fatal():
     # missing config file -> PC_ERR_CONFIG
     assert L.pcm_setup(h, b"/nonexistent/pc.config", 1, 0, 0.0) == 2
     assert "Unable to open config file" in _lib.last_error()
+    # release window: 1 .. 16 ticks
+    assert L.pcm_set_release_window(h, 0) == 1 and "release window" in _lib.last_error()
+    assert L.pcm_set_release_window(h, 17) == 1
+    assert L.pcm_set_release_window(h, 16) == 0
+    assert L.pcm_set_release_window(None, 2) == 1
+    assert L.pcxr_set_release_window(None, 2) == 1
     L.pcm_destroy(h)
 
 
