@@ -34,7 +34,6 @@ TAPS = 262144
 PART_N = [64, 256, 2048, 16384]
 PART_P = [6, 14, 14, 0]
 TICKS_PER_STEP = 256
-MIMO_WINDOW = 2  # release window of the real-time MIMO line (measured: 2 ahead of 1 and 3)
 
 
 def build_engine(mode, irs):
@@ -447,11 +446,13 @@ def bench_mimo(args):
     lst = m.stats()
     m.cleanup()
     # the real-time MIMO engine on the same frames: warmup and timed steps one call
-    # each, with a release window of MIMO_WINDOW ticks; then one tick at a time
+    # each, with the release window the tuner chose (before the timed region);
+    # then one tick at a time
+    from partconv_amd import autotune
     h = torch.from_numpy(np.stack([np.stack(irs[o]) for o in range(n)]).astype(np.float32)).cuda()
     rt = MimoRealtime(FS, n0, [256, 4096], [30, 0], taps, n, n)
     rt.set_filters(h)
-    rt.set_release_window(MIMO_WINDOW)
+    mtune = autotune.tune_mimo(rt)
     yr = torch.zeros_like(xd)
     roff = args.warmup * tps * n0  # rows (frames x buffer size) of the warmup ticks
     if args.warmup:
@@ -549,7 +550,9 @@ def bench_mimo(args):
         "data": "synthetic (seeded white noise, seeded decaying-noise IRs)",
         "config": {"workload": "MIMO 32x32, 65536-tap IRs, [256x30, 4096x15], buffer 256 at 48 kHz; real-time "
                                "MIMO engine (shared input transforms and delay lines, per-bin matrix-vector "
-                               "products; level 0 in one persistent kernel)", "release_window": MIMO_WINDOW},
+                               "products; level 0 in one persistent kernel)",
+                   "release_window": mtune["release_window"]},
+        "tuning": {k: mtune[k] for k in ("release_window", "from_cache", "measured") if k in mtune},
         "latency_us": {"p50": rst.p50_us, "p99": rst.p99_us, "max": rst.max_us, "period": rst.period_us,
                        "misses": int(rst.deadline_misses), "ticks": int(rst.ticks)},
         "one_tick_released_at_a_time": one_at_a_time,

@@ -16,6 +16,8 @@ cache keyed by (device, workload shape):
     then the release window of persistent device-resident runs
     (pcm_set_release_window: 1 to 4 ticks released ahead) -- chosen by the wall
     time per tick, so the smallest window within the tie of the fastest wins;
+  * real-time MIMO engine (MimoRealtime): its release window
+    (pcxr_set_release_window: 1 to 3 ticks), chosen by the wall time per tick;
   * differentiable form (PartitionedConv): each level's chunk sizes of the
     spectral products and of dL/dh (pcc_tune), replayed from the cache with
     pcc_set_level_tuning.
@@ -27,6 +29,7 @@ same in every later run.  None of the choices changes an engine's output.
 
     from partconv_amd import autotune
     info = autotune.tune_engine(m)                  # PartConvMulti after setup
+    info = autotune.tune_mimo(rt)                   # MimoRealtime after set_filters
     info = autotune.tune_offline(conv, x, h)        # PartitionedConv + clips
 """
 from __future__ import annotations
@@ -193,6 +196,58 @@ def tune_engine(m, nticks=None, cache=None, force=False):
     m.clear_stats()
     ent = {"workers": workers, "bins_per_thread": width, "release_window": window, "measured": measured,
            "ticks": [nticks, nwide], "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    store_entry(path, key, ent)
+    return dict(ent, from_cache=False, key=key)
+
+
+MIMO_WINDOWS = (1, 2, 3)  # default first
+
+
+def mimo_key(rt, device=None):
+    N, P = rt.partition()
+    parts = ",".join(f"{a}x{b}" for a, b in zip(N, P))
+    return (f"mimo|{device or device_name()}|N0={rt.buffer_size}|in={rt.n_in}|out={rt.n_out}"
+            f"|taps={rt.taps}|parts={parts}")
+
+
+def tune_mimo(rt, nticks=None, cache=None, force=False):
+    """Choose the release window of a MimoRealtime with its filters set, on
+    this device: the wall time per tick of device-resident runs of zero input
+    for each window (the engine reset before each timed call, and after);
+    apply it and return {"release_window": w, "from_cache": bool, "key": ...,
+    "measured": {...}}."""
+    import torch
+    path = default_cache_path() if cache is None else pathlib.Path(cache)
+    key = mimo_key(rt)
+    ent = load_cache(path)["entries"].get(key)
+    if ent and not force:
+        rt.set_release_window(ent["release_window"])
+        return dict(ent, from_cache=True, key=key)
+    N, _ = rt.partition()
+    n0 = rt.buffer_size
+    if nticks is None:  # two blocks of the longest level, at least 128 ticks
+        nticks = max(128, 2 * (max(N) // n0))
+    x = torch.zeros((nticks * n0, rt.n_in), dtype=torch.float32, device="cuda")
+    y = torch.zeros((nticks * n0, rt.n_out), dtype=torch.float32, device="cuda")
+    costs, measured = [], {}
+    for w in MIMO_WINDOWS:
+        rt.set_release_window(w)
+        rt.reset()
+        rt.run_device(y, x, nticks)              # warm-up
+        rt.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt.run_device(y, x, nticks)
+        torch.cuda.synchronize()
+        us = 1e6 * (time.perf_counter() - t0) / nticks
+        costs.append(us)
+        measured[str(w)] = us
+    window = choose(list(MIMO_WINDOWS), costs, MIMO_WINDOWS[0])
+    rt.set_release_window(window)
+    rt.reset()
+    rt.clear_stats()
+    ent = {"release_window": window, "measured": {"window_us_per_tick": measured}, "ticks": nticks,
+           "when": time.strftime("%Y-%m-%dT%H:%M:%S")}
     store_entry(path, key, ent)
     return dict(ent, from_cache=False, key=key)
 

@@ -85,3 +85,33 @@ def test_offline_tuning_cached(tmp_path):
     # the same choice: forward and both gradients bitwise equal
     for u, v in zip(*outs):
         assert torch.equal(u, v)
+
+
+def test_mimo_window_tuning_cached_and_output_unchanged(tmp_path):
+    """The real-time MIMO engine's release window: measured on a cache miss,
+    replayed from the cache, output bitwise that of one tick at a time."""
+    import torch
+    from partconv_amd.mimo import MimoRealtime
+    cache = tmp_path / "tuning.json"
+    g = torch.Generator(device="cuda").manual_seed(71)
+    n_in, n_out, taps, nfr = 4, 3, 6000, 120
+    h = torch.randn((n_out, n_in, taps), device="cuda", generator=g) * 0.05
+    x = torch.randn((nfr * 128, n_in), device="cuda", generator=g)
+
+    def mk():
+        rt = MimoRealtime(48000, 128, [128, 1024], [14, 0], taps, n_in, n_out)
+        rt.set_filters(h)
+        return rt
+    ref = mk()
+    yr = torch.zeros((nfr * 128, n_out), device="cuda")
+    ref.run_device(yr, x, nfr)
+    rt = mk()
+    info = autotune.tune_mimo(rt, nticks=96, cache=cache)
+    print("mimo window:", info["release_window"], info["measured"], flush=True)
+    assert not info["from_cache"] and info["release_window"] in autotune.MIMO_WINDOWS
+    y = torch.zeros_like(yr)
+    rt.run_device(y, x, nfr)
+    torch.cuda.synchronize()
+    assert torch.equal(y, yr)
+    info2 = autotune.tune_mimo(mk(), cache=cache)
+    assert info2["from_cache"] and info2["release_window"] == info["release_window"]
