@@ -862,10 +862,47 @@ __device__ __forceinline__ void tick64_load_state(const LevelJob &J, uint64_t f,
     }
 }
 
+// Level-block samples of the first four levels for one tick, read ahead of
+// the tick (bit c of mask: level c's pair is in v[c]).
+struct Lev64Pre {
+    float2 v[4];
+    unsigned mask;
+};
+
+// Reads ahead the level-block samples tick f will add, for the levels whose
+// block tick f does not read first: that block was complete, and acquired,
+// when an earlier tick of this workgroup first read it, and the half of the
+// level's double buffer it sits in is not written again before every tick
+// that reads it has run.  So the values are those tick f would load itself.
+// A persistent tick workgroup issues these loads before it waits for tick f's
+// release, where they overlap the host's round trip.
+__device__ __forceinline__ Lev64Pre tick64_prefetch(const TickJob &TJ, const TickArgs &A, uint64_t f, int lane)
+{
+    Lev64Pre p;
+    p.mask = 0;
+    const uint64_t fr = A.relaxed ? f : f + 1;
+    const int lo = lane & 31;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        p.v[c] = make_float2(0.f, 0.f);
+        if (c >= TJ.nlev) continue;
+        const int ks = TJ.kshift[c];
+        const uint64_t qb = fr >> ks, sub = fr & ((1ull << ks) - 1);
+        if (qb == 0 || sub == 0) continue;
+        if (lane < 32)
+            p.v[c] = gload2(reinterpret_cast<const float2 *>(TJ.ybuf[c] + (size_t)((qb - 1) & 1) * TJ.NL[c] +
+                                                             sub * 64) + lo);
+        p.mask |= 1u << c;
+    }
+    return p;
+}
+
 // Phase A for buffer_size 64; leaves the instance's output sample `lane` in
-// *acc (through LDS) and X_f of bin st.w.k in *xr.
+// *acc (through LDS) and X_f of bin st.w.k in *xr.  Levels in pre->mask take
+// their samples from *pre (tick64_prefetch) instead of loading them.
 __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs &A, uint64_t f, float *obuf, int lane,
-                                              TickState64 &st, float *acc, cf *xr, const float *llx = nullptr)
+                                              TickState64 &st, float *acc, cf *xr, const float *llx = nullptr,
+                                              const Lev64Pre *pre = nullptr)
 {
     const LevelJob &J = TJ.l0;
     const long long mask = J.ring_mask;
@@ -889,12 +926,15 @@ __device__ __forceinline__ void tick64_phaseA(const TickJob &TJ, const TickArgs 
         // level blocks four at a time: loads in flight together, summed in level order
         for (int L0 = 0; L0 < TJ.nlev; L0 += 4) {
             float2 v[4];
+            const unsigned pm = (pre && L0 == 0) ? pre->mask : 0u;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const int L = L0 + c;
                 const uint64_t qb = (L < TJ.nlev) ? (fr >> TJ.kshift[L]) : 0;
                 v[c] = make_float2(0.f, 0.f);
-                if (qb != 0) {
+                if ((pm >> c) & 1u) {
+                    v[c] = pre->v[c];
+                } else if (qb != 0) {
                     const uint64_t sub = fr & ((1ull << TJ.kshift[L]) - 1);
                     v[c] = gload2(reinterpret_cast<const float2 *>(TJ.ybuf[L] + (size_t)((qb - 1) & 1) * TJ.NL[L] +
                                                                    sub * 64) + lo);
@@ -1278,6 +1318,7 @@ struct PersistArgs {
     unsigned present_tag;          // into present[w] (host-visible) on entry; null otherwise
     int tw_lds;                    // twiddle-table entries tw[0 .. tw_lds) copied into LDS at entry (0: none)
     int tw_lds_off;                // their offset (float2) in the dynamic LDS, after the block buffers
+    int prefetch;                  // 64-point ticks: read the next tick's level samples before its release
 };
 
 // Residency stamp of a persistent kernel launched without the cooperative
@@ -1666,6 +1707,9 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     // release word last read by thread 0: when the host releases ticks ahead
     // (a release window), the ticks it already covers need no new read
     unsigned long long g_seen = 0;
+    // level samples read ahead for the next tick (64-point ticks)
+    Lev64Pre pre;
+    pre.mask = 0;
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
         float llx[LLCnt<LOGN>::v];
@@ -1730,7 +1774,8 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         if constexpr (LOGN == 6) {
             cf xr;
             tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr,
-                          P.A.ll ? llx : nullptr);
+                          P.A.ll ? llx : nullptr, &pre);
+            pre.mask = 0;
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
             tick64_phaseB(sTJ, f, xr, st64);
@@ -1771,6 +1816,13 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         }
         if (tr && tslot == 0) trace_stamp(P.A, f, 7);
         __syncthreads();
+        if constexpr (LOGN == 6) {
+            // the next tick is not released yet as far as this workgroup knows:
+            // read its level samples now, while the release makes its round trip
+            // (thread 0 holds the release word; the workgroup is one wavefront)
+            if (P.prefetch && t + 1 < P.nticks && __shfl((int)!go_released(g_seen, f + 1), 0))
+                pre = tick64_prefetch(sTJ, P.A, f + 1, tid);
+        }
     }
 }
 
