@@ -1776,7 +1776,8 @@ static hipError_t launch_persist_t(Engine &E, PersistArgs &pa, hipStream_t s, in
 {
     const cf *t = E.tw;
     void *args[] = {(void *)&pa, (void *)&t};
-    return launch_persistent((const void *)k_persist<LOGN>, nblocks, TickCfg<LOGN>::NT, args, lds, s, E.resid, E.ncu,
+    const void *k = (LOGN == 6 && pa.prefetch) ? (const void *)k_persist<LOGN, true> : (const void *)k_persist<LOGN>;
+    return launch_persistent(k, nblocks, TickCfg<LOGN>::NT, args, lds, s, E.resid, E.ncu,
                              &pa.present, &pa.present_tag, plain);
 }
 
@@ -1801,6 +1802,10 @@ static hipError_t launch_persist(Engine &E, PersistArgs &p, hipStream_t s, int l
 template <int LOGN>
 static hipError_t persist_occupancy_t(int *nb, size_t lds)
 {
+    // the read-ahead form of the 64-point kernel holds more registers: the
+    // residency check counts with it
+    if constexpr (LOGN == 6)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_persist<LOGN, true>, TickCfg<LOGN>::NT, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_persist<LOGN>, TickCfg<LOGN>::NT, lds);
 }
 
@@ -2023,11 +2028,12 @@ static int persist_begin(Engine &E, const float *d_in, long long in_stride, floa
     P.nticks = nticks;
     P.go = E.go_d;
     P.spin_limit = PC_SPIN_LIMIT;
-    // Reading a tick's level samples ahead of its release pays when every tick
-    // waits for its own release (measured at 16 channels x 262144 taps: 5.05 ->
-    // 4.83 us p50); inside a release window the next tick is mostly released
-    // already and the extra loads delay the release read (3.74 -> 3.97 us per
-    // tick at a window of 3), so it is left out there.
+    // Reading a tick's level samples and worker counter ahead of its release
+    // pays when every tick waits for its own release (measured at 16 channels x
+    // 262144 taps: p50 5.05 -> 4.6 us); inside a release window the next tick
+    // is mostly released already and the extra loads delay the release read
+    // (3.74 -> 3.97 us per tick at a window of 3), so windowed runs launch the
+    // plain kernel.
     P.prefetch = (E.release_window <= 1 || !E.use_wgflags || ll) ? 1 : 0;
     // the levels the first tick reads must be complete before the kernel owns the tick stream
     const int G = (int)E.gsize.size();

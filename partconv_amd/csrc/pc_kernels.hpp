@@ -1318,7 +1318,7 @@ struct PersistArgs {
     unsigned present_tag;          // into present[w] (host-visible) on entry; null otherwise
     int tw_lds;                    // twiddle-table entries tw[0 .. tw_lds) copied into LDS at entry (0: none)
     int tw_lds_off;                // their offset (float2) in the dynamic LDS, after the block buffers
-    int prefetch;                  // 64-point ticks: read the next tick's level samples before its release
+    int prefetch;                  // 64-point ticks: launch k_persist<6, true> (reads ahead of each release)
 };
 
 // Residency stamp of a persistent kernel launched without the cooperative
@@ -1659,7 +1659,11 @@ __device__ __forceinline__ int ll_wait_frame(const TickJob &TJ, const TickArgs &
     return !stop;
 }
 
-template <int LOGN>
+// AHEAD (64-point ticks, one tick released at a time): before it waits for a
+// tick's release, each tick workgroup reads that tick's level samples and the
+// worker-level counter it will wait on (a separate instantiation, so the
+// windowed runs keep the plain kernel's code and registers)
+template <int LOGN, bool AHEAD = false>
 __global__ void __launch_bounds__(TickCfg<LOGN>::NT, TickCfg<LOGN>::MINW)
 k_persist(PersistArgs P, const cf *__restrict__ tw)
 {
@@ -1710,6 +1714,11 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
     // level samples read ahead for the next tick (64-point ticks)
     Lev64Pre pre;
     pre.mask = 0;
+    // thread 0, one tick at a time: the worker-level counter lvl_done[ld_slot]
+    // as read before this tick's release, for the block this tick reads first
+    // (-1: none read)
+    unsigned long long ld_pre = 0;
+    int ld_slot = -1;
     for (long long t = 0; t < P.nticks; t++) {
         const uint64_t f = P.f0 + (uint64_t)t;
         float llx[LLCnt<LOGN>::v];
@@ -1752,9 +1761,14 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
                 const int ks = sTJ.kshift[L];
                 if ((fr & ((1ull << ks) - 1)) != 0 || (fr >> ks) == 0) continue;
                 const unsigned long long need = 2 * ((fr >> ks) - 1) + 1;
-                go = spin_geq(P.lvl_done + slot, need, P.spin_limit, 0);
                 waited = true;
+                // the counter only grows: a value read before the release that
+                // already covers the block ends the wait (the acquire below
+                // orders this tick's loads after that read)
+                if (AHEAD && slot == ld_slot && !(ld_pre & PC_GO_STOP) && ld_pre >= need) continue;
+                go = spin_geq(P.lvl_done + slot, need, P.spin_limit, 0);
             }
+            ld_slot = -1;
             // level blocks written by other workgroups or kernels (other XCDs) are
             // first read at deadline ticks: acquire at device scope there
             if (go && (waited || (fr & P.acq_mask) == 0)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1774,7 +1788,7 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         if constexpr (LOGN == 6) {
             cf xr;
             tick64_phaseA(sTJ, P.A, f, reinterpret_cast<float *>(bufA), tid, st64, &acc[0], &xr,
-                          P.A.ll ? llx : nullptr, &pre);
+                          P.A.ll ? llx : nullptr, AHEAD ? &pre : nullptr);
             pre.mask = 0;
             tick_arrive<LOGN>(sTJ, P.A, f, tid, acc, &s_last);
             if (tr) trace_stamp(P.A, f, tslot + 1);
@@ -1816,12 +1830,25 @@ k_persist(PersistArgs P, const cf *__restrict__ tw)
         }
         if (tr && tslot == 0) trace_stamp(P.A, f, 7);
         __syncthreads();
-        if constexpr (LOGN == 6) {
+        if constexpr (LOGN == 6 && AHEAD) {
             // the next tick is not released yet as far as this workgroup knows:
             // read its level samples now, while the release makes its round trip
             // (thread 0 holds the release word; the workgroup is one wavefront)
-            if (P.prefetch && t + 1 < P.nticks && __shfl((int)!go_released(g_seen, f + 1), 0))
+            if (t + 1 < P.nticks && __shfl((int)!go_released(g_seen, f + 1), 0)) {
                 pre = tick64_prefetch(sTJ, P.A, f + 1, tid);
+                // so is the counter of a worker-level block the next tick reads
+                // first (a device-scope read of a line another XCD writes)
+                const uint64_t fr1 = (P.A.relaxed ? f : f + 1) + 1;
+                for (int L = 0; tid == 0 && P.nworkers > 0 && L < sTJ.nlev; L++) {
+                    const int slot = sTJ.wslot[L];
+                    if (slot < 0) continue;
+                    const int ks = sTJ.kshift[L];
+                    if ((fr1 & ((1ull << ks) - 1)) != 0 || (fr1 >> ks) == 0) continue;
+                    ld_pre = __hip_atomic_load(P.lvl_done + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ld_slot = slot;
+                    break;
+                }
+            }
         }
     }
 }
